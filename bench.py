@@ -1,0 +1,298 @@
+#!/usr/bin/env python3
+"""Benchmark of the data-parallel primitive core on one MI355X.
+
+Headline (the `value`): radix sort of 2^27 uint32 keys (uniform over 32 bits,
+mt19937 seed 0xB0057001), ascending, through the C-ABI, inputs resident in
+HBM.  One step = one sort of fresh (unsorted) keys: an untimed device-to-device
+copy restores the input, then the sort is timed with HIP events on the
+queue's stream.  The same JSON line carries:
+  * roofline: the dominant kernel (one onesweep pass) -- algorithmic bytes
+    per launch (8 B per key: read + write) over its average launch duration,
+    measured with HIP events around each launch on the queue's stream;
+    traffic from the committed rocprofv3 PMC summary when present;
+  * cpu_baseline: the oracle's restatement of the reference's CPU-device sort
+    (merge_sort_on_cpu) timed on a bounded sample on this host;
+  * secondary: scan / reduce / saxpy / sort_by_key / reduce_by_key /
+    segmented scan / segmented sort at their configured sizes, each with GB/s
+    and fraction of the 8 TB/s HBM peak, plus the float error of the
+    compensated reduce against FP64.
+
+Usage: python bench.py [--gpus 1] [--steps K] [--warmup W] [--only a,b,..]
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+PEAK_GBS = 8000.0  # MI355X HBM3E
+SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "segsort"]
+
+
+def timed_steps(ca, q, fn, steps, warmup, restore=None):
+    for _ in range(warmup):
+        if restore:
+            restore()
+        fn()
+    q.finish()
+    evs = [(ca.Event(), ca.Event()) for _ in range(steps)]
+    t0 = time.perf_counter()
+    for s, e in evs:
+        if restore:
+            restore()
+        s.record(q)
+        fn()
+        e.record(q)
+    q.finish()
+    wall = time.perf_counter() - t0
+    ms = [s.elapsed_ms(e) for s, e in evs]
+    return ms, wall
+
+
+def kernel_avg_ms(q, fn, kernel, reps=3, restore=None):
+    q.profile_reset()
+    q.profile(True)
+    for _ in range(reps):
+        if restore:
+            restore()
+        fn()
+    q.profile(False)
+    n, tot = q.profile_query(kernel)
+    q.profile_reset()
+    return (tot / n if n else float("nan")), n // reps if reps else n
+
+
+def d2d(ca, q, dst, src):
+    ca.check(ca.lib().bcmi_memcpy(q.h, dst.ptr, src.ptr, src.nbytes, ca.D2D, 0))
+
+
+def gbs(nbytes, ms):
+    return nbytes / (ms * 1e-3) / 1e9
+
+
+def load_traffic(name):
+    path = os.path.join(ROOT, "profiles", name)
+    try:
+        with open(path) as f:
+            return json.load(f)
+    except (OSError, ValueError):
+        return None
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--only", default="all")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
+              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        sys.exit(2)
+    only = SECTIONS if args.only == "all" else args.only.split(",")
+
+    import compute_amd as ca
+    import compute_amd.workloads as W
+
+    q = ca.Queue(0)
+    info = ca.device_info(0)
+    K, Wm = args.steps, args.warmup
+    sec = {}
+    out = None
+
+    # ---------------------------------------------------- headline: sort u32
+    if "sort" in only:
+        n = 1 << 27
+        keys = W.sort_u32(n)
+        src = ca.DeviceArray.from_host(keys, q)
+        work = ca.DeviceArray(n, np.uint32, q)
+        restore = lambda: d2d(ca, q, work, src)
+        fn = lambda: ca.sort(q, work)
+        ms, wall = timed_steps(ca, q, fn, K, Wm, restore)
+        got = work.to_host(q)
+        assert np.all(got[1:] >= got[:-1]), "sort output not sorted"
+        ms_step = float(np.mean(ms))
+        os_ms, launches = kernel_avg_ms(q, fn, "radix_onesweep", restore=restore)
+        h_ms, _ = kernel_avg_ms(q, fn, "radix_histogram", restore=restore)
+        per_launch = 8 * n  # one pass: read 4 B + write 4 B per key
+        ach = gbs(per_launch, os_ms)
+        tr = load_traffic("radix_onesweep_u32_2p27_traffic.json")
+        out = {
+            "metric": "sort Gkeys/s & scan/reduce GB/s (%HBM peak) at 2^27; batched vs per-call",
+            "value": round(n / (ms_step * 1e-3) / 1e9, 3),
+            "unit": "Gkeys/s",
+            "n_gpus": 1, "steps": K, "warmup": Wm,
+            "ms_per_step": round(ms_step, 4),
+            "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+            "dtype": "u32",
+            "data": "synthetic: 2^27 uint32 keys = raw mt19937(0xB0057001) output",
+            "config": {"workload": "radix sort of 2^27 uint32 keys, uniform over 32 bits, ascending "
+                                   "(boost::compute::sort / detail::radix_sort)",
+                       "n": n, "device": info["name"], "arch": info["arch"],
+                       "timing": "hipEvent around each sort call on the queue stream; input restored "
+                                 "by an untimed D2D copy before each step",
+                       "wall_ms_per_step_incl_restore": round(wall * 1e3 / K, 4),
+                       "alg_bytes_per_key": 36,
+                       "sort_GBps_alg": round(gbs(36 * n, ms_step), 1),
+                       "sort_frac_of_hbm": round(gbs(36 * n, ms_step) / PEAK_GBS, 4),
+                       "kernels_ms": {"radix_histogram": round(h_ms, 4), "radix_onesweep_avg": round(os_ms, 4),
+                                      "onesweep_launches_per_sort": launches}},
+            "roofline": {"bound": "hbm", "kernel": "radix_onesweep (one 8-bit digit pass)",
+                         "achieved": round(ach, 1), "peak": PEAK_GBS, "unit": "GB/s",
+                         "frac": round(ach / PEAK_GBS, 4),
+                         "alg_bytes_per_launch": per_launch,
+                         "traffic": tr.get("bytes_per_launch") if tr else None},
+        }
+        del src, work, keys, got
+
+    # ------------------------------------------------------- sort_by_key f32
+    if "sort_by_key" in only:
+        n = 1 << 27
+        k, v = W.sort_by_key_f32(n)
+        sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+        wk, wv = ca.DeviceArray(n, np.float32, q), ca.DeviceArray(n, np.uint32, q)
+
+        def restore():
+            d2d(ca, q, wk, sk)
+            d2d(ca, q, wv, sv)
+
+        fn = lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY)
+        ms, _ = timed_steps(ca, q, fn, max(2, K // 2), 1, restore)
+        t = float(np.mean(ms))
+        os_ms, _ = kernel_avg_ms(q, fn, "radix_onesweep", restore=restore)
+        sec["sort_by_key_f32_u32_2p27"] = {
+            "Gpairs_per_s": round(n / (t * 1e-3) / 1e9, 3), "ms": round(t, 4),
+            "GBps_alg_68B": round(gbs(68 * n, t), 1), "frac": round(gbs(68 * n, t) / PEAK_GBS, 4),
+            "onesweep_pass_GBps": round(gbs(16 * n, os_ms), 1)}
+        del sk, sv, wk, wv, k, v
+
+    # ------------------------------------------------------ exclusive scan
+    if "scan" in only:
+        n = 1 << 28
+        x = W.scan_i32(n)
+        dx = ca.DeviceArray.from_host(x, q)
+        do = ca.DeviceArray(n, np.int32, q)
+        fn = lambda: ca.scan(q, dx, do, True, init=0)
+        ms, _ = timed_steps(ca, q, fn, K, Wm)
+        t = float(np.mean(ms))
+        kms, _ = kernel_avg_ms(q, fn, "scan")
+        sec["exclusive_scan_i32_2p28"] = {"GBps": round(gbs(8 * n, t), 1), "ms": round(t, 4),
+                                          "frac": round(gbs(8 * n, t) / PEAK_GBS, 4),
+                                          "kernel_ms": round(kms, 4)}
+        del dx, do, x
+
+    # --------------------------------------------- reduce / inclusive scan
+    if "reduce" in only:
+        n = 1 << 28
+        x = W.lognormal_f32(n)
+        dx = ca.DeviceArray.from_host(x, q)
+        r = ca.DeviceArray(1, np.float32, q)
+        fn = lambda: ca.reduce(q, dx, r)
+        ms, _ = timed_steps(ca, q, fn, K, Wm)
+        t = float(np.mean(ms))
+        got = float(r.to_host(q)[0])
+        ca.reduce(q, dx, r, precision=ca.NAIVE)
+        naive = float(r.to_host(q)[0])
+        ref = float(np.sum(x, dtype=np.float64))  # pairwise FP64: error << 1 float ulp here
+        ulp = float(np.spacing(np.float32(abs(ref))))
+        do = ca.DeviceArray(n, np.float32, q)
+        fs = lambda: ca.scan(q, dx, do, False)
+        ms2, _ = timed_steps(ca, q, fs, max(2, K // 2), 1)
+        t2 = float(np.mean(ms2))
+        sec["reduce_f32_lognormal_2p28"] = {
+            "GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4), "frac": round(gbs(4 * n, t) / PEAK_GBS, 4),
+            "compensated_err_ulp_vs_fp64": round(abs(got - ref) / ulp, 3),
+            "naive_fp32_err_ulp_vs_fp64": round(abs(naive - ref) / ulp, 3),
+            "inclusive_scan_GBps": round(gbs(8 * n, t2), 1), "inclusive_scan_ms": round(t2, 4),
+            "inclusive_scan_frac": round(gbs(8 * n, t2) / PEAK_GBS, 4)}
+        del dx, do, x
+
+    # ---------------------------------------------------------------- saxpy
+    if "saxpy" in only:
+        n = 1 << 28
+        x, y = W.saxpy_f32(n)
+        dx, dy = ca.DeviceArray.from_host(x, q), ca.DeviceArray.from_host(y, q)
+        do = ca.DeviceArray(n, np.float32, q)
+        fn = lambda: ca.saxpy(q, 2.5, dx, dy, do)
+        ms, _ = timed_steps(ca, q, fn, K, Wm)
+        t = float(np.mean(ms))
+        sec["saxpy_f32_2p28"] = {"GBps": round(gbs(12 * n, t), 1), "ms": round(t, 4),
+                                 "frac": round(gbs(12 * n, t) / PEAK_GBS, 4)}
+        del dx, dy, do, x, y
+
+    # -------------------------------------------- reduce_by_key / seg scan
+    if "rbk" in only or "segscan" in only:
+        n = 1 << 27
+        k, v = W.runs_i32_f32(n)
+        dk, dv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+        ko, vo = ca.DeviceArray(n, np.int32, q), ca.DeviceArray(n, np.float32, q)
+        cnt = ca.DeviceArray(1, np.uint64, q)
+        fn = lambda: ca.check(ca.lib().bcmi_reduce_by_key(q.h, ca.I32, ca.F32, ca.PLUS, dk.ptr, dv.ptr, n,
+                                                          ko.ptr, vo.ptr, cnt.ptr, ca.COMPENSATED))
+        ms, _ = timed_steps(ca, q, fn, K, Wm)
+        t = float(np.mean(ms))
+        runs = int(cnt.to_host(q)[0])
+        alg = 8 * n + 8 * runs
+        sec["reduce_by_key_i32_f32_2p27"] = {"GBps": round(gbs(alg, t), 1), "ms": round(t, 4),
+                                             "frac": round(gbs(alg, t) / PEAK_GBS, 4), "runs": runs}
+        fs = lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0)
+        ms, _ = timed_steps(ca, q, fs, K, Wm)
+        t = float(np.mean(ms))
+        sec["segmented_exclusive_scan_2p27"] = {"GBps": round(gbs(12 * n, t), 1), "ms": round(t, 4),
+                                                "frac": round(gbs(12 * n, t) / PEAK_GBS, 4)}
+        del dk, dv, ko, vo, k, v
+
+    # ------------------------------------------------------ segmented sort
+    if "segsort" in only:
+        try:
+            off = W.segments(65536)
+            n = int(off[-1])
+            k, v = W.segment_data(n)
+            sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+            wk, wv = ca.DeviceArray(n, np.uint64, q), ca.DeviceArray(n, np.uint32, q)
+            doff = ca.DeviceArray.from_host(off, q)
+
+            def restore():
+                d2d(ca, q, wk, sk)
+                d2d(ca, q, wv, sv)
+
+            fn = lambda: ca.segmented_sort(q, wk, wv, doff, off)
+            ms, _ = timed_steps(ca, q, fn, max(2, K // 2), 1, restore)
+            t = float(np.mean(ms))
+            sec["segmented_sort_u64_u32_65536seg"] = {"pairs": n, "ms": round(t, 4),
+                                                      "Gpairs_per_s": round(n / (t * 1e-3) / 1e9, 3),
+                                                      "GBps_alg_24B": round(gbs(24 * n, t), 1)}
+        except ca.BcmiError as e:
+            sec["segmented_sort_u64_u32_65536seg"] = {"unavailable": str(e)}
+
+    if out is None:
+        out = {"metric": "sort Gkeys/s & scan/reduce GB/s (%HBM peak) at 2^27; batched vs per-call",
+               "value": None, "unit": "Gkeys/s", "n_gpus": 1, "steps": K, "warmup": Wm,
+               "ms_per_step": None, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+               "dtype": "u32", "data": "synthetic", "config": {"workload": "partial run: " + args.only}}
+    out["secondary"] = sec
+
+    # ---------------------------------------------------------- CPU baseline
+    if not args.no_cpu_baseline and "sort" in only:
+        import oracle as O
+        m = 1 << 24
+        sample = O.gen_sort_u32(m)
+        t0 = time.perf_counter()
+        O.compare_stable_sort(sample)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": round(m / dt / 1e9, 5), "unit": "Gkeys/s", "cores": 1, "kind": "port",
+                               "sample": "first 2^24 of the same mt19937(0xB0057001) keys, sorted by the "
+                                         "oracle's restatement of merge_sort_on_cpu (the reference's "
+                                         "CPU-device sort path), one thread; %.2f s" % dt}
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,339 @@
+"""compute_amd -- MI355X-native data-parallel primitive core (Boost.Compute's
+sort / scan / reduce / reduce_by_key / transform hot path).
+
+The product is the C-ABI library ``libbcmi.so`` (declared in
+``include/bcmi.h``) with hand-written gfx950 HIP kernels, and the header-only
+C++ templates under ``include/boost/compute/`` that keep the reference's
+signatures.  This module is a thin ctypes binding of that C-ABI for Python
+callers, tests and the benchmark.  It never falls back to a CPU path: if the
+library is missing or no gfx950 device is visible, calls raise.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbcmi.so")
+INCLUDE_DIR = os.path.join(os.path.dirname(_HERE), "include")
+
+# enums from include/bcmi.h
+I8, U8, I16, U16, I32, U32, I64, U64, F32, F64 = range(1, 11)
+PLUS, MULTIPLIES, MIN, MAX, MINUS, DIVIDES = range(6)
+SORT, SORT_BY_KEY, STABLE_SORT, STABLE_SORT_BY_KEY, RADIX_SORT = range(5)
+COMPENSATED, NAIVE = 0, 1
+H2D, D2H, D2D = 1, 2, 3
+
+DTYPE = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8, np.dtype(np.int16): I16,
+         np.dtype(np.uint16): U16, np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
+         np.dtype(np.int64): I64, np.dtype(np.uint64): U64, np.dtype(np.float32): F32,
+         np.dtype(np.float64): F64}
+
+EXPORTS = [
+    "bcmi_last_error", "bcmi_get_version", "bcmi_device_count", "bcmi_device_get_info",
+    "bcmi_queue_create", "bcmi_queue_destroy", "bcmi_queue_finish", "bcmi_queue_device",
+    "bcmi_queue_native", "bcmi_queue_reserve", "bcmi_malloc", "bcmi_free", "bcmi_memcpy",
+    "bcmi_fill", "bcmi_event_create", "bcmi_event_destroy", "bcmi_event_record",
+    "bcmi_event_synchronize", "bcmi_event_elapsed_ms", "bcmi_sort", "bcmi_scan", "bcmi_reduce",
+    "bcmi_reduce_by_key", "bcmi_transform", "bcmi_saxpy", "bcmi_segmented_sort",
+    "bcmi_segmented_scan", "bcmi_graph_begin", "bcmi_graph_end", "bcmi_graph_launch",
+    "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
+    "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
+]
+
+
+class BcmiError(RuntimeError):
+    """Mirrors boost::compute::opencl_error: carries the negative status."""
+
+    def __init__(self, code, msg):
+        super().__init__("bcmi error %d: %s" % (code, msg))
+        self.code = code
+
+
+class DeviceInfo(C.Structure):
+    _fields_ = [("name", C.c_char * 256), ("arch", C.c_char * 64), ("compute_units", C.c_int),
+                ("wavefront_size", C.c_int), ("clock_khz", C.c_int),
+                ("global_mem_bytes", C.c_size_t), ("lds_bytes_per_block", C.c_size_t),
+                ("pci_bus_id", C.c_int)]
+
+
+_lib = None
+
+
+def lib():
+    """Load libbcmi.so (raises if it has not been built)."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise BcmiError(-9999, "libbcmi.so not built: run __graft_entry__.build() or "
+                                   "make -C compute_amd/csrc")
+        L = C.CDLL(LIB_PATH)
+        vp, sz, i, u64p = C.c_void_p, C.c_size_t, C.c_int, C.POINTER(C.c_uint64)
+        L.bcmi_last_error.restype = C.c_char_p
+        L.bcmi_get_version.argtypes = [C.POINTER(C.c_int), C.POINTER(C.c_int)]
+        L.bcmi_device_count.argtypes = [C.POINTER(C.c_int)]
+        L.bcmi_device_get_info.argtypes = [i, C.POINTER(DeviceInfo)]
+        L.bcmi_queue_create.argtypes = [i, C.POINTER(vp)]
+        L.bcmi_queue_destroy.argtypes = [vp]
+        L.bcmi_queue_finish.argtypes = [vp]
+        L.bcmi_queue_device.argtypes = [vp, C.POINTER(C.c_int)]
+        L.bcmi_queue_native.argtypes = [vp]
+        L.bcmi_queue_native.restype = vp
+        L.bcmi_queue_reserve.argtypes = [vp, sz, sz]
+        L.bcmi_malloc.argtypes = [sz, C.POINTER(vp)]
+        L.bcmi_free.argtypes = [vp]
+        L.bcmi_memcpy.argtypes = [vp, vp, vp, sz, i, i]
+        L.bcmi_fill.argtypes = [vp, vp, vp, sz, sz]
+        L.bcmi_event_create.argtypes = [C.POINTER(vp)]
+        L.bcmi_event_destroy.argtypes = [vp]
+        L.bcmi_event_record.argtypes = [vp, vp]
+        L.bcmi_event_synchronize.argtypes = [vp]
+        L.bcmi_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
+        L.bcmi_sort.argtypes = [vp, i, i, vp, vp, sz, sz, i]
+        L.bcmi_scan.argtypes = [vp, i, i, vp, vp, sz, vp, i, i]
+        L.bcmi_reduce.argtypes = [vp, i, i, vp, sz, vp, vp, C.POINTER(C.c_double), i]
+        L.bcmi_reduce_by_key.argtypes = [vp, i, i, i, vp, vp, sz, vp, vp, vp, i]
+        L.bcmi_transform.argtypes = [vp, i, i, vp, vp, vp, sz]
+        L.bcmi_saxpy.argtypes = [vp, C.c_float, vp, vp, vp, sz]
+        L.bcmi_segmented_sort.argtypes = [vp, i, vp, vp, sz, vp, vp, sz, i]
+        L.bcmi_segmented_scan.argtypes = [vp, i, i, vp, vp, vp, sz, vp, i, i]
+        L.bcmi_graph_begin.argtypes = [vp]
+        L.bcmi_graph_end.argtypes = [vp, C.POINTER(vp)]
+        L.bcmi_graph_launch.argtypes = [vp, vp]
+        L.bcmi_graph_destroy.argtypes = [vp]
+        L.bcmi_tune_get.argtypes = [C.c_char_p, C.c_char_p, C.c_uint, C.POINTER(C.c_uint)]
+        L.bcmi_tune_set.argtypes = [C.c_char_p, C.c_char_p, C.c_uint]
+        L.bcmi_tuning_cache_load.argtypes = [C.c_char_p]
+        L.bcmi_tuning_cache_save.argtypes = [C.c_char_p]
+        L.bcmi_profile_enable.argtypes = [vp, i]
+        L.bcmi_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
+        L.bcmi_profile_reset.argtypes = [vp]
+        _lib = L
+    return _lib
+
+
+def check(status):
+    if status != 0:
+        msg = lib().bcmi_last_error()
+        raise BcmiError(status, msg.decode() if msg else "")
+    return status
+
+
+def device_count():
+    n = C.c_int()
+    check(lib().bcmi_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info(dev=0):
+    info = DeviceInfo()
+    check(lib().bcmi_device_get_info(dev, C.byref(info)))
+    return {"name": info.name.decode(), "arch": info.arch.decode(),
+            "compute_units": info.compute_units, "wavefront_size": info.wavefront_size,
+            "clock_khz": info.clock_khz, "global_mem_bytes": info.global_mem_bytes,
+            "lds_bytes_per_block": info.lds_bytes_per_block}
+
+
+class Queue:
+    """An in-order queue (the reference's command_queue) on one device."""
+
+    def __init__(self, device=0):
+        self.h = C.c_void_p()
+        check(lib().bcmi_queue_create(device, C.byref(self.h)))
+
+    def finish(self):
+        check(lib().bcmi_queue_finish(self.h))
+
+    def reserve(self, n, elem_bytes):
+        check(lib().bcmi_queue_reserve(self.h, n, elem_bytes))
+
+    def close(self):
+        if self.h:
+            lib().bcmi_queue_destroy(self.h)
+            self.h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # profiling (HIP events around each kernel launch on this queue's stream)
+    def profile(self, on=True):
+        check(lib().bcmi_profile_enable(self.h, int(on)))
+
+    def profile_query(self, kernel):
+        n = C.c_int()
+        ms = C.c_double()
+        check(lib().bcmi_profile_query(self.h, kernel.encode(), C.byref(n), C.byref(ms)))
+        return n.value, ms.value
+
+    def profile_reset(self):
+        check(lib().bcmi_profile_reset(self.h))
+
+
+class Event:
+    def __init__(self):
+        self.h = C.c_void_p()
+        check(lib().bcmi_event_create(C.byref(self.h)))
+
+    def record(self, q):
+        check(lib().bcmi_event_record(self.h, q.h))
+
+    def synchronize(self):
+        check(lib().bcmi_event_synchronize(self.h))
+
+    def elapsed_ms(self, end):
+        ms = C.c_float()
+        check(lib().bcmi_event_elapsed_ms(self.h, end.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            lib().bcmi_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+class DeviceArray:
+    """Device allocation holding `size` elements of `dtype` (like vector<T>)."""
+
+    def __init__(self, size, dtype, queue=None):
+        self.dtype = np.dtype(dtype)
+        self.size = int(size)
+        self.nbytes = self.size * self.dtype.itemsize
+        self.ptr = C.c_void_p()
+        check(lib().bcmi_malloc(max(self.nbytes, 1), C.byref(self.ptr)))
+        self.queue = queue
+
+    @classmethod
+    def from_host(cls, a, queue):
+        a = np.ascontiguousarray(a)
+        d = cls(a.size, a.dtype, queue)
+        if a.size:
+            check(lib().bcmi_memcpy(queue.h, d.ptr, a.ctypes.data_as(C.c_void_p), a.nbytes, H2D, 1))
+        return d
+
+    def to_host(self, queue=None, count=None):
+        q = queue or self.queue
+        count = self.size if count is None else count
+        out = np.empty(count, self.dtype)
+        if count:
+            check(lib().bcmi_memcpy(q.h, out.ctypes.data_as(C.c_void_p), self.ptr,
+                                    count * self.dtype.itemsize, D2H, 1))
+        return out
+
+    def at(self, offset):
+        """Element pointer (buffer_iterator index) as a raw address."""
+        return C.c_void_p(self.ptr.value + offset * self.dtype.itemsize)
+
+    def free(self):
+        if self.ptr:
+            lib().bcmi_free(self.ptr)
+            self.ptr = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _p(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.ptr
+    return x
+
+
+# ---------------------------------------------------------------- algorithms
+def sort(q, keys, values=None, api=SORT, descending=False, n=None, key_ptr=None, val_ptr=None):
+    n = keys.size if n is None else n
+    vb = values.dtype.itemsize if values is not None else 0
+    check(lib().bcmi_sort(q.h, api, DTYPE[keys.dtype], key_ptr or keys.ptr,
+                          (val_ptr or values.ptr) if values is not None else None, vb, n,
+                          int(descending)))
+
+
+def scan(q, inp, out, exclusive, init=None, op=PLUS, precision=COMPENSATED, n=None):
+    n = inp.size if n is None else n
+    ini = None
+    if init is not None:
+        ini_arr = np.array([init], dtype=inp.dtype)
+        ini = ini_arr.ctypes.data_as(C.c_void_p)
+    check(lib().bcmi_scan(q.h, DTYPE[inp.dtype], op, _p(inp), _p(out), n, ini, int(exclusive),
+                          precision))
+
+
+def reduce(q, inp, out, op=PLUS, init=None, precision=COMPENSATED, n=None):
+    """Device-side reduce into out (DeviceArray of >=1 element). Returns the FP64
+    value before final rounding (blocking) for float sums."""
+    n = inp.size if n is None else n
+    ini = None
+    if init is not None:
+        ini_arr = np.array([init], dtype=inp.dtype)
+        ini = ini_arr.ctypes.data_as(C.c_void_p)
+    f64 = DeviceArray(1, np.float64, q)
+    check(lib().bcmi_reduce(q.h, DTYPE[inp.dtype], op, _p(inp), n, ini, _p(out),
+                            C.cast(f64.ptr, C.POINTER(C.c_double)), precision))
+    return f64
+
+
+def reduce_by_key(q, keys, vals, keys_out, vals_out, op=PLUS, precision=COMPENSATED):
+    cnt = DeviceArray(1, np.uint64, q)
+    check(lib().bcmi_reduce_by_key(q.h, DTYPE[keys.dtype], DTYPE[vals.dtype], op, keys.ptr,
+                                   vals.ptr, keys.size, keys_out.ptr, vals_out.ptr, cnt.ptr,
+                                   precision))
+    return int(cnt.to_host(q)[0])
+
+
+def segmented_scan(q, keys, vals, out, exclusive, init=None, precision=COMPENSATED):
+    ini = None
+    if init is not None:
+        ini_arr = np.array([init], dtype=vals.dtype)
+        ini = ini_arr.ctypes.data_as(C.c_void_p)
+    check(lib().bcmi_segmented_scan(q.h, DTYPE[keys.dtype], DTYPE[vals.dtype], keys.ptr, vals.ptr,
+                                    out.ptr, keys.size, ini, int(exclusive), precision))
+
+
+def transform(q, a, b, out, op):
+    check(lib().bcmi_transform(q.h, DTYPE[a.dtype], op, _p(a), _p(b), _p(out), a.size))
+
+
+def saxpy(q, a, x, y, out):
+    check(lib().bcmi_saxpy(q.h, float(a), x.ptr, y.ptr, out.ptr, x.size))
+
+
+def segmented_sort(q, keys, values, offsets_dev, offsets_host, descending=False):
+    offsets_host = np.ascontiguousarray(offsets_host, np.uint64)
+    vb = values.dtype.itemsize if values is not None else 0
+    check(lib().bcmi_segmented_sort(q.h, DTYPE[keys.dtype], keys.ptr,
+                                    values.ptr if values is not None else None, vb, offsets_dev.ptr,
+                                    offsets_host.ctypes.data_as(C.c_void_p), offsets_host.size - 1,
+                                    int(descending)))
+
+
+class Graph:
+    """hipGraph capture/replay of work enqueued on a queue."""
+
+    def __init__(self, q):
+        self.q = q
+        self.h = C.c_void_p()
+
+    def __enter__(self):
+        check(lib().bcmi_graph_begin(self.q.h))
+        return self
+
+    def __exit__(self, et, ev, tb):
+        check(lib().bcmi_graph_end(self.q.h, C.byref(self.h)))
+        return False
+
+    def launch(self, q=None):
+        check(lib().bcmi_graph_launch(self.h, (q or self.q).h))
+
+    def __del__(self):
+        try:
+            lib().bcmi_graph_destroy(self.h)
+        except Exception:
+            pass

@@ -1,0 +1,289 @@
+// device.hpp -- gfx950 device primitives shared by the kernels.
+//
+// Wave64 building blocks (DPP row shifts/broadcasts, ballots, mbcnt),
+// decoupled look-back status words, the last-block protocol that keeps the
+// queue's control block zeroed, and error-free float transforms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "runtime.hpp"
+
+namespace bcmi {
+
+#define BDEV __device__ __forceinline__
+#define BHD __host__ __device__ __forceinline__
+
+constexpr int kWave = 64;
+
+BDEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
+
+BDEV uint64_t ballot(bool p) { return __ballot(p); }
+
+// number of set bits of m in lanes below this lane
+BDEV uint32_t mbcnt(uint64_t m)
+{
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+BDEV uint32_t popc64(uint64_t m) { return (uint32_t)__builtin_popcountll(m); }
+
+// ------------------------------------------------------------------ DPP
+// dpp_ctrl encodings (GFX9): row_shr:n = 0x110+n, row_bcast:15 = 0x142,
+// row_bcast:31 = 0x143, wave_shr:1 = 0x138.  Lanes whose source is invalid
+// or whose row is masked off keep `old`.
+template <int CTRL, int ROWMASK = 0xf, int BANKMASK = 0xf>
+BDEV uint32_t dpp32(uint32_t old, uint32_t v)
+{
+    return (uint32_t)__builtin_amdgcn_update_dpp((int)old, (int)v, CTRL, ROWMASK, BANKMASK, false);
+}
+
+// Any trivially copyable value whose size is a multiple of 4 bytes moves
+// as that many 32-bit DPP moves.
+template <int N>
+struct words {
+    uint32_t w[N];
+};
+
+template <int CTRL, int ROWMASK = 0xf, int BANKMASK = 0xf, typename T>
+BDEV T dpp(T old, T v)
+{
+    static_assert(sizeof(T) % 4 == 0, "dpp on multiples of 4 bytes");
+    constexpr int N = sizeof(T) / 4;
+    words<N> o = __builtin_bit_cast(words<N>, old), x = __builtin_bit_cast(words<N>, v), r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.w[i] = dpp32<CTRL, ROWMASK, BANKMASK>(o.w[i], x.w[i]);
+    return __builtin_bit_cast(T, r);
+}
+
+template <typename T>
+BDEV T readlane(T v, int lane)
+{
+    static_assert(sizeof(T) % 4 == 0, "readlane on multiples of 4 bytes");
+    constexpr int N = sizeof(T) / 4;
+    words<N> x = __builtin_bit_cast(words<N>, v), r;
+#pragma unroll
+    for (int i = 0; i < N; ++i) r.w[i] = (uint32_t)__builtin_amdgcn_readlane((int)x.w[i], lane);
+    return __builtin_bit_cast(T, r);
+}
+
+// Inclusive scan across the 64 lanes: Hillis-Steele inside each 16-lane
+// row (row_shr 1,2,4,8), then row_bcast:15 into rows 1,3 and row_bcast:31
+// into rows 2,3.  `op` must be associative; `id` its identity.
+template <typename T, typename Op>
+BDEV T wave_inclusive_scan(T x, T id, Op op)
+{
+    x = op(dpp<0x111>(id, x), x);
+    x = op(dpp<0x112>(id, x), x);
+    x = op(dpp<0x114>(id, x), x);
+    x = op(dpp<0x118>(id, x), x);
+    x = op(dpp<0x142, 0xa>(id, x), x);
+    x = op(dpp<0x143, 0xc>(id, x), x);
+    return x;
+}
+
+// inclusive -> exclusive: shift the whole wave right by one lane
+template <typename T>
+BDEV T wave_shr1(T incl, T id) { return dpp<0x138>(id, incl); }
+
+template <typename T, typename Op>
+BDEV T wave_reduce(T x, T id, Op op) { return readlane(wave_inclusive_scan(x, id, op), 63); }
+
+// --------------------------------------------------- decoupled look-back
+// A status word is [tag:32 | payload:32]; tag = epoch << 2 | flag.  A tag
+// whose epoch differs from the current launch's reads as "not published",
+// so the array never needs clearing between launches.  Payloads wider than
+// 32 bits are split over several words that carry the same tag; a reader
+// accepts them only when all tags agree.
+enum : uint32_t { LB_X = 0, LB_A = 1, LB_P = 2 };
+
+BDEV uint64_t lb_word(uint32_t epoch, uint32_t flag, uint32_t payload)
+{
+    return ((uint64_t)((epoch << 2) | flag) << 32) | payload;
+}
+BDEV uint32_t lb_tag(uint64_t w) { return (uint32_t)(w >> 32); }
+BDEV uint32_t lb_flag(uint64_t w) { return (uint32_t)(w >> 32) & 3u; }
+BDEV uint32_t lb_epoch(uint64_t w) { return (uint32_t)(w >> 34); }
+BDEV uint32_t lb_payload(uint64_t w) { return (uint32_t)w; }
+
+BDEV void lb_store(uint64_t *p, uint64_t w) { __hip_atomic_store(p, w, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT); }
+BDEV uint64_t lb_load(const uint64_t *p)
+{
+    return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// Epoch of the current launch: read once per block at kernel start.
+BDEV uint32_t launch_epoch(const Ctrl *ctrl)
+{
+    uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(&ctrl->epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return (e % kEpochMask) + 1u;
+}
+
+// Dynamic tile id: tiles are handed out in the order blocks start, so every
+// tile's predecessors are resident or finished (forward progress for the
+// look-back spin).  A counter left non-zero by an aborted launch is reported
+// through ctrl->fault and folded back into range.
+BDEV uint32_t next_tile(Ctrl *ctrl)
+{
+    uint32_t t = __hip_atomic_fetch_add(&ctrl->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (t >= gridDim.x) {
+        __hip_atomic_store(&ctrl->fault, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        t %= gridDim.x;
+    }
+    return t;
+}
+
+// Every look-back wait is bounded: after kSpinLimit polls (each followed by
+// s_sleep 1, ~64 clocks; about 0.1 s in total) the wait gives up, records
+// ctrl->fault and proceeds, so a protocol error can never hang the device.
+constexpr uint32_t kSpinLimit = 1u << 21;
+
+BDEV bool spin_expired(Ctrl *ctrl, uint32_t &spins)
+{
+    if (++spins < kSpinLimit) {
+        __builtin_amdgcn_s_sleep(1);
+        return false;
+    }
+    __hip_atomic_store(&ctrl->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return true;
+}
+
+// Last-block protocol.  Every block calls this once at its end (all threads,
+// after its global writes).  Returns true in exactly one block, the last to
+// finish; that block has acquired every other block's writes.
+BDEV bool arrive_last(Ctrl *ctrl, uint32_t nblocks, bool *s_flag)
+{
+    // every wave releases its own global writes at agent scope before the
+    // block is counted
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t prev = __hip_atomic_fetch_add(&ctrl->done_counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        *s_flag = (prev == nblocks - 1);
+    }
+    __syncthreads();
+    bool last = *s_flag;
+    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return last;
+}
+
+// Called by one thread of the last block: reset the counters and advance
+// the epoch for the next look-back launch on this queue.
+BDEV void finish_launch(Ctrl *ctrl, bool advance_epoch)
+{
+    __hip_atomic_store(&ctrl->tile_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(&ctrl->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    if (advance_epoch) {
+        uint32_t e = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctrl->epoch, e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+// ------------------------------------------------ error-free transforms
+// Knuth's TwoSum: s + e == a + b exactly (no FMA involved, so contraction
+// cannot alter it).
+BDEV void two_sum(float a, float b, float &s, float &e)
+{
+    s = a + b;
+    float bp = s - a;
+    e = (a - (s - bp)) + (b - bp);
+}
+
+// compensated FP32 accumulator: value = s + c
+struct cf32 {
+    float s, c;
+};
+
+BDEV cf32 cf_add(cf32 a, float x)
+{
+    float s, e;
+    two_sum(a.s, x, s, e);
+    return cf32{s, a.c + e};
+}
+
+BDEV cf32 cf_merge(cf32 a, cf32 b)
+{
+    float s, e;
+    two_sum(a.s, b.s, s, e);
+    return cf32{s, (a.c + b.c) + e};
+}
+
+BDEV double cf_value(cf32 a) { return (double)a.s + (double)a.c; }
+
+}  // namespace bcmi
+
+namespace bcmi {
+
+// ------------------------------------------- wave-parallel decoupled look-back
+// A carry of 4*N bytes occupies N consecutive status words per tile.
+template <typename Cr>
+BDEV void lb_publish(uint64_t *status, uint32_t tile, uint32_t epoch, uint32_t flag, Cr v)
+{
+    static_assert(sizeof(Cr) % 4 == 0, "carry of a multiple of 4 bytes");
+    constexpr int N = sizeof(Cr) / 4;
+    words<N> x = __builtin_bit_cast(words<N>, v);
+#pragma unroll
+    for (int i = 0; i < N; ++i) lb_store(&status[(size_t)N * tile + i], lb_word(epoch, flag, x.w[i]));
+}
+
+template <typename Cr>
+BDEV uint32_t lb_read(const uint64_t *status, size_t tile, uint32_t epoch, Cr &v)
+{
+    constexpr int N = sizeof(Cr) / 4;
+    uint64_t w[N];
+#pragma unroll
+    for (int i = 0; i < N; ++i) w[i] = lb_load(&status[(size_t)N * tile + i]);
+    bool ok = lb_epoch(w[0]) == epoch;
+#pragma unroll
+    for (int i = 1; i < N; ++i) ok = ok && (lb_tag(w[i]) == lb_tag(w[0]));
+    if (!ok) return LB_X;
+    words<N> x;
+#pragma unroll
+    for (int i = 0; i < N; ++i) x.w[i] = lb_payload(w[i]);
+    v = __builtin_bit_cast(Cr, x);
+    return lb_flag(w[0]);
+}
+
+// Called by all 64 lanes of one wave.  Publishes the tile's aggregate,
+// folds the predecessors' aggregates / inclusive prefixes 64 tiles at a
+// time until an inclusive prefix is found, publishes this tile's inclusive
+// prefix and returns its exclusive prefix (init folded in at tile 0).
+// `cop(a, b)` must be associative; a is the earlier operand.  Lane l reads
+// tile (base - l), so the lanes are folded with the operands swapped to keep
+// tile order.
+template <typename Cr, typename Op>
+BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, Cr init, Cr cid, Op cop,
+                      Ctrl *ctrl)
+{
+    uint32_t spins = 0;
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) lb_publish(status, 0, epoch, LB_P, cop(init, agg));
+        return init;
+    }
+    if (lane == 0) lb_publish(status, tile, epoch, LB_A, agg);
+    Cr acc = cid;
+    int64_t base = (int64_t)tile - 1;
+    while (true) {
+        int64_t idx = base - (int64_t)lane;
+        Cr v = cid;
+        uint32_t flag = LB_P;
+        if (idx >= 0) flag = lb_read(status, (size_t)idx, epoch, v);
+        uint64_t pm = ballot(flag == LB_P);
+        uint64_t xm = ballot(flag == LB_X);
+        uint32_t first = pm ? (uint32_t)__builtin_ctzll(pm) : 64u;
+        uint64_t rel = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+        if ((xm & rel) && !spin_expired(ctrl, spins)) continue;
+        Cr mine = lane <= first ? v : cid;
+        Cr older = wave_reduce(mine, cid, [&](Cr a, Cr b) { return cop(b, a); });
+        acc = cop(older, acc);
+        if (first < 64) break;
+        base -= 64;
+    }
+    if (lane == 0) lb_publish(status, tile, epoch, LB_P, cop(acc, agg));
+    return acc;
+}
+
+}  // namespace bcmi

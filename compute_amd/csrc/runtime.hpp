@@ -1,0 +1,136 @@
+// runtime.hpp -- host-side internals shared by the C-ABI entry points.
+//
+// A bcmi_queue is the MI355X counterpart of the reference's command_queue
+// (command_queue.hpp): one in-order HIP stream plus everything the
+// algorithms need between calls on that stream:
+//   * a grow-only scratch arena for temporaries.  The reference allocates a
+//     vector per call and relies on OpenCL's deferred release
+//     (radix_sort.hpp:333-336); HIP frees synchronously, so the arena is
+//     reused across calls and only grows (after a stream sync);
+//   * a look-back status array (decoupled look-back scans and onesweep
+//     digit offsets).  Status words are tagged with an epoch that the last
+//     block of every look-back kernel advances on the device, so the array
+//     is never cleared between calls and graph replays stay correct;
+//   * a small device control block (epoch, tile/done counters, digit
+//     histograms) that every kernel leaves zeroed for the next one.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <string>
+#include <vector>
+
+#include "../../include/bcmi.h"
+
+namespace bcmi {
+
+// Device control block, zero-initialised at queue creation.
+struct Ctrl {
+    uint32_t epoch;          // look-back epoch counter
+    uint32_t tile_counter;   // dynamic tile id dispenser
+    uint32_t done_counter;   // finished-block counter
+    uint32_t fault;          // set by a kernel whose look-back spin limit expired
+    uint32_t hist[8][256];   // radix digit histograms (self-clearing)
+    uint32_t offsets[8][256];// exclusive digit offsets per pass
+};
+
+constexpr uint32_t kEpochMask = (1u << 30) - 1u;
+
+struct ProfRec {
+    const char *name;
+    hipEvent_t start, stop;
+};
+
+}  // namespace bcmi
+
+struct bcmi_queue_s {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    int cu_count = 0;
+    size_t lds_per_block = 0;
+    // scratch arena
+    void *scratch = nullptr;
+    size_t scratch_bytes = 0;
+    // look-back status words (uint64)
+    uint64_t *status = nullptr;
+    size_t status_words = 0;
+    uint64_t lookback_launches = 0;
+    // control block
+    bcmi::Ctrl *ctrl = nullptr;
+    // pinned host staging for small blocking readbacks
+    void *pinned = nullptr;
+    bool capturing = false;
+    uint64_t capture_lookback_launches = 0;
+    // profiling
+    bool profile = false;
+    std::vector<bcmi::ProfRec> prof_pending;
+    std::vector<hipEvent_t> event_pool;
+    std::map<std::string, std::pair<int, double>> prof_acc;
+};
+
+struct bcmi_graph_s {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    uint64_t lookback_launches = 0;
+};
+
+namespace bcmi {
+
+void set_error(const char *fmt, ...);
+int hip_fail(hipError_t e, const char *what, const char *file, int line);
+
+#define BCMI_HIP(call)                                                       \
+    do {                                                                     \
+        hipError_t _e = (call);                                              \
+        if (_e != hipSuccess) return ::bcmi::hip_fail(_e, #call, __FILE__, __LINE__); \
+    } while (0)
+
+#define BCMI_CHECK(cond, code, ...)                                          \
+    do {                                                                     \
+        if (!(cond)) { ::bcmi::set_error(__VA_ARGS__); return (code); }     \
+    } while (0)
+
+#define BCMI_TRY(expr)                                                       \
+    do {                                                                     \
+        int _s = (expr);                                                     \
+        if (_s != 0) return _s;                                              \
+    } while (0)
+
+int dtype_size(int dtype);
+bool dtype_valid(int dtype);
+
+// Make the queue's device current for the calling thread.
+int use_queue(bcmi_queue q);
+// Scratch arena of at least `bytes` (256-byte aligned). Grows only outside
+// graph capture.
+int scratch(bcmi_queue q, size_t bytes, void **ptr);
+// Look-back status array of at least `words` uint64.
+int status_array(bcmi_queue q, size_t words, uint64_t **ptr);
+// Account for one launch of a look-back kernel (epoch wrap bookkeeping).
+int note_lookback_launch(bcmi_queue q, int launches);
+
+// Profiling brackets around a kernel launch on q->stream.
+void prof_begin(bcmi_queue q, const char *name, ProfRec *rec);
+int prof_end(bcmi_queue q, ProfRec *rec);
+int check_launch(const char *name);
+
+// Persistent grid sizing: blocks of `threads` that fill the device.
+inline int cu_blocks(bcmi_queue q, int per_cu) { return q->cu_count * per_cu; }
+
+// Tuning cache (parameter_cache analogue).
+unsigned tune_get(const std::string &key, const char *param, unsigned def);
+
+}  // namespace bcmi
+
+// Launch with profiling brackets and error check.
+#define BCMI_LAUNCH(q, name, ...)                                            \
+    do {                                                                     \
+        ::bcmi::ProfRec _rec;                                                \
+        ::bcmi::prof_begin((q), (name), &_rec);                              \
+        __VA_ARGS__;                                                         \
+        BCMI_TRY(::bcmi::check_launch(name));                                \
+        BCMI_TRY(::bcmi::prof_end((q), &_rec));                              \
+    } while (0)

@@ -1,0 +1,235 @@
+// scan.hip -- exclusive_scan / inclusive_scan / partial_sum.
+//
+// Reference: scan_on_gpu (algorithm/detail/scan_on_gpu.hpp:187-324): a
+// 256-wide Hillis-Steele block scan that writes block sums, a recursive scan
+// of the block sums, then a second kernel that adds them back (about 16
+// bytes of traffic per element over 4+ launches; in-place scans first copy
+// the input, :292-303).
+//
+// Here: one launch, single pass, 8 bytes per element.  Each block takes the
+// next tile id from a counter, scans its tile in registers (per-thread
+// chunk folds, DPP wave scans, an LDS step across the block's 4 waves), and
+// obtains the tile's exclusive prefix by decoupled look-back over the
+// predecessors' published aggregates.  Integer scans wrap exactly.  Float
+// plus carries compensated FP32 pairs inside the tile and an FP64 carry
+// between tiles.  In-place works because a tile reads its inputs before it
+// writes.
+#include "device.hpp"
+#include "ops.hpp"
+
+#include <cstring>
+
+namespace bcmi {
+
+constexpr int kScanThreads = 256;
+constexpr int kScanQ = 4;  // 16-byte chunks per thread
+
+// Integer / plain policy.  Narrow types compute in 32 bits (DPP moves 32 or
+// 64 bits) and truncate on output, which keeps modular results exact.
+template <typename T, int OP>
+struct ScanPlain {
+    using S = std::conditional_t<(sizeof(T) < 4), std::conditional_t<std::is_signed<T>::value, int32_t, uint32_t>, T>;
+    using Cr = S;
+    static BHD S id() { return op_identity<S, OP>(); }
+    static BDEV S add(S s, T x) { return apply_op<S, OP>(s, (S)x); }
+    static BDEV S merge(S a, S b) { return apply_op<S, OP>(a, b); }
+    static BDEV Cr carry(S s) { return s; }
+    static BHD Cr cid() { return id(); }
+    static BDEV Cr cmerge(Cr a, Cr b) { return apply_op<S, OP>(a, b); }
+    static BDEV T out(Cr prefix, S s) { return (T)apply_op<S, OP>(prefix, s); }
+    static Cr host_init(const void *p)
+    {
+        T v;
+        memcpy(&v, p, sizeof(T));
+        return (Cr)v;
+    }
+};
+
+// Compensated float plus: TwoSum pairs within the tile, FP64 between tiles.
+struct ScanCompF32 {
+    using S = cf32;
+    using Cr = double;
+    static BHD S id() { return cf32{0.f, 0.f}; }
+    static BDEV S add(S s, float x) { return cf_add(s, x); }
+    static BDEV S merge(S a, S b) { return cf_merge(a, b); }
+    static BDEV Cr carry(S s) { return cf_value(s); }
+    static BHD Cr cid() { return 0.0; }
+    static BDEV Cr cmerge(Cr a, Cr b) { return a + b; }
+    static BDEV float out(Cr prefix, S s) { return (float)(prefix + cf_value(s)); }
+    static Cr host_init(const void *p)
+    {
+        float v;
+        memcpy(&v, p, sizeof(float));
+        return (double)v;
+    }
+};
+
+template <typename T>
+struct alignas(16) sc_chunk {
+    static constexpr int N = 16 / sizeof(T);
+    T v[N];
+};
+
+template <typename T, typename P, bool VEC>
+__global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size_t n, Ctrl *ctrl,
+                                                       uint64_t *status, typename P::Cr init, int exclusive)
+{
+    using S = typename P::S;
+    using Cr = typename P::Cr;
+    constexpr int V = 16 / sizeof(T);
+    constexpr int WSEG = kScanQ * 64 * V;   // elements per wave
+    constexpr int TILE = 4 * WSEG;          // 4 waves
+    __shared__ uint32_t s_tile;
+    __shared__ S s_wave[4];
+    __shared__ Cr s_prefix;
+    __shared__ bool s_last;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t epoch = launch_epoch(ctrl);
+    if (tid == 0) s_tile = next_tile(ctrl);
+    __syncthreads();
+    const uint32_t tile = s_tile;
+    const size_t tbase = (size_t)tile * TILE;
+    const bool full = tbase + TILE <= n;
+    const size_t wbase = tbase + (size_t)w * WSEG + (size_t)lane * V;
+
+    T x[kScanQ][V];
+#pragma unroll
+    for (int q = 0; q < kScanQ; ++q) {
+        size_t p0 = wbase + (size_t)q * 64 * V;
+        if (VEC && full) {
+            sc_chunk<T> c = *(const sc_chunk<T> *)(in + p0);
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[q][k] = c.v[k];
+        } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[q][k] = (p0 + k < n) ? in[p0 + k] : T(0);
+        }
+    }
+    // chunk folds
+    S tot[kScanQ];
+#pragma unroll
+    for (int q = 0; q < kScanQ; ++q) {
+        size_t p0 = wbase + (size_t)q * 64 * V;
+        S s = P::id();
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (full || p0 + k < n) s = P::add(s, x[q][k]);
+        tot[q] = s;
+    }
+    // wave scans of the chunk folds, carried across q
+    auto mrg = [](S a, S b) { return P::merge(a, b); };
+    S carry = P::id();
+    S pre[kScanQ];
+#pragma unroll
+    for (int q = 0; q < kScanQ; ++q) {
+        S inc = wave_inclusive_scan(tot[q], P::id(), mrg);
+        S exc = wave_shr1(inc, P::id());
+        pre[q] = P::merge(carry, exc);
+        carry = P::merge(carry, readlane(inc, 63));
+    }
+    if (lane == 0) s_wave[w] = carry;
+    __syncthreads();
+    S wpre = P::id();
+    for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, s_wave[i]);
+    if (w == 0) {
+        S total = P::merge(P::merge(s_wave[0], s_wave[1]), P::merge(s_wave[2], s_wave[3]));
+        Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
+                                      [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
+        if (lane == 0) s_prefix = prefix;
+    }
+    __syncthreads();
+    const Cr prefix = s_prefix;
+#pragma unroll
+    for (int q = 0; q < kScanQ; ++q) {
+        size_t p0 = wbase + (size_t)q * 64 * V;
+        S s = P::merge(wpre, pre[q]);
+        T y[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (exclusive) {
+                y[k] = P::out(prefix, s);
+                s = P::add(s, x[q][k]);
+            } else {
+                s = P::add(s, x[q][k]);
+                y[k] = P::out(prefix, s);
+            }
+        }
+        if (VEC && full) {
+            sc_chunk<T> c;
+#pragma unroll
+            for (int k = 0; k < V; ++k) c.v[k] = y[k];
+            *(sc_chunk<T> *)(out + p0) = c;
+        } else {
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                if (p0 + k < n) out[p0 + k] = y[k];
+        }
+    }
+    if (arrive_last(ctrl, gridDim.x, &s_last) && tid == 0) finish_launch(ctrl, true);
+}
+
+template <typename T, typename P>
+static int run_scan(bcmi_queue q, const void *in_, void *out_, size_t n, const void *init, int exclusive)
+{
+    constexpr size_t TILE = (size_t)4 * kScanQ * 64 * (16 / sizeof(T));
+    const T *in = (const T *)in_;
+    T *out = (T *)out_;
+    size_t tiles = (n + TILE - 1) / TILE;
+    BCMI_CHECK(tiles < (1ull << 31), BCMI_INVALID_BUFFER_SIZE, "scan: n=%zu too large", n);
+    uint64_t *status;
+    BCMI_TRY(status_array(q, tiles * (sizeof(typename P::Cr) / 4), &status));
+    typename P::Cr ini = P::cid();
+    if (exclusive && init) ini = P::host_init(init);
+    bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
+    if (vec) {
+        BCMI_LAUNCH(q, "scan",
+                    hipLaunchKernelGGL((k_scan<T, P, true>), dim3((unsigned)tiles), dim3(kScanThreads), 0, q->stream,
+                                       in, out, n, q->ctrl, status, ini, exclusive));
+    } else {
+        BCMI_LAUNCH(q, "scan",
+                    hipLaunchKernelGGL((k_scan<T, P, false>), dim3((unsigned)tiles), dim3(kScanThreads), 0, q->stream,
+                                       in, out, n, q->ctrl, status, ini, exclusive));
+    }
+    return note_lookback_launch(q, 1);
+}
+
+template <typename T>
+static int scan_plain(bcmi_queue q, int op, const void *in, void *out, size_t n, const void *init, int excl)
+{
+    switch (op) {
+    case BCMI_PLUS: return run_scan<T, ScanPlain<T, BCMI_PLUS>>(q, in, out, n, init, excl);
+    case BCMI_MULTIPLIES: return run_scan<T, ScanPlain<T, BCMI_MULTIPLIES>>(q, in, out, n, init, excl);
+    case BCMI_MIN: return run_scan<T, ScanPlain<T, BCMI_MIN>>(q, in, out, n, init, excl);
+    case BCMI_MAX: return run_scan<T, ScanPlain<T, BCMI_MAX>>(q, in, out, n, init, excl);
+    default: set_error("scan: unsupported op %d", op); return BCMI_INVALID_VALUE;
+    }
+}
+
+}  // namespace bcmi
+
+using namespace bcmi;
+
+extern "C" int bcmi_scan(bcmi_queue q, int dtype, int op, const void *in, void *out, size_t n, const void *init,
+                         int exclusive, int precision)
+{
+    BCMI_TRY(use_queue(q));
+    if (n == 0) return 0;
+    BCMI_CHECK(in && out, BCMI_INVALID_VALUE, "scan: null pointer");
+    switch (dtype) {
+    case BCMI_I8: return scan_plain<int8_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_U8: return scan_plain<uint8_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_I16: return scan_plain<int16_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_U16: return scan_plain<uint16_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_I32: return scan_plain<int32_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_U32: return scan_plain<uint32_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_I64: return scan_plain<int64_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_U64: return scan_plain<uint64_t>(q, op, in, out, n, init, exclusive);
+    case BCMI_F32:
+        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED)
+            return run_scan<float, ScanCompF32>(q, in, out, n, init, exclusive);
+        return scan_plain<float>(q, op, in, out, n, init, exclusive);
+    case BCMI_F64: return scan_plain<double>(q, op, in, out, n, init, exclusive);
+    default: set_error("scan: bad dtype %d", dtype); return BCMI_INVALID_VALUE;
+    }
+}

@@ -1,0 +1,81 @@
+"""Seeded synthetic inputs for the benchmark configurations (numpy arrays),
+from the host-only helper libbcmi_workloads.so (std::mt19937 engines with
+explicit transforms of their raw output)."""
+import ctypes as C
+import os
+
+import numpy as np
+
+_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "libbcmi_workloads.so")
+_lib = None
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        L = C.CDLL(_PATH)
+        vp, sz = C.c_void_p, C.c_size_t
+        L.wl_sort_u32.argtypes = [C.c_uint32, vp, sz]
+        L.wl_sort_by_key_f32.argtypes = [vp, vp, sz]
+        L.wl_scan_i32.argtypes = [vp, sz]
+        L.wl_saxpy_f32.argtypes = [vp, vp, sz]
+        L.wl_lognormal_f32.argtypes = [vp, sz]
+        L.wl_segments.argtypes = [vp, sz, sz]
+        L.wl_segments.restype = sz
+        L.wl_segment_data.argtypes = [vp, vp, sz]
+        L.wl_runs_i32_f32.argtypes = [vp, vp, sz, sz]
+        L.wl_runs_i32_f32.restype = sz
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p)
+
+
+def sort_u32(n, seed=0xB0057001):
+    k = np.empty(n, np.uint32)
+    lib().wl_sort_u32(seed, _p(k), n)
+    return k
+
+
+def sort_by_key_f32(n):
+    k, v = np.empty(n, np.float32), np.empty(n, np.uint32)
+    lib().wl_sort_by_key_f32(_p(k), _p(v), n)
+    return k, v
+
+
+def scan_i32(n):
+    x = np.empty(n, np.int32)
+    lib().wl_scan_i32(_p(x), n)
+    return x
+
+
+def saxpy_f32(n):
+    x, y = np.empty(n, np.float32), np.empty(n, np.float32)
+    lib().wl_saxpy_f32(_p(x), _p(y), n)
+    return x, y
+
+
+def lognormal_f32(n):
+    x = np.empty(n, np.float32)
+    lib().wl_lognormal_f32(_p(x), n)
+    return x
+
+
+def segments(nseg, max_len=1 << 20):
+    off = np.empty(nseg + 1, np.uint64)
+    lib().wl_segments(_p(off), nseg, max_len)
+    return off
+
+
+def segment_data(n):
+    k, v = np.empty(n, np.uint64), np.empty(n, np.uint32)
+    lib().wl_segment_data(_p(k), _p(v), n)
+    return k, v
+
+
+def runs_i32_f32(n, max_run=4096):
+    k, v = np.empty(n, np.int32), np.empty(n, np.float32)
+    lib().wl_runs_i32_f32(_p(k), _p(v), n, max_run)
+    return k, v

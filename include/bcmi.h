@@ -1,0 +1,232 @@
+/*
+ * bcmi.h -- C-ABI of the MI355X-native data-parallel primitive core.
+ *
+ * The reference (Boost.Compute 0.5.0) has no C-ABI: its boundary is the
+ * header-only C++ templates in namespace boost::compute over vector
+ * iterators and a command_queue.  This library is the native layer those
+ * templates call in this build; include/boost/compute/ holds the
+ * source-compatible templates (the "reference-side binding").  Every entry
+ * point below names the reference interface it replaces.
+ *
+ * Conventions
+ *   - Plain pointers and sizes; no C++ or torch types.
+ *   - Every function returns an int status: 0 on success, otherwise a
+ *     negative code whose value matches the OpenCL error the reference would
+ *     raise (exception/opencl_error.hpp:30-62); bcmi_last_error() gives a
+ *     thread-local message.
+ *   - Device pointers come from bcmi_malloc (hipMalloc).  A
+ *     buffer_iterator's (buffer, index) pair is passed as one element
+ *     pointer, so no offset is ever baked into code.
+ *   - Algorithms are asynchronous on the queue (one HIP stream) unless the
+ *     comment says "blocking", mirroring which reference calls block
+ *     (reduce.hpp:225, accumulate.hpp:53-55,
+ *     reduce_by_key_with_scan.hpp:485-487).
+ *   - Sizes are size_t; sorts accept n < 2^32 (radix_sort.hpp:131-137 uses
+ *     uint counts).
+ */
+#ifndef BCMI_H
+#define BCMI_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------ status */
+enum bcmi_status {
+    BCMI_SUCCESS = 0,
+    BCMI_DEVICE_NOT_FOUND = -1,              /* CL_DEVICE_NOT_FOUND */
+    BCMI_MEM_ALLOCATION_FAILURE = -4,        /* CL_MEM_OBJECT_ALLOCATION_FAILURE */
+    BCMI_OUT_OF_RESOURCES = -5,              /* CL_OUT_OF_RESOURCES */
+    BCMI_INVALID_VALUE = -30,                /* CL_INVALID_VALUE */
+    BCMI_INVALID_COMMAND_QUEUE = -36,        /* CL_INVALID_COMMAND_QUEUE */
+    BCMI_INVALID_OPERATION = -59,            /* CL_INVALID_OPERATION */
+    BCMI_INVALID_BUFFER_SIZE = -61,          /* CL_INVALID_BUFFER_SIZE */
+    BCMI_RUNTIME_ERROR = -9999               /* HIP runtime failure */
+};
+
+/* element types (a key or value type is named by one of these) */
+enum bcmi_dtype {
+    BCMI_I8 = 1, BCMI_U8, BCMI_I16, BCMI_U16, BCMI_I32, BCMI_U32,
+    BCMI_I64, BCMI_U64, BCMI_F32, BCMI_F64
+};
+
+/* binary functors recognised by the dispatch: plus<T>, multiplies<T>,
+ * min<T>, max<T>, minus<T>, divides<T> (functional/operator.hpp:57-96,
+ * functional/common.hpp) */
+enum bcmi_op {
+    BCMI_PLUS = 0, BCMI_MULTIPLIES = 1, BCMI_MIN = 2, BCMI_MAX = 3,
+    BCMI_MINUS = 4, BCMI_DIVIDES = 5
+};
+
+/* sort front-ends (which reference template is being served) */
+enum bcmi_sort_api {
+    BCMI_SORT = 0,               /* sort.hpp:182-202: n<=32 insertion sort, else radix */
+    BCMI_SORT_BY_KEY = 1,        /* sort_by_key.hpp:135-163: n<32 insertion, else radix */
+    BCMI_STABLE_SORT = 2,        /* stable_sort.hpp:81-110: radix at any n */
+    BCMI_STABLE_SORT_BY_KEY = 3, /* stable_sort_by_key.hpp:135-163: n<32 insertion */
+    BCMI_RADIX_SORT = 4          /* detail::radix_sort(_by_key), radix_sort.hpp:428-462 */
+};
+
+/* float reduction/scan accumulation */
+enum bcmi_precision {
+    BCMI_PRECISION_COMPENSATED = 0, /* two-sum FP32 in waves, FP64 cross-block (default) */
+    BCMI_PRECISION_NAIVE = 1        /* plain FP32 tree (for error comparison only) */
+};
+
+/* ---------------------------------------------------- errors / devices */
+const char *bcmi_last_error(void);
+int bcmi_get_version(int *major, int *minor);
+
+typedef struct bcmi_device_info {
+    char name[256];          /* marketing name */
+    char arch[64];           /* gcnArchName, e.g. gfx950 */
+    int compute_units;       /* CL_DEVICE_MAX_COMPUTE_UNITS analogue */
+    int wavefront_size;
+    int clock_khz;
+    size_t global_mem_bytes;
+    size_t lds_bytes_per_block;
+    int pci_bus_id;
+} bcmi_device_info;
+
+/* device.hpp / system.hpp:92-184 (default device = device 0 unless
+ * BOOST_COMPUTE_DEFAULT_DEVICE names another, system.hpp:244-274) */
+int bcmi_device_count(int *count);
+int bcmi_device_get_info(int device, bcmi_device_info *info);
+
+/* -------------------------------------------------------------- queues */
+/* command_queue.hpp: an in-order queue = one HIP stream plus the queue's
+ * scratch arena (temporaries) and look-back state.  Not thread-safe: like a
+ * command_queue, use one per host thread. */
+typedef struct bcmi_queue_s *bcmi_queue;
+int bcmi_queue_create(int device, bcmi_queue *queue);
+int bcmi_queue_destroy(bcmi_queue queue);
+/* command_queue::finish (command_queue.hpp:1564): blocking */
+int bcmi_queue_finish(bcmi_queue queue);
+int bcmi_queue_device(bcmi_queue queue, int *device);
+/* the underlying hipStream_t, for interop */
+void *bcmi_queue_native(bcmi_queue queue);
+/* Pre-size the scratch arena and look-back state for n elements of up to
+ * elem_bytes key + value bytes (call before hipGraph capture). */
+int bcmi_queue_reserve(bcmi_queue queue, size_t n, size_t elem_bytes);
+
+/* ------------------------------------------------------------- memory */
+/* buffer.hpp:53-121 (clCreateBuffer / release) */
+int bcmi_malloc(size_t bytes, void **ptr);
+int bcmi_free(void *ptr);
+/* command_queue::enqueue_{read,write,copy}_buffer (command_queue.hpp:297-766);
+ * blocking != 0 waits for completion */
+enum bcmi_copy_kind { BCMI_H2D = 1, BCMI_D2H = 2, BCMI_D2D = 3 };
+int bcmi_memcpy(bcmi_queue queue, void *dst, const void *src, size_t bytes,
+                int kind, int blocking);
+/* fill (algorithm/fill.hpp): pattern of 1, 2, 4, 8 or 16 bytes, by kernel
+ * (safe under graph capture) */
+int bcmi_fill(bcmi_queue queue, void *dst, const void *pattern,
+              size_t pattern_bytes, size_t count);
+
+/* --------------------------------------------------------------- events */
+/* event.hpp:86-89,204,264 (profiling duration) */
+typedef struct bcmi_event_s *bcmi_event;
+int bcmi_event_create(bcmi_event *event);
+int bcmi_event_destroy(bcmi_event event);
+int bcmi_event_record(bcmi_event event, bcmi_queue queue);
+int bcmi_event_synchronize(bcmi_event event);
+int bcmi_event_elapsed_ms(bcmi_event start, bcmi_event end, float *ms);
+
+/* ----------------------------------------------------------- algorithms */
+/* sort / sort_by_key / stable_sort / stable_sort_by_key / detail::radix_sort.
+ * keys: n elements of key_dtype, sorted in place.  values: NULL, or n
+ * elements of value_bytes (1, 2, 4, 8 or 16) permuted with the keys.
+ * descending != 0 serves greater<T>.  The result is bit-identical to the
+ * reference's GPU column: radix order by the radix() key transform
+ * (radix_sort.hpp:98-129; -0.0 before +0.0, NaNs by sign at the ends) and,
+ * below the api's threshold, the one-work-item insertion sort
+ * (insertion_sort.hpp:25-159). */
+int bcmi_sort(bcmi_queue queue, int api, int key_dtype, void *keys,
+              void *values, size_t value_bytes, size_t n, int descending);
+
+/* exclusive_scan (exclusive_scan.hpp:55-104), inclusive_scan
+ * (inclusive_scan.hpp:53-87), partial_sum (partial_sum.hpp:31-41).
+ * in == out is allowed.  init: host pointer to one element (exclusive only;
+ * NULL means the op's identity for inclusive).  Integers wrap; float plus is
+ * compensated (precision) with an FP64 carry between tiles. */
+int bcmi_scan(bcmi_queue queue, int dtype, int op, const void *in, void *out,
+              size_t n, const void *init, int exclusive, int precision);
+
+/* reduce (reduce.hpp:275-305) and accumulate (accumulate.hpp:165-188).
+ * Writes op-fold of [in, in+n) (then folded with *init when init != NULL,
+ * i.e. accumulate) to the device pointer out (one element).  Float plus is
+ * compensated with an FP64 cross-block combine; out_f64 (may be NULL)
+ * receives the FP64 value before the final rounding. */
+int bcmi_reduce(bcmi_queue queue, int dtype, int op, const void *in, size_t n,
+                const void *init, void *out, double *out_f64, int precision);
+
+/* reduce_by_key (reduce_by_key.hpp:61-132) with equal_to on keys: for each
+ * run of adjacent equal keys, the run's last key and the op-fold of its
+ * values.  *count_out (device pointer, uint64) receives the number of runs;
+ * the host-side shim reads it back (blocking, as the reference does). */
+int bcmi_reduce_by_key(bcmi_queue queue, int key_dtype, int value_dtype,
+                       int op, const void *keys, const void *values, size_t n,
+                       void *keys_out, void *values_out, uint64_t *count_out,
+                       int precision);
+
+/* transform (transform.hpp:38-82) with a recognised binary functor:
+ * out[i] = op(a[i], b[i]). */
+int bcmi_transform(bcmi_queue queue, int dtype, int op, const void *a,
+                   const void *b, void *out, size_t n);
+/* transform with the lambda a*_1+_2 (perf/perf_saxpy.cpp:50-55):
+ * out[i] = fmaf(a, x[i], y[i]) (the contracted form of "(a*x[i])+y[i]"). */
+int bcmi_saxpy(bcmi_queue queue, float a, const float *x, const float *y,
+               float *out, size_t n);
+
+/* ------------------------------------------------- batched / segmented */
+/* Stable sort of nseg independent segments in one batch: segment s is
+ * [offsets[s], offsets[s+1]) of keys/values.  offsets: device pointer to
+ * nseg+1 uint64; host_offsets: the same array in host memory (needed to
+ * plan the launch).  No reference counterpart as one call: it replaces one
+ * stable_sort_by_key call per segment. */
+int bcmi_segmented_sort(bcmi_queue queue, int key_dtype, void *keys,
+                        void *values, size_t value_bytes,
+                        const uint64_t *offsets, const uint64_t *host_offsets,
+                        size_t nseg, int descending);
+
+/* Segmented scan over runs of adjacent equal keys (the structure of
+ * reduce_by_key_with_scan.hpp:104-293 exposed as a scan): exclusive runs
+ * start at *init, inclusive runs restart at each run head. */
+int bcmi_segmented_scan(bcmi_queue queue, int key_dtype, int value_dtype,
+                        const void *keys, const void *values, void *out,
+                        size_t n, const void *init, int exclusive,
+                        int precision);
+
+/* --------------------------------------------------------- hipGraphs */
+/* Capture everything enqueued on the queue between begin and end into a
+ * replayable graph (queue scratch must be reserved beforehand). */
+typedef struct bcmi_graph_s *bcmi_graph;
+int bcmi_graph_begin(bcmi_queue queue);
+int bcmi_graph_end(bcmi_queue queue, bcmi_graph *graph);
+int bcmi_graph_launch(bcmi_graph graph, bcmi_queue queue);
+int bcmi_graph_destroy(bcmi_graph graph);
+
+/* ------------------------------------------------------------ tuning */
+/* parameter_cache (detail/parameter_cache.hpp:63-82): per-device
+ * (key, param) -> value, persisted as JSON. */
+int bcmi_tune_get(const char *key, const char *param, unsigned default_value,
+                  unsigned *value);
+int bcmi_tune_set(const char *key, const char *param, unsigned value);
+int bcmi_tuning_cache_load(const char *path);
+int bcmi_tuning_cache_save(const char *path);
+
+/* ------------------------------------------------------------ profiling */
+/* When enabled, the queue brackets each kernel launch with HIP events on
+ * its own stream; bcmi_profile_query sums them per kernel name (blocking). */
+int bcmi_profile_enable(bcmi_queue queue, int enable);
+int bcmi_profile_query(bcmi_queue queue, const char *kernel, int *launches,
+                       double *total_ms);
+int bcmi_profile_reset(bcmi_queue queue);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* BCMI_H */

@@ -1,0 +1,311 @@
+"""CPU oracle for the data-parallel primitive core -- TEST INFRASTRUCTURE.
+
+Only the test suite, ``__graft_entry__.smoke()`` and the CPU-baseline leg of
+``bench.py`` may import this package: it is the checker, never the product.
+The product library (``compute_amd``) neither links nor calls it.
+
+Thin numpy/ctypes wrappers over ``oracle/build/liboracle.so`` (built from
+``bcmi_oracle.c`` by ``oracle/Makefile``).  Each C function cites the
+reference (Boost.Compute 0.5.0) file:line it restates.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+_lib = None
+
+# dtype codes (shared with the product C-ABI)
+I8, U8, I16, U16, I32, U32, I64, U64, F32, F64 = range(1, 11)
+PLUS, MULTIPLIES, MIN, MAX = 0, 1, 2, 3
+
+NP2CODE = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8, np.dtype(np.int16): I16,
+           np.dtype(np.uint16): U16, np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
+           np.dtype(np.int64): I64, np.dtype(np.uint64): U64, np.dtype(np.float32): F32,
+           np.dtype(np.float64): F64}
+
+
+def kind_of(dt):
+    """0 unsigned, 1 signed, 2 floating point."""
+    dt = np.dtype(dt)
+    if dt.kind == "f":
+        return 2
+    if dt.kind == "i":
+        return 1
+    return 0
+
+
+def build():
+    import subprocess
+    subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(_LIB_PATH):
+            build()
+        L = C.CDLL(_LIB_PATH)
+        vp, sz, i = C.c_void_p, C.c_size_t, C.c_int
+        L.orc_mt32_fill.argtypes = [C.c_uint32, vp, sz]
+        L.orc_mt64_fill.argtypes = [C.c_uint64, vp, sz]
+        L.orc_gen_sort_u32.argtypes = [C.c_uint32, vp, sz]
+        L.orc_gen_sort_by_key_f32.argtypes = [vp, vp, sz]
+        L.orc_gen_scan_i32.argtypes = [vp, sz]
+        L.orc_gen_saxpy_f32.argtypes = [vp, vp, sz]
+        L.orc_gen_lognormal_f32.argtypes = [vp, sz]
+        L.orc_gen_segments.argtypes = [vp, sz, sz]
+        L.orc_gen_segments.restype = sz
+        L.orc_gen_segment_data.argtypes = [vp, vp, sz]
+        L.orc_gen_runs_i32_f32.argtypes = [vp, vp, sz, sz]
+        L.orc_gen_runs_i32_f32.restype = sz
+        L.orc_radix_key.argtypes = [C.c_uint64, i, i, i]
+        L.orc_radix_key.restype = C.c_uint64
+        for f in ("orc_radix_order_sort", "orc_insertion_sort", "orc_compare_stable_sort"):
+            getattr(L, f).argtypes = [vp, vp, sz, i, i, i, i]
+        L.orc_sort_dispatch.argtypes = [i, vp, vp, sz, i, i, i, i]
+        L.orc_segmented_sort.argtypes = [vp, vp, vp, sz, i, i, i, i]
+        L.orc_scan.argtypes = [i, i, vp, vp, sz, vp, i]
+        L.orc_scan_f32_dd.argtypes = [vp, vp, sz, i, C.c_double]
+        L.orc_segmented_scan_i32keys.argtypes = [vp, i, vp, vp, sz, i, vp]
+        L.orc_segmented_scan_f32_dd.argtypes = [vp, vp, vp, sz, i]
+        L.orc_reduce.argtypes = [i, i, vp, sz, vp]
+        L.orc_sum_f32_dd.argtypes = [vp, sz, C.POINTER(C.c_double)]
+        L.orc_sum_f32_dd.restype = C.c_double
+        L.orc_sum_f64_dd.argtypes = [vp, sz, C.POINTER(C.c_double)]
+        L.orc_sum_f64_dd.restype = C.c_double
+        L.orc_reduce_gpu_order_f32.argtypes = [vp, sz]
+        L.orc_reduce_gpu_order_f32.restype = C.c_float
+        L.orc_reduce_cpu_order_f32.argtypes = [vp, sz, C.c_uint]
+        L.orc_reduce_cpu_order_f32.restype = C.c_float
+        L.orc_accumulate_f32.argtypes = [vp, sz, C.c_float]
+        L.orc_accumulate_f32.restype = C.c_float
+        L.orc_scan_gpu_order_f32.argtypes = [vp, vp, sz, i]
+        L.orc_reduce_by_key.argtypes = [i, i, i, vp, vp, sz, vp, vp]
+        L.orc_reduce_by_key.restype = sz
+        L.orc_reduce_by_key_f32_dd.argtypes = [vp, vp, sz, vp, vp]
+        L.orc_reduce_by_key_f32_dd.restype = sz
+        L.orc_saxpy_f32.argtypes = [C.c_float, vp, vp, vp, sz]
+        L.orc_transform.argtypes = [i, i, vp, vp, vp, sz]
+        _lib = L
+    return _lib
+
+
+def _p(a):
+    return a.ctypes.data_as(C.c_void_p) if a is not None else None
+
+
+# ------------------------------------------------------------ generators
+def mt19937(seed, n):
+    out = np.empty(n, np.uint32)
+    lib().orc_mt32_fill(seed, _p(out), n)
+    return out
+
+
+def mt19937_64(seed, n):
+    out = np.empty(n, np.uint64)
+    lib().orc_mt64_fill(seed, _p(out), n)
+    return out
+
+
+def gen_sort_u32(n, seed=0xB0057001):
+    out = np.empty(n, np.uint32)
+    lib().orc_gen_sort_u32(seed, _p(out), n)
+    return out
+
+
+def gen_sort_by_key_f32(n):
+    k = np.empty(n, np.float32)
+    v = np.empty(n, np.uint32)
+    lib().orc_gen_sort_by_key_f32(_p(k), _p(v), n)
+    return k, v
+
+
+def gen_scan_i32(n):
+    out = np.empty(n, np.int32)
+    lib().orc_gen_scan_i32(_p(out), n)
+    return out
+
+
+def gen_saxpy_f32(n):
+    x = np.empty(n, np.float32)
+    y = np.empty(n, np.float32)
+    lib().orc_gen_saxpy_f32(_p(x), _p(y), n)
+    return x, y
+
+
+def gen_lognormal_f32(n):
+    out = np.empty(n, np.float32)
+    lib().orc_gen_lognormal_f32(_p(out), n)
+    return out
+
+
+def gen_segments(nseg, max_len=1 << 20):
+    off = np.empty(nseg + 1, np.uint64)
+    lib().orc_gen_segments(_p(off), nseg, max_len)
+    return off
+
+
+def gen_segment_data(n):
+    k = np.empty(n, np.uint64)
+    v = np.empty(n, np.uint32)
+    lib().orc_gen_segment_data(_p(k), _p(v), n)
+    return k, v
+
+
+def gen_runs_i32_f32(n, max_run=4096):
+    k = np.empty(n, np.int32)
+    v = np.empty(n, np.float32)
+    lib().orc_gen_runs_i32_f32(_p(k), _p(v), n, max_run)
+    return k, v
+
+
+# ----------------------------------------------------------------- sorts
+def radix_key(bits, dtype, descending=False):
+    dt = np.dtype(dtype)
+    return lib().orc_radix_key(int(bits), dt.itemsize, kind_of(dt), int(descending))
+
+
+def _sortcall(fn, keys, vals, descending, *lead):
+    k = np.ascontiguousarray(keys).copy()
+    v = None if vals is None else np.ascontiguousarray(vals).copy()
+    fn(*lead, _p(k), _p(v), k.size, k.dtype.itemsize, kind_of(k.dtype), int(descending),
+       0 if v is None else v.dtype.itemsize)
+    return k if v is None else (k, v)
+
+
+def radix_order_sort(keys, vals=None, descending=False):
+    return _sortcall(lib().orc_radix_order_sort, keys, vals, descending)
+
+
+def insertion_sort(keys, vals=None, descending=False):
+    return _sortcall(lib().orc_insertion_sort, keys, vals, descending)
+
+
+def compare_stable_sort(keys, vals=None, descending=False):
+    return _sortcall(lib().orc_compare_stable_sort, keys, vals, descending)
+
+
+SORT, SORT_BY_KEY, STABLE_SORT, STABLE_SORT_BY_KEY = 0, 1, 2, 3
+
+
+def sort_dispatch(api, keys, vals=None, descending=False):
+    return _sortcall(lib().orc_sort_dispatch, keys, vals, descending, api)
+
+
+def segmented_sort(keys, vals, offsets, descending=False):
+    k = keys.copy()
+    v = None if vals is None else vals.copy()
+    off = np.ascontiguousarray(offsets, np.uint64)
+    lib().orc_segmented_sort(_p(k), _p(v), _p(off), off.size - 1, k.dtype.itemsize,
+                             kind_of(k.dtype), int(descending),
+                             0 if v is None else v.dtype.itemsize)
+    return k if v is None else (k, v)
+
+
+# ------------------------------------------------------------------ scans
+def scan(x, exclusive, init=0, op=PLUS):
+    x = np.ascontiguousarray(x)
+    out = np.empty_like(x)
+    ini = np.array([init], dtype=x.dtype)
+    lib().orc_scan(NP2CODE[x.dtype], op, _p(x), _p(out), x.size, _p(ini), int(exclusive))
+    return out
+
+
+def scan_f32_dd(x, exclusive, init=0.0):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty(x.size, np.float64)
+    lib().orc_scan_f32_dd(_p(x), _p(out), x.size, int(exclusive), float(init))
+    return out
+
+
+def segmented_scan(keys, vals, exclusive, init=0):
+    keys = np.ascontiguousarray(keys, np.int32)
+    vals = np.ascontiguousarray(vals)
+    out = np.empty_like(vals)
+    ini = np.array([init], dtype=vals.dtype)
+    lib().orc_segmented_scan_i32keys(_p(keys), NP2CODE[vals.dtype], _p(vals), _p(out),
+                                     vals.size, int(exclusive), _p(ini))
+    return out
+
+
+def segmented_scan_f32_dd(keys, vals, exclusive):
+    keys = np.ascontiguousarray(keys, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    out = np.empty(vals.size, np.float64)
+    lib().orc_segmented_scan_f32_dd(_p(keys), _p(vals), _p(out), vals.size, int(exclusive))
+    return out
+
+
+# --------------------------------------------------------------- reduce
+def reduce(x, op=PLUS):
+    x = np.ascontiguousarray(x)
+    out = np.empty(1, x.dtype)
+    lib().orc_reduce(NP2CODE[x.dtype], op, _p(x), x.size, _p(out))
+    return out[0]
+
+
+def sum_dd(x):
+    """Double-double sum: returns (hi, lo)."""
+    lo = C.c_double()
+    if x.dtype == np.float32:
+        hi = lib().orc_sum_f32_dd(_p(np.ascontiguousarray(x)), x.size, C.byref(lo))
+    else:
+        hi = lib().orc_sum_f64_dd(_p(np.ascontiguousarray(x, np.float64)), x.size, C.byref(lo))
+    return hi, lo.value
+
+
+def reduce_gpu_order_f32(x):
+    return np.float32(lib().orc_reduce_gpu_order_f32(_p(np.ascontiguousarray(x, np.float32)), x.size))
+
+
+def reduce_cpu_order_f32(x, cu):
+    return np.float32(lib().orc_reduce_cpu_order_f32(_p(np.ascontiguousarray(x, np.float32)), x.size, cu))
+
+
+def accumulate_f32(x, init=0.0):
+    return np.float32(lib().orc_accumulate_f32(_p(np.ascontiguousarray(x, np.float32)), x.size, init))
+
+
+def scan_gpu_order_f32(x, exclusive):
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty_like(x)
+    lib().orc_scan_gpu_order_f32(_p(x), _p(out), x.size, int(exclusive))
+    return out
+
+
+def reduce_by_key(keys, vals, op=PLUS):
+    keys = np.ascontiguousarray(keys)
+    vals = np.ascontiguousarray(vals)
+    ko = np.empty_like(keys)
+    vo = np.empty_like(vals)
+    r = lib().orc_reduce_by_key(NP2CODE[keys.dtype], NP2CODE[vals.dtype], op, _p(keys), _p(vals),
+                                keys.size, _p(ko), _p(vo))
+    return ko[:r], vo[:r]
+
+
+def reduce_by_key_f32_dd(keys, vals):
+    keys = np.ascontiguousarray(keys, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    ko = np.empty_like(keys)
+    so = np.empty(keys.size, np.float64)
+    r = lib().orc_reduce_by_key_f32_dd(_p(keys), _p(vals), keys.size, _p(ko), _p(so))
+    return ko[:r], so[:r]
+
+
+def saxpy(a, x, y):
+    x = np.ascontiguousarray(x, np.float32)
+    y = np.ascontiguousarray(y, np.float32)
+    out = np.empty_like(x)
+    lib().orc_saxpy_f32(a, _p(x), _p(y), _p(out), x.size)
+    return out
+
+
+def transform(a, b, op):
+    a = np.ascontiguousarray(a)
+    b = np.ascontiguousarray(b, a.dtype)
+    out = np.empty_like(a)
+    lib().orc_transform(NP2CODE[a.dtype], op, _p(a), _p(b), _p(out), a.size)
+    return out

@@ -1,0 +1,112 @@
+/*
+ * bcmi_oracle.h -- CPU oracle for the data-parallel primitive core.
+ *
+ * TEST INFRASTRUCTURE ONLY.  This library is the checker: only the test
+ * suite, the smoke check and the CPU-baseline leg of the benchmark may load
+ * it.  Nothing in the product library (compute_amd/) links or calls it.
+ *
+ * Every function restates the reference's algorithm (Boost.Compute 0.5.0,
+ * include/boost/compute/...) in plain C; each definition cites the file:line
+ * it follows.  Float helpers that have no reference counterpart (double-double
+ * sums, used as the FP64 oracle) say so.
+ *
+ * How each function is pinned (known-answer tests, the reference's own golden
+ * vectors, closed forms, brute force) is listed in tests/test_oracle.py.
+ */
+#ifndef BCMI_ORACLE_H
+#define BCMI_ORACLE_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ---------------------------------------------------------------- RNG -- */
+typedef struct { uint32_t mt[624]; int idx; } orc_mt32;
+typedef struct { uint64_t mt[312]; int idx; } orc_mt64;
+
+void     orc_mt32_seed(orc_mt32 *g, uint32_t seed);
+uint32_t orc_mt32_next(orc_mt32 *g);
+void     orc_mt64_seed(orc_mt64 *g, uint64_t seed);
+uint64_t orc_mt64_next(orc_mt64 *g);
+/* n raw outputs of std::mt19937(seed) / std::mt19937_64(seed) */
+void orc_mt32_fill(uint32_t seed, uint32_t *out, size_t n);
+void orc_mt64_fill(uint64_t seed, uint64_t *out, size_t n);
+
+/* ------------------------------------------------ workload generators -- */
+void   orc_gen_sort_u32(uint32_t seed, uint32_t *keys, size_t n);
+void   orc_gen_sort_by_key_f32(float *keys, uint32_t *vals, size_t n);
+void   orc_gen_scan_i32(int32_t *x, size_t n);
+void   orc_gen_saxpy_f32(float *x, float *y, size_t n);
+void   orc_gen_lognormal_f32(float *x, size_t n);
+size_t orc_gen_segments(uint64_t *offsets, size_t nseg, size_t max_len);
+void   orc_gen_segment_data(uint64_t *keys, uint32_t *vals, size_t n);
+size_t orc_gen_runs_i32_f32(int32_t *keys, float *vals, size_t n, size_t max_run);
+
+/* --------------------------------------------------------- sort paths -- */
+/* kind: 0 unsigned, 1 signed, 2 floating point (IEEE binary32/64) */
+uint64_t orc_radix_key(uint64_t bits, int kbytes, int kind, int descending);
+/* stable sort by the radix key: the GPU column of sort/sort_by_key/stable_sort */
+void orc_radix_order_sort(void *keys, void *vals, size_t n, int kbytes,
+                          int kind, int descending, int vbytes);
+/* one-work-item insertion sort with operator< / operator> on the native type */
+void orc_insertion_sort(void *keys, void *vals, size_t n, int kbytes,
+                        int kind, int descending, int vbytes);
+/* public sort/sort_by_key/stable_sort/stable_sort_by_key dispatch (GPU column)
+ * api: 0 sort, 1 sort_by_key, 2 stable_sort, 3 stable_sort_by_key */
+void orc_sort_dispatch(int api, void *keys, void *vals, size_t n, int kbytes,
+                       int kind, int descending, int vbytes);
+/* stable comparison sort (merge sort, operator<): CPU-device column */
+void orc_compare_stable_sort(void *keys, void *vals, size_t n, int kbytes,
+                             int kind, int descending, int vbytes);
+void orc_segmented_sort(void *keys, void *vals, const uint64_t *offsets,
+                        size_t nseg, int kbytes, int kind, int descending,
+                        int vbytes);
+
+/* --------------------------------------------------------------- scans -- */
+/* dtype codes shared with the product C-ABI */
+enum { ORC_I8 = 1, ORC_U8, ORC_I16, ORC_U16, ORC_I32, ORC_U32, ORC_I64, ORC_U64,
+       ORC_F32, ORC_F64 };
+enum { ORC_PLUS = 0, ORC_MULTIPLIES = 1, ORC_MIN = 2, ORC_MAX = 3 };
+/* sequential left-fold scan in the element type (wrapping for integers) */
+void orc_scan(int dtype, int op, const void *in, void *out, size_t n,
+              const void *init, int exclusive);
+/* float scans: double-double accurate prefix sums rounded to double */
+void orc_scan_f32_dd(const float *in, double *out, size_t n, int exclusive,
+                     double init);
+/* segmented (by adjacent-equal int32 keys) exclusive/inclusive plus-scan */
+void orc_segmented_scan_i32keys(const int32_t *keys, int vdtype,
+                                const void *vals, void *out, size_t n,
+                                int exclusive, const void *init);
+void orc_segmented_scan_f32_dd(const int32_t *keys, const float *vals,
+                               double *out, size_t n, int exclusive);
+
+/* ------------------------------------------------------------- reduce -- */
+void   orc_reduce(int dtype, int op, const void *in, size_t n, void *out);
+double orc_sum_f32_dd(const float *x, size_t n, double *lo);
+double orc_sum_f64_dd(const double *x, size_t n, double *lo);
+float  orc_reduce_gpu_order_f32(const float *x, size_t n);
+float  orc_reduce_cpu_order_f32(const float *x, size_t n, unsigned cu);
+float  orc_accumulate_f32(const float *x, size_t n, float init);
+float  orc_scan_gpu_order_f32_last(const float *x, size_t n);
+void   orc_scan_gpu_order_f32(const float *x, float *out, size_t n, int exclusive);
+
+/* ------------------------------------------------------ reduce_by_key -- */
+size_t orc_reduce_by_key(int kdtype, int vdtype, int op, const void *keys,
+                         const void *vals, size_t n, void *keys_out,
+                         void *vals_out);
+size_t orc_reduce_by_key_f32_dd(const int32_t *keys, const float *vals,
+                                size_t n, int32_t *keys_out, double *sums);
+
+/* ---------------------------------------------------------- transform -- */
+void orc_saxpy_f32(float a, const float *x, const float *y, float *out,
+                   size_t n);
+void orc_transform(int dtype, int op, const void *a, const void *b, void *out,
+                   size_t n);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,261 @@
+"""Writes reference_vectors.json: the hot-path golden vectors that Boost.Compute
+0.5.0's own unit tests hold, transcribed as data (inputs and expected outputs).
+
+Each case cites the reference test (path:line relative to the reference root)
+it comes from.  "cmp": "eq" marks expectations the reference compares with
+operator== (BOOST_CHECK_EQUAL_COLLECTIONS, test/check_macros.hpp:19-31), which
+cannot tell -0.0 from +0.0; those cases are checked by value equality, and the
+bit-level order is checked against the oracle separately.
+
+Run:  python tests/golden/make_reference_vectors.py
+"""
+import json
+import os
+
+CASES = []
+
+
+def case(cid, src, algo, dtype, **kw):
+    d = {"id": cid, "source": src, "algo": algo, "dtype": dtype}
+    d.update(kw)
+    CASES.append(d)
+
+
+# ------------------------------------------------------------ radix_sort
+CHARS = [ord(c) for c in "ca07BF"] + [0, ord("$")]
+radix_vectors = [
+    ("char", "int8", CHARS, [0, ord("$"), ord("0"), ord("7"), ord("B"), ord("F"), ord("a"), ord("c")],
+     [ord("c"), ord("a"), ord("F"), ord("B"), ord("7"), ord("0"), ord("$"), 0], 41, 237),
+    ("uchar", "uint8", [0x12, 0x00, 0xFF, 0xB4, 0x80, 0x32, 0x64, 0xA2],
+     [0x00, 0x12, 0x32, 0x64, 0x80, 0xA2, 0xB4, 0xFF],
+     [0xFF, 0xB4, 0xA2, 0x80, 0x64, 0x32, 0x12, 0x00], 62, 270),
+    ("short", "int16", [-4, 152, -94, 963, 31002, -456, 0, -2113],
+     [-2113, -456, -94, -4, 0, 152, 963, 31002],
+     [31002, 963, 152, 0, -4, -94, -456, -2113], 79, 303),
+    ("ushort", "uint16", [4, 152, 94, 963, 63202, 34560, 0, 2113],
+     [0, 4, 94, 152, 963, 2113, 34560, 63202],
+     [63202, 34560, 2113, 963, 152, 94, 4, 0], 96, 336),
+    ("int", "int32", [-4, 152, -5000, 963, 75321, -456, 0, 1112],
+     [-5000, -456, -4, 0, 152, 963, 1112, 75321],
+     [75321, 1112, 963, 152, 0, -4, -456, -5000], 113, 369),
+    ("uint", "uint32", [500, 1988, 123456, 562, 0, 4000000, 9852, 102030],
+     [0, 500, 562, 1988, 9852, 102030, 123456, 4000000],
+     [4000000, 123456, 102030, 9852, 1988, 562, 500, 0], 128, 402),
+    ("long", "int64", [500, 1988, 123456, 562, 0, 4000000, 9852, 102030],
+     [0, 500, 562, 1988, 9852, 102030, 123456, 4000000], None, 145, None),
+    ("ulong", "uint64", [500, 1988, 123456, 562, 0, 4000000, 9852, 102030],
+     [0, 500, 562, 1988, 9852, 102030, 123456, 4000000],
+     [4000000, 123456, 102030, 9852, 1988, 562, 500, 0], 162, 468),
+]
+for name, dt, inp, asc, desc, line_a, line_d in radix_vectors:
+    case("radix_sort_%s" % name, "test/test_radix_sort.cpp:%d" % line_a, "radix_sort", dt,
+         input=inp, expected=asc, descending=False)
+    if desc is not None:
+        case("radix_sort_%s_desc" % name, "test/test_radix_sort.cpp:%d" % line_d, "radix_sort", dt,
+             input=inp, expected=desc, descending=True)
+case("radix_sort_long_desc", "test/test_radix_sort.cpp:435", "radix_sort", "int64",
+     input=[-500, 1988, 123456, 562, 0, 4000000, 9852, 102030],
+     expected=[4000000, 123456, 102030, 9852, 1988, 562, 0, -500], descending=True)
+FLOATS = [-6023.0, 152.5, -63.0, 1234567.0, 11.2, -5000.1, 0.0, 14.0, -8.25, -0.0]
+case("radix_sort_float", "test/test_radix_sort.cpp:179", "radix_sort", "float32", input=FLOATS,
+     expected=[-6023.0, -5000.1, -63.0, -8.25, -0.0, 0.0, 11.2, 14.0, 152.5, 1234567.0],
+     descending=False, cmp="eq")
+case("radix_sort_float_desc", "test/test_radix_sort.cpp:481", "radix_sort", "float32", input=FLOATS,
+     expected=[1234567.0, 152.5, 14.0, 11.2, 0.0, -0.0, -8.25, -63.0, -5000.1, -6023.0],
+     descending=True, cmp="eq")
+case("radix_sort_double", "test/test_radix_sort.cpp:205", "radix_sort", "float64", input=FLOATS,
+     expected=[-6023.0, -5000.1, -63.0, -8.25, -0.0, 0.0, 11.2, 14.0, 152.5, 1234567.0],
+     descending=False, cmp="eq")
+case("radix_sort_double_desc", "test/test_radix_sort.cpp:511", "radix_sort", "float64", input=FLOATS,
+     expected=[1234567.0, 152.5, 14.0, 11.2, 0.0, -0.0, -8.25, -63.0, -5000.1, -6023.0],
+     descending=True, cmp="eq")
+# partial range vec.begin()+2 .. end()-2 (test/test_radix_sort.cpp:530-540)
+case("radix_sort_partial_range", "test/test_radix_sort.cpp:530", "radix_sort_range", "int32",
+     input=[9, 8, 7, 6, 5, 4, 3, 2, 1, 0], first=2, last=8,
+     expected=[9, 8, 2, 3, 4, 5, 6, 7, 1, 0], descending=False)
+
+# ------------------------------------------------------ radix_sort_by_key
+KI = [10, 9, 2, 7, 6, -1, 4, 2, 2, 10]
+VI = [1, 2, 3, 4, 5, 6, 7, 8, 9, 10]
+case("radix_sort_by_key_int_int", "test/test_radix_sort_by_key.cpp:40", "radix_sort_by_key", "int32",
+     vdtype="int32", keys=KI, values=VI, expected_keys=[-1, 2, 2, 2, 4, 6, 7, 9, 10, 10],
+     expected_values=[6, 3, 8, 9, 7, 5, 4, 2, 1, 10], descending=False)
+case("radix_sort_by_key_int_int_desc", "test/test_radix_sort_by_key.cpp:69", "radix_sort_by_key",
+     "int32", vdtype="int32", keys=KI, values=VI, expected_keys=[10, 10, 9, 7, 6, 4, 2, 2, 2, -1],
+     expected_values=[1, 10, 2, 4, 5, 7, 3, 8, 9, 6], descending=True)
+KU = [10, 9, 2, 7, 6, 1, 4, 2, 2, 10]
+case("radix_sort_by_key_uint_uint", "test/test_radix_sort_by_key.cpp:108", "radix_sort_by_key",
+     "uint32", vdtype="uint32", keys=KU, values=VI, expected_keys=[1, 2, 2, 2, 4, 6, 7, 9, 10, 10],
+     expected_values=[6, 3, 8, 9, 7, 5, 4, 2, 1, 10], descending=False)
+case("radix_sort_by_key_uint_uint_desc", "test/test_radix_sort_by_key.cpp:137", "radix_sort_by_key",
+     "uint32", vdtype="uint32", keys=KU, values=VI, expected_keys=[10, 10, 9, 7, 6, 4, 2, 2, 2, 1],
+     expected_values=[1, 10, 2, 4, 5, 7, 3, 8, 9, 6], descending=True)
+KF = [10.0, 5.5, 10.0, 7.0, 5.5]
+VF = [1, 200, -10, 2, 4]
+case("radix_sort_by_key_float_int", "test/test_radix_sort_by_key.cpp:177", "radix_sort_by_key",
+     "float32", vdtype="int32", keys=KF, values=VF, expected_keys=[5.5, 5.5, 7.0, 10.0, 10.0],
+     expected_values=[200, 4, 2, 1, -10], descending=False)
+case("radix_sort_by_key_float_int_desc", "test/test_radix_sort_by_key.cpp:206", "radix_sort_by_key",
+     "float32", vdtype="int32", keys=KF, values=VF, expected_keys=[10.0, 10.0, 7.0, 5.5, 5.5],
+     expected_values=[1, -10, 2, 200, 4], descending=True)
+case("radix_sort_by_key_int_char", "test/test_radix_sort_by_key.cpp:246", "radix_sort_by_key", "int32",
+     vdtype="int8", keys=[6, 1, 1, 3, 4, 7, 5, 1], values=[ord(c) for c in "gcbdehfa"],
+     expected_keys=[1, 1, 1, 3, 4, 5, 6, 7], expected_values=[ord(c) for c in "cbadefgh"],
+     descending=False)
+# int2_ values (8 bytes): (1, i) packed as int64 with the pair's memory layout
+int2 = [(1, i) for i in range(1, 9)]
+packed = [((y & 0xFFFFFFFF) << 32) | (x & 0xFFFFFFFF) for x, y in int2]
+order = [1, 2, 7, 3, 4, 6, 0, 5]
+case("radix_sort_by_key_int_int2", "test/test_radix_sort_by_key.cpp:273", "radix_sort_by_key", "int32",
+     vdtype="uint64", keys=[6, 1, 1, 3, 4, 7, 5, 1], values=packed,
+     expected_keys=[1, 1, 1, 3, 4, 5, 6, 7], expected_values=[packed[i] for i in order],
+     descending=False)
+
+# -------------------------------------------------------------- public sort
+case("sort_int_2", "test/test_sort.cpp:52", "sort", "int32", input=[4, 2], expected=[2, 4])
+case("sort_float_3", "test/test_sort.cpp:71", "sort", "float32", input=[2.3, 0.1, 1.2],
+     expected=[0.1, 1.2, 2.3])
+for name, dt, inp, asc, desc, line_a, line_d in radix_vectors:
+    line = {"char": 79, "uchar": 93, "short": 107, "ushort": 121, "int": 135, "uint": 147,
+            "long": 161, "ulong": 175}[name]
+    case("sort_%s" % name, "test/test_sort.cpp:%d" % line, "sort", dt, input=inp, expected=asc)
+case("sort_float", "test/test_sort.cpp:189", "sort", "float32", input=FLOATS,
+     expected=[-6023.0, -5000.1, -63.0, -8.25, -0.0, 0.0, 11.2, 14.0, 152.5, 1234567.0], cmp="eq")
+case("sort_double", "test/test_sort.cpp:205", "sort", "float64", input=FLOATS,
+     expected=[-6023.0, -5000.1, -63.0, -8.25, -0.0, 0.0, 11.2, 14.0, 152.5, 1234567.0], cmp="eq")
+case("reverse_sort_int", "test/test_sort.cpp:227", "sort", "int32",
+     input=[-4, 152, -5000, 963, 75321, -456, 0, 1112],
+     expected=[75321, 1112, 963, 152, 0, -4, -456, -5000], descending=True)
+case("sort_host_vector", "test/test_sort.cpp:285", "sort", "int32", input=[5, 2, 3, 6, 7, 4, 0, 1],
+     expected=[0, 1, 2, 3, 4, 5, 6, 7])
+
+# ------------------------------------------------------------ sort_by_key
+case("sort_by_key_int_2", "test/test_sort_by_key.cpp:56", "sort_by_key", "int32", vdtype="int32",
+     keys=[4, 2], values=[42, 24], expected_keys=[2, 4], expected_values=[24, 42])
+case("sort_by_key_char_by_int", "test/test_sort_by_key.cpp:70", "sort_by_key", "int32", vdtype="int8",
+     keys=[6, 2, 1, 3, 4, 7, 5, 0], values=[ord(c) for c in "gcbdehfa"],
+     expected_keys=list(range(8)), expected_values=[ord(c) for c in "abcdefgh"])
+n = 1024
+case("sort_by_key_int_and_float", "test/test_sort_by_key.cpp:82", "sort_by_key", "int32",
+     vdtype="float32", keys=[n - i for i in range(n)], values=[(n - i) / 2.0 for i in range(n)],
+     expected_keys=[i + 1 for i in range(n)], expected_values=[(i + 1) / 2.0 for i in range(n)])
+# stable_sort_by_key tie order (test/test_stable_sort_by_key.cpp:165-204)
+size = 128
+k = [-i for i in range(size)]
+v = [-i for i in range(size)]
+k[size // 2] = -256
+k[size - 2] = -256
+k[size - 1] = -256
+v[size // 2] = 3
+v[size - 2] = 1
+v[size - 1] = 2
+case("stable_sort_by_key_mid", "test/test_stable_sort_by_key.cpp:165", "stable_sort_by_key_prefix",
+     "int32", vdtype="int32", keys=k, values=v, expected_keys_prefix=[-256, -256, -256],
+     expected_values_prefix=[3, 1, 2])
+
+# ------------------------------------------------------------------- scans
+d12 = list(range(12))
+case("inclusive_scan_int", "test/test_scan.cpp:41", "inclusive_scan", "int32", input=d12,
+     expected=[0, 1, 3, 6, 10, 15, 21, 28, 36, 45, 55, 66])
+case("exclusive_scan_int", "test/test_scan.cpp:93", "exclusive_scan", "int32", input=d12, init=0,
+     expected=[0, 0, 1, 3, 6, 10, 15, 21, 28, 36, 45, 55])
+case("inclusive_scan_int_multiplies", "test/test_scan.cpp:329", "inclusive_scan", "int32",
+     input=[1, 2, 1, 2, 3], op="multiplies", expected=[1, 2, 2, 4, 12])
+case("exclusive_scan_int_multiplies", "test/test_scan.cpp:359", "exclusive_scan", "int32",
+     input=[1, 2, 1, 2, 3], init=10, op="multiplies", expected=[10, 10, 20, 20, 40])
+
+
+def wrap32(x):
+    x &= 0xFFFFFFFF
+    return x - (1 << 32) if x >= (1 << 31) else x
+
+
+acc, inc = 1, []
+for _ in range(1000):
+    acc = wrap32(acc * 2)
+    inc.append(acc)
+case("inclusive_scan_int_multiplies_long", "test/test_scan.cpp:392", "inclusive_scan", "int32",
+     input=[2] * 1000, op="multiplies", expected=inc)
+acc, exc = 10, []
+for _ in range(1000):
+    exc.append(acc)
+    acc = wrap32(acc * 2)
+case("exclusive_scan_int_multiplies_long", "test/test_scan.cpp:413", "exclusive_scan", "int32",
+     input=[2] * 1000, init=10, op="multiplies", expected=exc)
+case("partial_sum_int", "test/test_partial_sum.cpp:28", "inclusive_scan", "int32",
+     input=[1, 2, 5, 3, 9, 1, 4, 2], expected=[1, 3, 8, 11, 20, 21, 25, 27])
+
+# ----------------------------------------------------------------- reduce
+case("reduce_int_plus", "test/test_reduce.cpp:29", "reduce", "int32", input=[1, 5, 9, 13, 17],
+     op="plus", expected=45)
+case("reduce_int_multiplies", "test/test_reduce.cpp:29", "reduce", "int32", input=[1, 5, 9, 13, 17],
+     op="multiplies", expected=9945)
+case("reduce_doctest", "test/test_reduce.cpp:50", "reduce", "int32", input=[1, 2, 3, 4], op="plus",
+     expected=10)
+case("reduce_twos_plus", "test/test_reduce.cpp:62", "reduce", "uint32", input=[2] * 8, op="plus",
+     expected=16)
+case("reduce_twos_multiplies", "test/test_reduce.cpp:62", "reduce", "uint32", input=[2] * 8,
+     op="multiplies", expected=256)
+case("reduce_on_device_lo", "test/test_reduce.cpp:77", "reduce", "int32", input=[1, 2, 3, 4],
+     op="plus", expected=10)
+case("reduce_on_device_hi", "test/test_reduce.cpp:77", "reduce", "int32", input=[5, 6, 7, 8],
+     op="plus", expected=26)
+case("reduce_int_min", "test/test_reduce.cpp:88", "reduce", "int32", input=[11, 5, 92, 13, 42],
+     op="min", expected=5)
+case("reduce_int_max", "test/test_reduce.cpp:88", "reduce", "int32", input=[11, 5, 92, 13, 42],
+     op="max", expected=92)
+
+# ------------------------------------------------------------- accumulate
+for init, exp in ((0, 20), (-10, 10), (5, 25)):
+    case("accumulate_sum_int_init%d" % init, "test/test_accumulate.cpp:25", "accumulate", "int32",
+         input=[2, 4, 6, 8], init=init, op="plus", expected=exp)
+for init, exp in ((1, 384), (-1, -384), (2, 768)):
+    case("accumulate_product_int_init%d" % init, "test/test_accumulate.cpp:42", "accumulate", "int32",
+         input=[2, 4, 6, 8], init=init, op="multiplies", expected=exp)
+case("accumulate_std_equality_int", "test/test_accumulate.cpp:268", "accumulate", "int32",
+     input=[1, 2, 3, 4], init=0, op="plus", expected=10)
+case("accumulate_std_equality_int_10000", "test/test_accumulate.cpp:272", "accumulate", "int32",
+     input=[2] * 10000, init=0, op="plus", expected=20000)
+
+# ----------------------------------------------------------- reduce_by_key
+case("reduce_by_key_int", "test/test_reduce_by_key.cpp:26", "reduce_by_key", "int32", vdtype="int32",
+     keys=[0, 2, -3, -3, -3, -3, -3, 4], values=[1, 1, 1, 1, 1, 2, 5, 1], op="plus",
+     expected_keys=[0, 2, -3, 4], expected_values=[1, 1, 10, 1])
+keys = [0] * 1024
+for i in (137, 677, 1001):
+    keys[i] = 1
+run = 0
+scanned = []
+for x in keys:
+    run += x
+    scanned.append(run)
+case("reduce_by_key_int_long_uniform", "test/test_reduce_by_key.cpp:48", "reduce_by_key", "int32",
+     vdtype="int32", keys=[0] * 1024, values=[1] * 1024, op="plus", expected_keys=[0],
+     expected_values=[1024])
+case("reduce_by_key_int_long_runs", "test/test_reduce_by_key.cpp:62", "reduce_by_key", "int32",
+     vdtype="int32", keys=scanned, values=[1] * 1024, op="plus", expected_keys=[0, 1, 2, 3],
+     expected_values=[137, 540, 324, 23])
+case("reduce_by_key_one", "test/test_reduce_by_key.cpp:88", "reduce_by_key", "int32", vdtype="int32",
+     keys=[22], values=[-9], op="plus", expected_keys=[22], expected_values=[-9])
+KM = [0, 2, 2, 3, 3, 3, 3, 3, 4]
+case("reduce_by_key_int_min", "test/test_reduce_by_key.cpp:104", "reduce_by_key", "int32",
+     vdtype="int32", keys=KM, values=[1, 2, 1, -3, 1, 4, 2, 5, 77], op="min",
+     expected_keys=[0, 2, 3, 4], expected_values=[1, 1, -3, 77])
+case("reduce_by_key_int_max", "test/test_reduce_by_key.cpp:104", "reduce_by_key", "int32",
+     vdtype="int32", keys=KM, values=[1, 2, 1, -3, 1, 4, 2, 5, 77], op="max",
+     expected_keys=[0, 2, 3, 4], expected_values=[1, 2, 5, 77])
+case("reduce_by_key_float_max", "test/test_reduce_by_key.cpp:131", "reduce_by_key", "int32",
+     vdtype="float32", keys=KM, values=[1.0, 2.0, -1.5, -3.0, 1.0, -0.24, 2, 5, 77.1], op="max",
+     expected_keys=[0, 2, 3, 4], expected_values=[1.0, 2.0, 5.0, 77.1], rtol=1e-6)
+
+# -------------------------------------------------------------- transform
+case("transform_add_int", "test/test_transform.cpp:75", "transform", "int32", a=[1, 2, 3, 4],
+     b=[10, 20, 30, 40], op="plus", expected=[11, 22, 33, 44])
+case("transform_mul_int", "test/test_transform.cpp:75", "transform", "int32", a=[1, 2, 3, 4],
+     b=[10, 20, 30, 40], op="multiplies", expected=[10, 40, 90, 160])
+
+if __name__ == "__main__":
+    out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.json")
+    with open(out, "w") as f:
+        json.dump({"reference": "Boost.Compute 0.5.0", "cases": CASES}, f, indent=1)
+    print("wrote %d cases to %s" % (len(CASES), out))

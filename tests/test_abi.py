@@ -1,0 +1,70 @@
+"""C-ABI boundary checks that need no GPU: the library loads, exports every
+entry point include/bcmi.h declares, and its host-side logic (errors, device
+selection, tuning cache) behaves.  No kernel is launched here."""
+import ctypes as C
+import os
+import re
+import subprocess
+
+import pytest
+
+import compute_amd as ca
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_entry_points():
+    src = open(os.path.join(ROOT, "include", "bcmi.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bcmi_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_every_declared_entry_point():
+    names = declared_entry_points()
+    assert len(names) >= 35
+    L = ca.lib()
+    missing = [n for n in names if not hasattr(L, n)]
+    assert not missing, missing
+    out = subprocess.check_output(["nm", "-D", "--defined-only", ca.LIB_PATH], text=True)
+    exported = set(re.findall(r" T (bcmi_\w+)", out))
+    assert set(names) <= exported
+    assert set(names) == set(ca.EXPORTS)
+
+
+def test_no_gpu_path_fails_loudly():
+    n = ca.device_count()
+    with pytest.raises(ca.BcmiError) as e:
+        ca.Queue(n + 3)  # a device that does not exist
+    assert e.value.code == -1  # CL_DEVICE_NOT_FOUND
+    assert "not found" in str(e.value)
+
+
+def test_null_arguments_are_rejected():
+    L = ca.lib()
+    assert L.bcmi_device_count(None) == -30
+    assert L.bcmi_queue_finish(None) == -36
+    assert L.bcmi_malloc(16, None) == -30
+    assert b"null" in L.bcmi_last_error()
+
+
+def test_tuning_cache_round_trip(tmp_path):
+    L = ca.lib()
+    ca.check(L.bcmi_tune_set(b"__bcmi_radix_sort_uint", b"ipt", 12))
+    v = C.c_uint()
+    ca.check(L.bcmi_tune_get(b"__bcmi_radix_sort_uint", b"ipt", 16, C.byref(v)))
+    assert v.value == 12
+    ca.check(L.bcmi_tune_get(b"__bcmi_radix_sort_uint", b"tpb", 256, C.byref(v)))
+    assert v.value == 256
+    path = str(tmp_path / "tune.json").encode()
+    ca.check(L.bcmi_tuning_cache_save(path))
+    ca.check(L.bcmi_tune_set(b"__bcmi_radix_sort_uint", b"ipt", 8))
+    ca.check(L.bcmi_tuning_cache_load(path))
+    ca.check(L.bcmi_tune_get(b"__bcmi_radix_sort_uint", b"ipt", 16, C.byref(v)))
+    assert v.value == 12
+    assert L.bcmi_tuning_cache_load(str(tmp_path / "missing.json").encode()) == -30
+
+
+def test_version():
+    a, b = C.c_int(), C.c_int()
+    ca.check(ca.lib().bcmi_get_version(C.byref(a), C.byref(b)))
+    assert (a.value, b.value) == (0, 5)

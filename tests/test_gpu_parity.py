@@ -1,0 +1,508 @@
+"""Parity of the HIP path (through the C-ABI) with the oracle and with the
+reference's own golden vectors.
+
+Bar: bit-exact for sorts, integer scans/reductions and reduce_by_key keys;
+float results within the stated bound of the double-double oracle (the
+reference's float association is not reproduced)."""
+import math
+
+import numpy as np
+import pytest
+
+import compute_amd as ca
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RNG = np.random.default_rng(20261202)
+OPS = {"plus": ca.PLUS, "multiplies": ca.MULTIPLIES, "min": ca.MIN, "max": ca.MAX}
+OOPS = {"plus": O.PLUS, "multiplies": O.MULTIPLIES, "min": O.MIN, "max": O.MAX}
+
+
+def dev(q, a):
+    return ca.DeviceArray.from_host(np.ascontiguousarray(a), q)
+
+
+def bits(a):
+    a = np.ascontiguousarray(a)
+    return a.view({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[a.itemsize])
+
+
+def _np(vals, dt):
+    if dt.startswith("float"):
+        return np.array(vals, dtype=dt)
+    return np.array(vals, dtype=np.int64).astype(dt)
+
+
+def random_keys(dt, n, rng=RNG):
+    dt = np.dtype(dt)
+    if dt.kind == "f":
+        k = (rng.normal(size=n) * 10.0 ** rng.integers(-3, 4, n)).astype(dt)
+        if n > 8:
+            sel = rng.integers(0, n, max(1, n // 50))
+            k[sel] = rng.choice(np.array([0.0, -0.0, np.inf, -np.inf, np.nan, -np.nan, 1.0], dt),
+                                sel.size)
+            k[rng.integers(0, n, max(1, n // 20))] = k[rng.integers(0, n, max(1, n // 20))]
+        return k
+    info = np.iinfo(dt)
+    k = rng.integers(info.min, info.max, n, endpoint=True, dtype=dt)
+    if n > 8:  # duplicates and extremes
+        k[rng.integers(0, n, n // 10)] = k[rng.integers(0, n, n // 10)]
+        k[rng.integers(0, n, 3)] = info.min
+        k[rng.integers(0, n, 3)] = info.max
+    return k
+
+
+# =========================================================== golden vectors
+def test_golden_sorts(queue, golden):
+    for c in golden:
+        if c["algo"] not in ("radix_sort", "sort", "radix_sort_range"):
+            continue
+        x = _np(c["input"], c["dtype"])
+        d = dev(queue, x)
+        desc = c.get("descending", False)
+        if c["algo"] == "radix_sort_range":
+            f, l = c["first"], c["last"]
+            ca.sort(queue, d, api=ca.RADIX_SORT, n=l - f, key_ptr=d.at(f), descending=desc)
+        else:
+            api = ca.RADIX_SORT if c["algo"] == "radix_sort" else ca.SORT
+            ca.sort(queue, d, api=api, descending=desc)
+        got = d.to_host(queue)
+        exp = _np(c["expected"], c["dtype"])
+        assert np.array_equal(got, exp), c["id"]
+        # and bit-for-bit the oracle's order (signed zeros included)
+        if c["algo"] == "radix_sort":
+            assert np.array_equal(bits(got), bits(O.radix_order_sort(x, descending=desc))), c["id"]
+        elif c["algo"] == "sort":
+            assert np.array_equal(bits(got), bits(O.sort_dispatch(O.SORT, x, descending=desc))), c["id"]
+
+
+def test_golden_sort_by_key(queue, golden):
+    for c in golden:
+        if c["algo"] not in ("radix_sort_by_key", "sort_by_key", "stable_sort_by_key_prefix"):
+            continue
+        k = _np(c["keys"], c["dtype"])
+        v = _np(c["values"], c["vdtype"])
+        dk, dv = dev(queue, k), dev(queue, v)
+        api = {"radix_sort_by_key": ca.RADIX_SORT, "sort_by_key": ca.SORT_BY_KEY,
+               "stable_sort_by_key_prefix": ca.STABLE_SORT_BY_KEY}[c["algo"]]
+        ca.sort(queue, dk, dv, api=api, descending=c.get("descending", False))
+        gk, gv = dk.to_host(queue), dv.to_host(queue)
+        if c["algo"] == "stable_sort_by_key_prefix":
+            m = len(c["expected_keys_prefix"])
+            assert list(gk[:m]) == c["expected_keys_prefix"], c["id"]
+            assert list(gv[:m]) == c["expected_values_prefix"], c["id"]
+        else:
+            assert np.array_equal(gk, _np(c["expected_keys"], c["dtype"])), c["id"]
+            assert np.array_equal(gv, _np(c["expected_values"], c["vdtype"])), c["id"]
+
+
+def test_golden_scans(queue, golden):
+    for c in golden:
+        if c["algo"] not in ("inclusive_scan", "exclusive_scan"):
+            continue
+        x = _np(c["input"], c["dtype"])
+        excl = c["algo"] == "exclusive_scan"
+        for inplace in (False, True):
+            d = dev(queue, x)
+            out = d if inplace else ca.DeviceArray(x.size, x.dtype, queue)
+            ca.scan(queue, d, out, excl, init=c.get("init") if excl else None,
+                    op=OPS[c.get("op", "plus")])
+            assert np.array_equal(out.to_host(queue), _np(c["expected"], c["dtype"])), c["id"]
+
+
+def test_golden_reduce_and_accumulate(queue, golden):
+    for c in golden:
+        if c["algo"] not in ("reduce", "accumulate"):
+            continue
+        x = _np(c["input"], c["dtype"])
+        out = ca.DeviceArray(1, x.dtype, queue)
+        init = c.get("init") if c["algo"] == "accumulate" else None
+        ca.reduce(queue, dev(queue, x), out, op=OPS[c["op"]], init=init)
+        assert int(out.to_host(queue)[0]) == c["expected"], c["id"]
+
+
+def test_golden_reduce_by_key(queue, golden):
+    for c in golden:
+        if c["algo"] != "reduce_by_key":
+            continue
+        k = _np(c["keys"], c["dtype"])
+        v = _np(c["values"], c["vdtype"])
+        ko = ca.DeviceArray(k.size, k.dtype, queue)
+        vo = ca.DeviceArray(k.size, v.dtype, queue)
+        r = ca.reduce_by_key(queue, dev(queue, k), dev(queue, v), ko, vo, op=OPS[c["op"]])
+        assert r == len(c["expected_keys"]), c["id"]
+        assert np.array_equal(ko.to_host(queue)[:r], _np(c["expected_keys"], c["dtype"])), c["id"]
+        gv = vo.to_host(queue)[:r]
+        if "rtol" in c:
+            assert np.allclose(gv, c["expected_values"], rtol=c["rtol"]), c["id"]
+        else:
+            assert np.array_equal(gv, _np(c["expected_values"], c["vdtype"])), c["id"]
+
+
+def test_golden_transform(queue, golden):
+    for c in golden:
+        if c["algo"] != "transform":
+            continue
+        a, b = _np(c["a"], c["dtype"]), _np(c["b"], c["dtype"])
+        out = ca.DeviceArray(a.size, a.dtype, queue)
+        ca.transform(queue, dev(queue, a), dev(queue, b), out, OPS[c["op"]])
+        assert list(out.to_host(queue)) == c["expected"], c["id"]
+
+
+# ================================================================== sorts
+ALL_KEYS = ["uint8", "int8", "uint16", "int16", "uint32", "int32", "uint64", "int64", "float32",
+            "float64"]
+
+
+@pytest.mark.parametrize("dt", ALL_KEYS)
+@pytest.mark.parametrize("n", [2, 17, 32, 33, 300, 4095, 4096, 4097, 12289, 70001])
+def test_sort_keys_bit_exact(queue, dt, n):
+    k = random_keys(dt, n)
+    for api in (ca.SORT, ca.STABLE_SORT):
+        for desc in (False, True):
+            d = dev(queue, k)
+            ca.sort(queue, d, api=api, descending=desc)
+            exp = O.sort_dispatch(api, k, descending=desc)
+            assert np.array_equal(bits(d.to_host(queue)), bits(exp)), (api, desc)
+
+
+@pytest.mark.parametrize("dt", ["uint32", "int32", "float32", "float64", "uint64", "int8", "uint16"])
+@pytest.mark.parametrize("vdt", ["uint8", "uint16", "uint32", "uint64"])
+@pytest.mark.parametrize("n", [5, 31, 32, 1000, 5000, 40000])
+def test_sort_by_key_stable_bit_exact(queue, dt, vdt, n):
+    k = random_keys(dt, n)
+    if n > 10:
+        k[RNG.integers(0, n, n // 3)] = k[0]  # many ties
+    v = np.arange(n).astype(vdt)
+    for api in (ca.SORT_BY_KEY, ca.STABLE_SORT_BY_KEY):
+        for desc in (False, True):
+            dk, dv = dev(queue, k), dev(queue, v)
+            ca.sort(queue, dk, dv, api=api, descending=desc)
+            ek, ev = O.sort_dispatch(api, k, v, descending=desc)
+            assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), (api, desc)
+            assert np.array_equal(dv.to_host(queue), ev), (api, desc)
+
+
+def test_sort_by_key_16_byte_values(queue):
+    n = 9000
+    k = random_keys("uint32", n)
+    v = RNG.integers(0, 2 ** 32, (n, 4), dtype=np.uint32)
+    dk = dev(queue, k)
+    dv = ca.DeviceArray.from_host(v.reshape(-1), queue)
+    ca.check(ca.lib().bcmi_sort(queue.h, ca.SORT_BY_KEY, ca.U32, dk.ptr, dv.ptr, 16, n, 0))
+    ek, order = O.radix_order_sort(k, np.arange(n, dtype=np.uint32))
+    assert np.array_equal(dk.to_host(queue), ek)
+    assert np.array_equal(dv.to_host(queue).reshape(n, 4), v[order])
+
+
+def test_sort_unaligned_subrange(queue):
+    k = random_keys("uint32", 100003)
+    d = dev(queue, k)
+    ca.sort(queue, d, api=ca.SORT, n=100003 - 7 - 3, key_ptr=d.at(7))
+    got = d.to_host(queue)
+    assert np.array_equal(got[:7], k[:7]) and np.array_equal(got[-3:], k[-3:])
+    assert np.array_equal(got[7:-3], np.sort(k[7:-3]))
+
+
+def test_sort_plain_uint32_config_bit_identical(queue):
+    """sort of 2^20 uint32 from mt19937(0xB0057001): bit-identical to the
+    reference (the sorted multiset is unique)."""
+    k = O.gen_sort_u32(1 << 20)
+    d = dev(queue, k)
+    ca.sort(queue, d)
+    assert np.array_equal(d.to_host(queue), np.sort(k))
+
+
+def test_sort_by_key_float_config_small(queue):
+    """The sort_by_key workload (float keys with signed zeros and duplicates,
+    payload = index) at 2^20: keys and payload bit-identical to the oracle."""
+    k, v = O.gen_sort_by_key_f32(1 << 20)
+    dk, dv = dev(queue, k), dev(queue, v)
+    ca.sort(queue, dk, dv, api=ca.SORT_BY_KEY)
+    ek, ev = O.radix_order_sort(k, v)
+    assert np.array_equal(bits(dk.to_host(queue)), bits(ek))
+    assert np.array_equal(dv.to_host(queue), ev)
+
+
+def test_sort_full_size_properties(queue):
+    """2^27 uint32 (the metric's size): sorted, and the same multiset
+    (checksums of the raw keys are order-independent)."""
+    n = 1 << 27
+    k = O.gen_sort_u32(n)
+    d = dev(queue, k)
+    ca.sort(queue, d)
+    got = d.to_host(queue)
+    assert np.all(got[1:] >= got[:-1])
+    assert int(got.astype(np.uint64).sum()) == int(k.astype(np.uint64).sum())
+    assert int(np.bitwise_xor.reduce(got)) == int(np.bitwise_xor.reduce(k))
+    h = lambda a: int(((a.astype(np.uint64) * np.uint64(0x9E3779B97F4A7C15)) >> np.uint64(17)).sum())
+    assert h(got) == h(k)
+
+
+# ================================================================== scans
+@pytest.mark.parametrize("dt", ["int32", "uint32", "int64", "uint64", "int8", "uint16", "float64"])
+@pytest.mark.parametrize("n", [1, 5, 1023, 4096, 4097, 65536 + 17, (1 << 20) + 3])
+def test_scan_exact(queue, dt, n):
+    if np.dtype(dt).kind == "f":
+        x = RNG.integers(-1000, 1000, n).astype(dt)  # exact in double
+    else:
+        x = RNG.integers(-100, 100, n).astype(dt)
+    d = dev(queue, x)
+    out = ca.DeviceArray(n, x.dtype, queue)
+    for excl in (False, True):
+        init = 5 if excl else None
+        ca.scan(queue, d, out, excl, init=init, precision=ca.NAIVE)
+        exp = O.scan(x, excl, 5 if excl else 0)
+        assert np.array_equal(out.to_host(queue), exp), excl
+
+
+@pytest.mark.parametrize("op", ["multiplies", "min", "max"])
+@pytest.mark.parametrize("dt", ["int32", "uint64", "int16"])
+def test_scan_ops(queue, op, dt):
+    n = 50001
+    x = RNG.integers(-3, 4, n).astype(dt) if op == "multiplies" else random_keys(dt, n)
+    d = dev(queue, x)
+    out = ca.DeviceArray(n, x.dtype, queue)
+    for excl in (False, True):
+        init = {"multiplies": 1, "min": 7, "max": 7}[op] if excl else None
+        ca.scan(queue, d, out, excl, init=init, op=OPS[op])
+        exp = O.scan(x, excl, init if excl else 0, OOPS[op])
+        assert np.array_equal(out.to_host(queue), exp), excl
+
+
+def test_scan_in_place_and_unaligned(queue):
+    n = 300007
+    x = RNG.integers(-100, 100, n).astype(np.int32)
+    d = dev(queue, x)
+    ca.scan(queue, d, d, True, init=3)
+    assert np.array_equal(d.to_host(queue), O.scan(x, True, 3))
+    d = dev(queue, x)
+    out = ca.DeviceArray(n, np.int32, queue)
+    # input at +1 element, output at +2: neither 16-byte aligned
+    m = n - 2
+    ca.check(ca.lib().bcmi_scan(queue.h, ca.I32, ca.PLUS, d.at(1), out.at(2), m, None, 0, 0))
+    assert np.array_equal(out.to_host(queue)[2:], O.scan(x[1:1 + m], False))
+
+
+def test_scan_full_size_int32(queue):
+    """exclusive_scan of 2^28 int32 in [-100, 100]: bit-exact."""
+    n = 1 << 28
+    x = O.gen_scan_i32(n)
+    d = dev(queue, x)
+    out = ca.DeviceArray(n, np.int32, queue)
+    ca.scan(queue, d, out, True, init=0)
+    got = out.to_host(queue)
+    exp = np.empty(n, np.int32)
+    exp[0] = 0
+    np.cumsum(x[:-1], dtype=np.int32, out=exp[1:])
+    assert np.array_equal(got, exp)
+
+
+def _scan_err_bound(x, dd):
+    # compensated within a tile + FP64 carry: |err| <= ulp(result) plus the
+    # FP64 carry's rounding, 2^-50 * sum|x| of the prefix
+    return np.spacing(np.abs(dd).astype(np.float32)).astype(np.float64) + \
+        2.0 ** -50 * np.cumsum(np.abs(x.astype(np.float64)))
+
+
+@pytest.mark.parametrize("n", [100, 4096 * 3 + 5, 1 << 20])
+def test_scan_float_compensated(queue, n):
+    x = O.gen_lognormal_f32(n)
+    d = dev(queue, x)
+    out = ca.DeviceArray(n, np.float32, queue)
+    ca.scan(queue, d, out, False)
+    dd = O.scan_f32_dd(x, False)
+    err = np.abs(out.to_host(queue).astype(np.float64) - dd)
+    assert np.all(err <= _scan_err_bound(x, dd))
+    ca.scan(queue, d, out, True, init=0.0)
+    dd = O.scan_f32_dd(x, True)
+    err = np.abs(out.to_host(queue).astype(np.float64) - dd)
+    assert np.all(err <= _scan_err_bound(x, dd))
+
+
+# ================================================================= reduce
+@pytest.mark.parametrize("dt", ["int32", "uint32", "int64", "uint8", "int16", "uint64"])
+@pytest.mark.parametrize("op", ["plus", "multiplies", "min", "max"])
+@pytest.mark.parametrize("n", [1, 7, 1000, 1 << 20])
+def test_reduce_integers_exact(queue, dt, op, n):
+    x = RNG.integers(-3, 4, n).astype(dt) if op == "multiplies" else random_keys(dt, n)
+    out = ca.DeviceArray(1, x.dtype, queue)
+    ca.reduce(queue, dev(queue, x), out, op=OPS[op])
+    assert out.to_host(queue)[0] == O.reduce(x, OOPS[op])
+    # accumulate = fold with init
+    ini = np.array([2], dtype=dt)
+    ca.reduce(queue, dev(queue, x), out, op=OPS[op], init=ini[0])
+    assert out.to_host(queue)[0] == O.reduce(np.r_[ini, x], OOPS[op])
+
+
+def test_reduce_unaligned(queue):
+    x = RNG.integers(-100, 100, 100001).astype(np.int32)
+    d = dev(queue, x)
+    out = ca.DeviceArray(1, np.int32, queue)
+    ca.check(ca.lib().bcmi_reduce(queue.h, ca.I32, ca.PLUS, d.at(3), 100001 - 5, None, out.ptr, None, 0))
+    assert out.to_host(queue)[0] == x[3:-2].sum()
+
+
+@pytest.mark.parametrize("n", [1, 33, 4096, 1 << 20, (1 << 22) + 11])
+def test_reduce_float_compensated_within_half_ulp_of_fp64(queue, n):
+    x = O.gen_lognormal_f32(n)
+    out = ca.DeviceArray(1, np.float32, queue)
+    f64 = ca.reduce(queue, dev(queue, x), out)
+    got = float(out.to_host(queue)[0])
+    hi, lo = O.sum_dd(x)
+    exact = hi + lo
+    ulp = float(np.spacing(np.float32(abs(exact))))
+    assert abs(got - exact) <= 0.5 * ulp + 1e-30
+    assert abs(float(f64.to_host(queue)[0]) - exact) <= 2.0 ** -45 * float(np.abs(x.astype(np.float64)).sum())
+
+
+def test_reduce_float_full_size_within_two_ulp(queue):
+    """accumulate/reduce(plus) on 2^28 lognormal(0,3) floats with random
+    signs: the compensated result is within 2 ulp of the FP64 oracle."""
+    n = 1 << 28
+    x = O.gen_lognormal_f32(n)
+    out = ca.DeviceArray(1, np.float32, queue)
+    ca.reduce(queue, dev(queue, x), out)
+    got = float(out.to_host(queue)[0])
+    hi, lo = O.sum_dd(x)
+    ulp = float(np.spacing(np.float32(abs(hi))))
+    assert abs(got - (hi + lo)) <= 2 * ulp
+
+
+def test_reduce_double_and_empty(queue):
+    x = RNG.normal(size=100000)
+    out = ca.DeviceArray(1, np.float64, queue)
+    ca.reduce(queue, dev(queue, x), out)
+    assert abs(out.to_host(queue)[0] - math.fsum(x)) <= 4 * np.spacing(abs(math.fsum(x)))
+    # accumulate of an empty range returns init
+    ca.check(ca.lib().bcmi_reduce(queue.h, ca.F64, ca.PLUS, None, 0,
+                                  np.array([2.5]).ctypes.data_as(ca.C.c_void_p), out.ptr, None, 0))
+    assert out.to_host(queue)[0] == 2.5
+
+
+# ========================================================== reduce_by_key
+@pytest.mark.parametrize("n", [1, 2, 257, 4096, 4097, 100003, 1 << 20])
+def test_reduce_by_key_int_exact(queue, n):
+    lens = RNG.geometric(1 / 20, n)
+    keys = np.repeat(np.arange(lens.size, dtype=np.int32), lens)[:n]
+    keys = (keys * 7 - 1000).astype(np.int32)
+    vals = RNG.integers(-1000, 1000, n).astype(np.int32)
+    ko = ca.DeviceArray(n, np.int32, queue)
+    vo = ca.DeviceArray(n, np.int32, queue)
+    for op in ("plus", "min", "max"):
+        r = ca.reduce_by_key(queue, dev(queue, keys), dev(queue, vals), ko, vo, op=OPS[op])
+        ek, ev = O.reduce_by_key(keys, vals, OOPS[op])
+        assert r == ek.size
+        assert np.array_equal(ko.to_host(queue)[:r], ek)
+        assert np.array_equal(vo.to_host(queue)[:r], ev)
+
+
+def test_reduce_by_key_float_keys_signed_zero_and_nan(queue):
+    k = np.array([1.0, 0.0, -0.0, 0.0, np.nan, np.nan, 2.0, 2.0] * 700, np.float32)
+    v = np.ones(k.size, np.int64)
+    ko = ca.DeviceArray(k.size, np.float32, queue)
+    vo = ca.DeviceArray(k.size, np.int64, queue)
+    r = ca.reduce_by_key(queue, dev(queue, k), dev(queue, v), ko, vo)
+    ek, ev = O.reduce_by_key(k, v)
+    assert r == ek.size
+    assert np.array_equal(bits(ko.to_host(queue)[:r]), bits(ek))
+    assert np.array_equal(vo.to_host(queue)[:r], ev)
+
+
+@pytest.mark.parametrize("n", [1000, 1 << 20])
+def test_reduce_by_key_float_compensated(queue, n):
+    keys, vals = O.gen_runs_i32_f32(n)
+    ko = ca.DeviceArray(n, np.int32, queue)
+    vo = ca.DeviceArray(n, np.float32, queue)
+    r = ca.reduce_by_key(queue, dev(queue, keys), dev(queue, vals), ko, vo)
+    ek, es = O.reduce_by_key_f32_dd(keys, vals)
+    assert r == ek.size
+    assert np.array_equal(ko.to_host(queue)[:r], ek)
+    got = vo.to_host(queue)[:r].astype(np.float64)
+    # each run: correctly rounded up to the FP64 carry's rounding
+    absum = np.add.reduceat(np.abs(vals.astype(np.float64)), np.r_[0, np.flatnonzero(np.diff(keys)) + 1])
+    bound = np.spacing(np.abs(es).astype(np.float32)).astype(np.float64) + 2.0 ** -50 * absum
+    assert np.all(np.abs(got - es) <= bound)
+
+
+# ========================================================= segmented scan
+@pytest.mark.parametrize("n", [1, 100, 4097, 200003])
+def test_segmented_scan_int_exact(queue, n):
+    lens = RNG.integers(1, 70, n)
+    keys = np.repeat(np.arange(lens.size, dtype=np.int32), lens)[:n]
+    vals = RNG.integers(-100, 100, n).astype(np.int32)
+    out = ca.DeviceArray(n, np.int32, queue)
+    for excl in (False, True):
+        ca.segmented_scan(queue, dev(queue, keys), dev(queue, vals), out, excl, init=4 if excl else None)
+        assert np.array_equal(out.to_host(queue), O.segmented_scan(keys, vals, excl, 4 if excl else 0))
+
+
+def test_segmented_scan_float(queue):
+    keys, vals = O.gen_runs_i32_f32(300001)
+    out = ca.DeviceArray(vals.size, np.float32, queue)
+    for excl in (False, True):
+        ca.segmented_scan(queue, dev(queue, keys), dev(queue, vals), out, excl, init=0.0 if excl else None)
+        dd = O.segmented_scan_f32_dd(keys, vals, excl)
+        err = np.abs(out.to_host(queue).astype(np.float64) - dd)
+        bound = np.spacing(np.abs(dd).astype(np.float32)).astype(np.float64) + \
+            2.0 ** -50 * np.cumsum(np.abs(vals.astype(np.float64)))
+        assert np.all(err <= bound)
+
+
+# ============================================================== transform
+@pytest.mark.parametrize("n", [1, 3, 1000, (1 << 20) + 7])
+def test_saxpy_bit_exact(queue, n):
+    x, y = O.gen_saxpy_f32(n)
+    out = ca.DeviceArray(n, np.float32, queue)
+    ca.saxpy(queue, 2.5, dev(queue, x), dev(queue, y), out)
+    assert np.array_equal(bits(out.to_host(queue)), bits(O.saxpy(2.5, x, y)))
+
+
+def test_saxpy_unaligned_in_place(queue):
+    n = 100001
+    x, y = O.gen_saxpy_f32(n)
+    dx, dy = dev(queue, x), dev(queue, y)
+    m = n - 5
+    ca.check(ca.lib().bcmi_saxpy(queue.h, 2.5, dx.at(1), dy.at(1), dy.at(1), m))
+    exp = y.copy()
+    exp[1:1 + m] = O.saxpy(2.5, x[1:1 + m], y[1:1 + m])
+    assert np.array_equal(bits(dy.to_host(queue)), bits(exp))
+    ca.check(ca.lib().bcmi_saxpy(queue.h, 2.5, dx.at(1), dx.at(2), dy.ptr, m))  # mutually unaligned
+    assert np.array_equal(bits(dy.to_host(queue)[:m]), bits(O.saxpy(2.5, x[1:1 + m], x[2:2 + m])))
+
+
+@pytest.mark.parametrize("dt", ["int32", "uint8", "int64", "float32", "float64"])
+@pytest.mark.parametrize("op", ["plus", "multiplies", "min", "max"])
+def test_transform_binary_ops(queue, dt, op):
+    n = 12345
+    a, b = random_keys(dt, n), random_keys(dt, n)
+    if np.dtype(dt).kind == "f":
+        a, b = np.nan_to_num(a), np.nan_to_num(b)
+    out = ca.DeviceArray(n, a.dtype, queue)
+    ca.transform(queue, dev(queue, a), dev(queue, b), out, OPS[op])
+    assert np.array_equal(bits(out.to_host(queue)), bits(O.transform(a, b, OOPS[op])))
+
+
+# ============================================================ hipGraph
+def test_graph_replay_matches_eager(queue):
+    n = 300000
+    k = random_keys("uint32", n)
+    x = RNG.integers(-100, 100, n).astype(np.int32)
+    dk, dx = dev(queue, k), dev(queue, x)
+    out = ca.DeviceArray(n, np.int32, queue)
+    queue.reserve(n, 8)
+    g = ca.Graph(queue)
+    with g:
+        ca.sort(queue, dk)
+        ca.scan(queue, dx, out, True, init=0)
+    exp_k, exp_s = np.sort(k), O.scan(x, True, 0)
+    for rep in range(3):
+        ca.check(ca.lib().bcmi_memcpy(queue.h, dk.ptr, k.ctypes.data_as(ca.C.c_void_p), k.nbytes, ca.H2D, 1))
+        g.launch()
+        queue.finish()
+        assert np.array_equal(dk.to_host(queue), exp_k), rep
+        assert np.array_equal(out.to_host(queue), exp_s), rep
+        # eager work between replays
+        ca.scan(queue, dx, out, False)
+        queue.finish()

@@ -1,0 +1,444 @@
+"""Pins the CPU oracle (oracle/bcmi_oracle.c) before anything is checked
+against it.  Every oracle function is tied to at least one of: the C++
+standard's known answers, the reference's own golden vectors
+(tests/golden/reference_vectors.json), closed forms worked out by hand from
+the reference's algorithm, or brute force in Python.  No GPU needed.
+"""
+import itertools
+import math
+import struct
+from fractions import Fraction
+
+import numpy as np
+import pytest
+
+import oracle as O
+
+RNG = np.random.default_rng(1234)
+
+
+# ---------------------------------------------------------------- RNG / KAT
+def test_mt19937_known_answers():
+    # C++11 [rand.predef]: the 10000th consecutive invocation of a
+    # default-constructed engine (seed 5489) produces these values.
+    assert int(O.mt19937(5489, 10000)[-1]) == 4123659995
+    assert int(O.mt19937_64(5489, 10000)[-1]) == 9981545732273789042
+
+
+def test_workload_generators_follow_their_definitions():
+    raw = O.mt19937(0xB0057001, 64)
+    assert np.array_equal(O.gen_sort_u32(64), raw)
+    raw3 = O.mt19937(0xB0057003, 1000).astype(np.int64)
+    x = O.gen_scan_i32(1000)
+    assert np.array_equal(x, (raw3 % 201 - 100).astype(np.int32))
+    assert x.min() >= -100 and x.max() <= 100
+    k, v = O.gen_sort_by_key_f32(1 << 16)
+    assert np.array_equal(v, np.arange(1 << 16, dtype=np.uint32))
+    assert np.all(np.abs(k) <= 1e6)
+    zeros = np.count_nonzero(k == 0) / k.size
+    assert 0.007 < zeros < 0.013  # ~1% signed zeros
+    assert np.count_nonzero(np.signbit(k) & (k == 0)) > 0
+    off = O.gen_segments(65536)
+    lens = np.diff(off.astype(np.int64))
+    assert lens[0] == 1 << 20 and lens.min() >= 1 and lens.max() <= 1 << 20
+    assert 200 < np.median(lens) < 320  # median 256
+    rk, rv = O.gen_runs_i32_f32(1 << 16)
+    assert np.all(np.diff(rk) >= 0) and np.all(np.diff(rk) <= 1)
+    runs = np.diff(np.flatnonzero(np.diff(np.r_[-1, rk, rk[-1] + 1])))
+    assert 25 < runs.mean() < 40  # geometric, mean 32
+
+
+# ----------------------------------------------------------- radix() keys
+INTS = {
+    "uint8": [0, 1, 2, 127, 128, 200, 255],
+    "int8": [-128, -127, -1, 0, 1, 100, 127],
+    "uint16": [0, 1, 32767, 32768, 65535],
+    "int16": [-32768, -1, 0, 1, 32767],
+    "uint32": [0, 1, 2 ** 31 - 1, 2 ** 31, 2 ** 32 - 1],
+    "int32": [-2 ** 31, -2 ** 31 + 1, -1, 0, 1, 2 ** 31 - 1],
+    "uint64": [0, 1, 2 ** 63 - 1, 2 ** 63, 2 ** 64 - 1],
+    "int64": [-2 ** 63, -1, 0, 1, 2 ** 63 - 1],
+}
+
+
+def bits_of(arr):
+    a = np.ascontiguousarray(arr)
+    return a.view({1: np.uint8, 2: np.uint16, 4: np.uint32, 8: np.uint64}[a.itemsize])
+
+
+@pytest.mark.parametrize("dt", list(INTS))
+def test_radix_key_orders_integers_numerically(dt):
+    info = np.iinfo(dt)
+    rnd = RNG.integers(info.min, info.max, 200, endpoint=True, dtype=dt)
+    vals = np.r_[np.array(INTS[dt], dtype=dt), rnd]
+    b = bits_of(vals)
+    asc = [O.radix_key(int(x), dt) for x in b]
+    order = np.argsort(np.array(asc, dtype=np.uint64), kind="stable")
+    assert np.all(np.diff(vals[order].astype(np.float64)) >= 0)
+    desc = [O.radix_key(int(x), dt, True) for x in b]
+    order = np.argsort(np.array(desc, dtype=np.uint64), kind="stable")
+    got = vals[order]
+    if np.dtype(dt).kind == "i":
+        # (-x) ^ signbit: INT_MIN negates to itself and maps to key 0, so it
+        # comes first; everything else is in descending order.
+        mn = np.iinfo(dt).min
+        nmin = int(np.count_nonzero(vals == mn))
+        assert np.all(got[:nmin] == mn)
+        assert np.all(np.diff(got[nmin:].astype(np.float64)) <= 0)
+    else:
+        assert np.all(np.diff(got.astype(np.float64)) <= 0)
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_radix_key_orders_floats_by_ieee_total_order(dt):
+    fi = np.finfo(dt)
+    special = [-np.inf, -fi.max, -1.0, -fi.tiny, -fi.smallest_subnormal, -0.0, 0.0,
+               fi.smallest_subnormal, fi.tiny, 1.0, fi.max, np.inf]
+    vals = np.array(special + list(RNG.normal(size=300) * 10.0 ** RNG.integers(-30, 30, 300)),
+                    dtype=dt)
+    b = bits_of(vals)
+    asc = np.array([O.radix_key(int(x), dt) for x in b], dtype=np.uint64)
+    s = vals[np.argsort(asc, kind="stable")]
+    assert np.all(np.diff(s) >= 0)
+    # -0.0 sorts strictly before +0.0 (bit order)
+    z = np.flatnonzero(s == 0)
+    assert np.signbit(s[z[0]]) and not np.signbit(s[z[-1]])
+    # NaNs: negative-signed NaN before -inf, positive NaN after +inf
+    nan_bits = {"float32": (0x7FC00000, 0xFFC00000), "float64": (0x7FF8000000000000, 0xFFF8000000000000)}[dt]
+    pos_nan = O.radix_key(nan_bits[0], dt)
+    neg_nan = O.radix_key(nan_bits[1], dt)
+    assert neg_nan < asc.min() and pos_nan > asc.max()
+    # descending: numerically non-increasing, except that +-denorm_min share
+    # keys with -+0.0 (see the collision test below), so they are left out
+    keep = np.abs(vals) != fi.smallest_subnormal
+    desc = np.array([O.radix_key(int(x), dt, True) for x in b[keep]], dtype=np.uint64)
+    s = vals[keep][np.argsort(desc, kind="stable")]
+    assert np.all(np.diff(s) <= 0)
+
+
+def test_radix_key_descending_float_collisions():
+    # (-x) ^ (-(x>>31) | signbit) maps +denorm_min and -0.0 to the same key,
+    # and +0.0 and -denorm_min to the same key.
+    f = lambda v: struct.unpack("<I", struct.pack("<f", v))[0]
+    dmin = float(np.finfo(np.float32).smallest_subnormal)
+    assert O.radix_key(f(dmin), "float32", True) == O.radix_key(f(-0.0), "float32", True) == 0x7FFFFFFF
+    assert O.radix_key(f(0.0), "float32", True) == O.radix_key(f(-dmin), "float32", True) == 0x80000000
+
+
+# ------------------------------------------------------ golden: sort paths
+def _np(vals, dt):
+    if dt.startswith("float"):
+        return np.array(vals, dtype=dt)
+    return np.array(vals, dtype=np.int64).astype(dt)
+
+
+def _same(got, exp, cmp):
+    if cmp == "eq":
+        return np.array_equal(got, exp)
+    return np.array_equal(bits_of(got), bits_of(exp))
+
+
+def test_golden_sorts(golden):
+    n = 0
+    for c in golden:
+        if c["algo"] not in ("radix_sort", "sort", "radix_sort_range"):
+            continue
+        x = _np(c["input"], c["dtype"])
+        exp = _np(c["expected"], c["dtype"])
+        desc = c.get("descending", False)
+        if c["algo"] == "radix_sort":
+            got = O.radix_order_sort(x, descending=desc)
+        elif c["algo"] == "sort":
+            got = O.sort_dispatch(O.SORT, x, descending=desc)
+        else:
+            got = x.copy()
+            got[c["first"]:c["last"]] = O.radix_order_sort(x[c["first"]:c["last"]], descending=desc)
+        assert _same(got, exp, c.get("cmp")), c["id"]
+        n += 1
+    assert n > 30
+
+
+def test_golden_sort_by_key(golden):
+    n = 0
+    for c in golden:
+        if c["algo"] not in ("radix_sort_by_key", "sort_by_key", "stable_sort_by_key_prefix"):
+            continue
+        k = _np(c["keys"], c["dtype"])
+        v = _np(c["values"], c["vdtype"])
+        desc = c.get("descending", False)
+        api = {"radix_sort_by_key": None, "sort_by_key": O.SORT_BY_KEY,
+               "stable_sort_by_key_prefix": O.STABLE_SORT_BY_KEY}[c["algo"]]
+        if api is None:
+            gk, gv = O.radix_order_sort(k, v, descending=desc)
+        else:
+            gk, gv = O.sort_dispatch(api, k, v, descending=desc)
+        if c["algo"] == "stable_sort_by_key_prefix":
+            m = len(c["expected_keys_prefix"])
+            assert list(gk[:m]) == c["expected_keys_prefix"] and list(gv[:m]) == c["expected_values_prefix"]
+        else:
+            assert np.array_equal(gk, _np(c["expected_keys"], c["dtype"])), c["id"]
+            assert np.array_equal(gv, _np(c["expected_values"], c["vdtype"])), c["id"]
+        n += 1
+    assert n >= 10
+
+
+@pytest.mark.parametrize("dt", ["int32", "uint8", "float32", "float64", "int64"])
+def test_radix_order_sort_is_the_stable_sort_by_radix_key(dt):
+    # brute force: Python's sorted() is stable; key = radix key
+    if dt.startswith("float"):
+        k = RNG.integers(-8, 8, 500).astype(dt) * 0.5
+        k[RNG.integers(0, 500, 20)] = -0.0
+    else:
+        info = np.iinfo(dt)
+        k = RNG.integers(max(info.min, -20), min(info.max, 20), 500, endpoint=True).astype(dt)
+    v = np.arange(k.size, dtype=np.uint32)
+    for desc in (False, True):
+        gk, gv = O.radix_order_sort(k, v, descending=desc)
+        b = bits_of(k)
+        ref = sorted(range(k.size), key=lambda i: O.radix_key(int(b[i]), dt, desc))
+        assert np.array_equal(gv, np.array(ref, dtype=np.uint32))
+        assert np.array_equal(bits_of(gk), b[ref])
+
+
+@pytest.mark.parametrize("dt", ["int32", "float32", "uint16"])
+def test_insertion_sort_is_a_stable_comparison_sort(dt):
+    for n in (0, 1, 2, 5, 17, 31, 32):
+        k = (RNG.integers(-5, 5, n) if dt != "uint16" else RNG.integers(0, 9, n)).astype(dt)
+        if dt == "float32" and n:
+            k[::3] = -0.0
+            k[1::4] = 0.0
+        v = np.arange(n, dtype=np.int32)
+        for desc in (False, True):
+            gk, gv = O.insertion_sort(k, v, descending=desc)
+            ref = sorted(range(n), key=lambda i: float(k[i]), reverse=False)
+            if desc:
+                # a stable descending sort keeps equal keys in input order
+                ref = sorted(range(n), key=lambda i: -float(k[i]))
+            assert np.array_equal(gv, np.array(ref, dtype=np.int32)), (n, desc)
+
+
+def test_sort_dispatch_thresholds():
+    # +0.0 before -0.0 in the input: comparison sorts keep that order, the
+    # radix path puts -0.0 first (sort.hpp:49-54: n <= 32 -> insertion).
+    for n, api, radix in ((32, O.SORT, False), (33, O.SORT, True), (31, O.SORT_BY_KEY, False),
+                          (32, O.SORT_BY_KEY, True), (2, O.STABLE_SORT, True),
+                          (31, O.STABLE_SORT_BY_KEY, False)):
+        k = np.ones(n, np.float32)
+        k[0], k[1] = 0.0, -0.0
+        v = np.arange(n, dtype=np.int32)
+        gk, gv = O.sort_dispatch(api, k, v)
+        assert (gv[0] == 1) == radix, (n, api)
+
+
+def test_compare_stable_sort_treats_signed_zeros_equal():
+    k = np.array([0.0, -0.0, 1.0, -0.0, 0.0], np.float32)
+    v = np.arange(5, dtype=np.int32)
+    gk, gv = O.compare_stable_sort(k, v)
+    assert list(gv) == [0, 1, 3, 4, 2]
+
+
+def test_segmented_sort_sorts_each_segment():
+    off = np.array([0, 3, 3, 10, 11, 40], np.uint64)
+    k = RNG.integers(0, 2 ** 63, 40, dtype=np.uint64)
+    v = np.arange(40, dtype=np.uint32)
+    gk, gv = O.segmented_sort(k, v, off)
+    for s in range(len(off) - 1):
+        b, e = int(off[s]), int(off[s + 1])
+        ref = sorted(range(b, e), key=lambda i: int(k[i]))
+        assert list(gv[b:e]) == ref
+
+
+# ------------------------------------------------------------------- scans
+def test_golden_scans(golden):
+    n = 0
+    for c in golden:
+        if c["algo"] not in ("inclusive_scan", "exclusive_scan"):
+            continue
+        x = _np(c["input"], c["dtype"])
+        op = {"plus": O.PLUS, "multiplies": O.MULTIPLIES}[c.get("op", "plus")]
+        got = O.scan(x, c["algo"] == "exclusive_scan", c.get("init", 0), op)
+        assert np.array_equal(got, _np(c["expected"], c["dtype"])), c["id"]
+        n += 1
+    assert n >= 7
+
+
+@pytest.mark.parametrize("dt", ["int32", "uint8", "int64", "uint32"])
+def test_integer_scans_wrap_like_numpy(dt):
+    x = RNG.integers(-(2 ** 20), 2 ** 20, 5000).astype(dt)
+    assert np.array_equal(O.scan(x, False), np.cumsum(x, dtype=dt))
+    e = O.scan(x, True, 7)
+    assert e[0] == 7 and np.array_equal(e[1:], (np.cumsum(x, dtype=dt) + np.array(7, dt)).astype(dt)[:-1])
+    m = O.scan(x, False, op=O.MAX)
+    assert np.array_equal(m, np.maximum.accumulate(x))
+    m = O.scan(x, False, op=O.MIN)
+    assert np.array_equal(m, np.minimum.accumulate(x))
+
+
+def test_double_double_prefix_is_correctly_rounded():
+    x = O.gen_lognormal_f32(3000)
+    got = O.scan_f32_dd(x, False)
+    for i in RNG.integers(0, 3000, 200):
+        assert got[i] == math.fsum(float(t) for t in x[:i + 1])
+    ex = O.scan_f32_dd(x, True, 0.5)
+    assert ex[0] == 0.5 and ex[10] == math.fsum([0.5] + [float(t) for t in x[:10]])
+
+
+def test_double_double_sum_is_correctly_rounded():
+    for n in (1, 10, 1000, 100000):
+        x = O.gen_lognormal_f32(n)
+        hi, lo = O.sum_dd(x)
+        assert hi == math.fsum(float(t) for t in x)
+    d = RNG.normal(size=1000) * 1e10
+    assert O.sum_dd(d)[0] == math.fsum(d)
+
+
+def test_segmented_scans_match_groupby():
+    keys = np.repeat(np.arange(50, dtype=np.int32), RNG.integers(1, 9, 50))
+    vals = RNG.integers(-50, 50, keys.size).astype(np.int32)
+    inc = O.segmented_scan(keys, vals, False)
+    exc = O.segmented_scan(keys, vals, True, 3)
+    i = 0
+    for _, grp in itertools.groupby(range(keys.size), key=lambda j: keys[j]):
+        idx = list(grp)
+        run = np.cumsum(vals[idx])
+        assert np.array_equal(inc[idx], run)
+        assert np.array_equal(exc[idx], np.r_[0, run[:-1]] + 3)
+        i += len(idx)
+    x = O.gen_lognormal_f32(keys.size)
+    dd = O.segmented_scan_f32_dd(keys, x, False)
+    for _, grp in itertools.groupby(range(keys.size), key=lambda j: keys[j]):
+        idx = list(grp)
+        for t, j in enumerate(idx):
+            assert dd[j] == math.fsum(float(x[m]) for m in idx[:t + 1])
+
+
+# --------------------------------------------------------------- reduction
+def test_golden_reduce_and_accumulate(golden):
+    n = 0
+    opmap = {"plus": O.PLUS, "multiplies": O.MULTIPLIES, "min": O.MIN, "max": O.MAX}
+    for c in golden:
+        if c["algo"] == "reduce":
+            got = O.reduce(_np(c["input"], c["dtype"]), opmap[c["op"]])
+        elif c["algo"] == "accumulate":
+            x = np.r_[np.array([c["init"]], dtype=c["dtype"]), _np(c["input"], c["dtype"])]
+            got = O.reduce(x, opmap[c["op"]])  # left fold from init
+        else:
+            continue
+        assert int(got) == c["expected"], c["id"]
+        n += 1
+    assert n >= 15
+
+
+def test_reference_association_orders_closed_forms():
+    big = np.float32(2 ** 24)
+    # GPU tree (reduce_on_gpu.hpp:186-212): lane l folds x[l + 128 i]; lanes then
+    # combine pairwise.  1s in lanes 2 and 3 meet first and survive: 2^24 + 2.
+    x = np.zeros(1024, np.float32)
+    x[0], x[2], x[3] = big, 1, 1
+    assert O.reduce_gpu_order_f32(x) == big + 2
+    # the same 1s strided into lane 0 are absorbed one at a time: 2^24.
+    x = np.zeros(1024, np.float32)
+    x[0], x[128], x[256] = big, 1, 1
+    assert O.reduce_gpu_order_f32(x) == big
+    # two groups, then the final pass adds the two partials
+    x = np.zeros(2048, np.float32)
+    x[0], x[1024] = big, 1
+    x[1025] = 1  # lane 1 of group 1 -> group partial 2 -> 2^24 + 2
+    assert O.reduce_gpu_order_f32(x) == big + 2
+    # CPU column (reduce_on_cpu.hpp:66-94): cu chunks, each a left fold
+    x = np.zeros(65536, np.float32)
+    x[0], x[1], x[32768], x[32769] = big, 1, 1, 1
+    assert O.reduce_cpu_order_f32(x, 2) == big + 2
+    assert O.reduce_cpu_order_f32(x[:1000], 2) == big  # below threshold: serial
+    # accumulate (serial_accumulate.hpp:38-40): left fold from init
+    assert O.accumulate_f32(np.ones(2, np.float32), float(big)) == big
+    # Hillis-Steele block scan (scan_on_gpu.hpp:84-93): s[2] = (x2+x1) + x0
+    x = np.zeros(512, np.float32)
+    x[0], x[1], x[2] = big, 1, 1
+    inc = O.scan_gpu_order_f32(x, False)
+    assert inc[1] == big and inc[2] == big + 2
+    # lane 3 = (x3+x2) + (x1+x0) = 1 + 2^24 -> 2^24, and that is the path of
+    # the block sum; block 1 adds it: 2^24 + 1 -> 2^24
+    assert inc[3] == big
+    x[256] = 1
+    assert O.scan_gpu_order_f32(x, False)[256] == big
+    # with x2 = x3 = 1 the block sum is (x3+x2) + (x1+x0) = 2^24 + 2, and
+    # block 1's first element is (2^24 + 2) + 1 -> 2^24 + 4 (ties to even)
+    x = np.zeros(512, np.float32)
+    x[0], x[2], x[3], x[256] = big, 1, 1, 1
+    inc = O.scan_gpu_order_f32(x, False)
+    assert inc[3] == big + 2 and inc[256] == big + 4
+
+
+# ------------------------------------------------------------ reduce_by_key
+def test_golden_reduce_by_key(golden):
+    n = 0
+    opmap = {"plus": O.PLUS, "min": O.MIN, "max": O.MAX}
+    for c in golden:
+        if c["algo"] != "reduce_by_key":
+            continue
+        k = _np(c["keys"], c["dtype"])
+        v = _np(c["values"], c["vdtype"])
+        gk, gv = O.reduce_by_key(k, v, opmap[c["op"]])
+        assert np.array_equal(gk, _np(c["expected_keys"], c["dtype"])), c["id"]
+        if "rtol" in c:
+            assert np.allclose(gv, c["expected_values"], rtol=c["rtol"])
+        else:
+            assert np.array_equal(gv, _np(c["expected_values"], c["vdtype"])), c["id"]
+        n += 1
+    assert n >= 6
+
+
+def test_reduce_by_key_matches_groupby_and_writes_last_key():
+    keys = np.repeat(RNG.integers(-5, 5, 300).astype(np.int32), RNG.integers(1, 6, 300))
+    vals = RNG.integers(-100, 100, keys.size).astype(np.int32)
+    gk, gv = O.reduce_by_key(keys, vals)
+    ek, ev = [], []
+    pos = 0
+    for key, grp in itertools.groupby(keys):
+        m = len(list(grp))
+        ek.append(key)
+        ev.append(int(vals[pos:pos + m].sum()))
+        pos += m
+    assert list(gk) == ek and list(gv) == ev
+    # equal_to on floats: -0.0 == +0.0 merges the run and the LAST key is
+    # written; NaN never equals, so each NaN is its own run.
+    fk = np.array([0.0, -0.0, np.nan, np.nan, 1.0], np.float32)
+    gk, gv = O.reduce_by_key(fk, np.ones(5, np.float32))
+    assert gk.size == 4 and np.signbit(gk[0]) and np.isnan(gk[1]) and np.isnan(gk[2])
+    assert list(gv) == [2.0, 1.0, 1.0, 1.0]
+    # double-double run sums are the correctly rounded sums
+    fk = np.repeat(np.arange(30, dtype=np.int32), 7)
+    fv = O.gen_lognormal_f32(fk.size)
+    k2, s2 = O.reduce_by_key_f32_dd(fk, fv)
+    assert list(k2) == list(range(30))
+    for r in range(30):
+        assert s2[r] == math.fsum(float(t) for t in fv[7 * r:7 * r + 7])
+
+
+# --------------------------------------------------------------- transform
+def _round_f32(fr):
+    """Correctly round a Fraction to float32 (ties to even), independently of
+    any float arithmetic."""
+    d = np.float32(float(fr))
+    cands = [np.nextafter(d, np.float32(-np.inf)), d, np.nextafter(d, np.float32(np.inf))]
+    best = min(cands, key=lambda c: (abs(Fraction(float(c)) - fr),
+                                     int(np.array(c).view(np.uint32)) & 1))
+    return best
+
+
+def test_saxpy_is_the_fused_multiply_add():
+    x, y = O.gen_saxpy_f32(2000)
+    got = O.saxpy(2.5, x, y)
+    for i in range(2000):
+        exact = Fraction(2.5) * Fraction(float(x[i])) + Fraction(float(y[i]))
+        assert got[i] == _round_f32(exact)
+
+
+def test_golden_transform(golden):
+    for c in golden:
+        if c["algo"] != "transform":
+            continue
+        op = {"plus": O.PLUS, "multiplies": O.MULTIPLIES}[c["op"]]
+        got = O.transform(_np(c["a"], c["dtype"]), _np(c["b"], c["dtype"]), op)
+        assert list(got) == c["expected"], c["id"]
