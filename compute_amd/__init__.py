@@ -36,6 +36,7 @@ EXPORTS = [
     "bcmi_fill", "bcmi_event_create", "bcmi_event_destroy", "bcmi_event_record",
     "bcmi_event_synchronize", "bcmi_event_elapsed_ms", "bcmi_sort", "bcmi_scan", "bcmi_reduce",
     "bcmi_reduce_by_key", "bcmi_transform", "bcmi_saxpy", "bcmi_segmented_sort",
+    "bcmi_segmented_plan_create", "bcmi_segmented_sort_plan", "bcmi_segmented_plan_destroy",
     "bcmi_segmented_scan", "bcmi_graph_begin", "bcmi_graph_end", "bcmi_graph_launch",
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
@@ -95,7 +96,10 @@ def lib():
         L.bcmi_reduce_by_key.argtypes = [vp, i, i, i, vp, vp, sz, vp, vp, vp, i]
         L.bcmi_transform.argtypes = [vp, i, i, vp, vp, vp, sz]
         L.bcmi_saxpy.argtypes = [vp, C.c_float, vp, vp, vp, sz]
-        L.bcmi_segmented_sort.argtypes = [vp, i, vp, vp, sz, vp, vp, sz, i]
+        L.bcmi_segmented_sort.argtypes = [vp, i, vp, vp, sz, vp, sz, i]
+        L.bcmi_segmented_plan_create.argtypes = [vp, i, sz, vp, sz, i, C.POINTER(vp)]
+        L.bcmi_segmented_sort_plan.argtypes = [vp, vp, vp, vp]
+        L.bcmi_segmented_plan_destroy.argtypes = [vp]
         L.bcmi_segmented_scan.argtypes = [vp, i, i, vp, vp, vp, sz, vp, i, i]
         L.bcmi_graph_begin.argtypes = [vp]
         L.bcmi_graph_end.argtypes = [vp, C.POINTER(vp)]
@@ -305,13 +309,37 @@ def saxpy(q, a, x, y, out):
     check(lib().bcmi_saxpy(q.h, float(a), x.ptr, y.ptr, out.ptr, x.size))
 
 
-def segmented_sort(q, keys, values, offsets_dev, offsets_host, descending=False):
-    offsets_host = np.ascontiguousarray(offsets_host, np.uint64)
+def segmented_sort(q, keys, values, offsets_host, descending=False):
+    """Sort every segment [off[s], off[s+1]) as one stable_sort_by_key call
+    would (blocking)."""
+    off = np.ascontiguousarray(offsets_host, np.uint64)
     vb = values.dtype.itemsize if values is not None else 0
     check(lib().bcmi_segmented_sort(q.h, DTYPE[keys.dtype], keys.ptr,
-                                    values.ptr if values is not None else None, vb, offsets_dev.ptr,
-                                    offsets_host.ctypes.data_as(C.c_void_p), offsets_host.size - 1,
-                                    int(descending)))
+                                    values.ptr if values is not None else None, vb,
+                                    off.ctypes.data_as(C.c_void_p), off.size - 1, int(descending)))
+
+
+class SegmentedPlan:
+    """A segmented-sort plan: routing tables and temporaries on the device,
+    built once; run() does no host work and allocates nothing (capturable)."""
+
+    def __init__(self, q, key_dtype, value_dtype, offsets_host, descending=False):
+        off = np.ascontiguousarray(offsets_host, np.uint64)
+        vb = np.dtype(value_dtype).itemsize if value_dtype is not None else 0
+        self.h = C.c_void_p()
+        check(lib().bcmi_segmented_plan_create(q.h, DTYPE[np.dtype(key_dtype)], vb,
+                                               off.ctypes.data_as(C.c_void_p), off.size - 1,
+                                               int(descending), C.byref(self.h)))
+
+    def run(self, q, keys, values=None):
+        check(lib().bcmi_segmented_sort_plan(q.h, self.h, keys.ptr,
+                                             values.ptr if values is not None else None))
+
+    def __del__(self):
+        try:
+            lib().bcmi_segmented_plan_destroy(self.h)
+        except Exception:
+            pass
 
 
 class Graph:

@@ -114,31 +114,27 @@ BDEV uint64_t lb_load(const uint64_t *p)
     return __hip_atomic_load(const_cast<uint64_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Epoch of the current launch: read once per block at kernel start.
-BDEV uint32_t launch_epoch(const Ctrl *ctrl)
+// Epoch of a look-back launch.  Before a sequence of L look-back launches
+// the host enqueues k_epoch_advance(L) (one thread: ctrl->epoch += L); launch
+// j of the sequence passes back = L - j and tags its words with
+// ((epoch - back) mod (2^30 - 1)) + 1, so every launch on the queue gets a
+// distinct non-zero epoch, graph replays included.  No per-block atomic is
+// involved: device-scope atomics on one address serialise across XCDs.
+BDEV uint32_t launch_epoch(const Ctrl *ctrl, uint32_t back)
 {
     uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(&ctrl->epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return (e % kEpochMask) + 1u;
+    return ((e - back) % kEpochMask) + 1u;
 }
 
-// Dynamic tile id: tiles are handed out in the order blocks start, so every
-// tile's predecessors are resident or finished (forward progress for the
-// look-back spin).  A counter left non-zero by an aborted launch is reported
-// through ctrl->fault and folded back into range.
-BDEV uint32_t next_tile(Ctrl *ctrl)
-{
-    uint32_t t = __hip_atomic_fetch_add(&ctrl->tile_counter, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (t >= gridDim.x) {
-        __hip_atomic_store(&ctrl->fault, 2u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        t %= gridDim.x;
-    }
-    return t;
-}
+// Tiles are numbered by blockIdx.x.  Workgroups are dispatched in id order,
+// so when a block runs, every block with a smaller id has been dispatched
+// and is resident or finished: the look-back wait always makes progress.
 
-// Every look-back wait is bounded: after kSpinLimit polls (each followed by
-// s_sleep 1, ~64 clocks; about 0.1 s in total) the wait gives up, records
-// ctrl->fault and proceeds, so a protocol error can never hang the device.
-constexpr uint32_t kSpinLimit = 1u << 21;
+// Every look-back wait is bounded: after kSpinLimit polls (each a device-
+// coherent load plus s_sleep 1; on the order of a second in total) the wait
+// gives up, records ctrl->fault and proceeds, so a protocol error can never
+// hang the device.
+constexpr uint32_t kSpinLimit = 1u << 20;
 
 BDEV bool spin_expired(Ctrl *ctrl, uint32_t &spins)
 {
@@ -148,37 +144,6 @@ BDEV bool spin_expired(Ctrl *ctrl, uint32_t &spins)
     }
     __hip_atomic_store(&ctrl->fault, 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return true;
-}
-
-// Last-block protocol.  Every block calls this once at its end (all threads,
-// after its global writes).  Returns true in exactly one block, the last to
-// finish; that block has acquired every other block's writes.
-BDEV bool arrive_last(Ctrl *ctrl, uint32_t nblocks, bool *s_flag)
-{
-    // every wave releases its own global writes at agent scope before the
-    // block is counted
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        uint32_t prev = __hip_atomic_fetch_add(&ctrl->done_counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        *s_flag = (prev == nblocks - 1);
-    }
-    __syncthreads();
-    bool last = *s_flag;
-    if (last) __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return last;
-}
-
-// Called by one thread of the last block: reset the counters and advance
-// the epoch for the next look-back launch on this queue.
-BDEV void finish_launch(Ctrl *ctrl, bool advance_epoch)
-{
-    __hip_atomic_store(&ctrl->tile_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(&ctrl->done_counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    if (advance_epoch) {
-        uint32_t e = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&ctrl->epoch, e + 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    }
 }
 
 // ------------------------------------------------ error-free transforms

@@ -10,239 +10,103 @@
 //
 // Here: onesweep LSD radix with 8-bit digits.  One histogram kernel reads
 // the keys once and builds the digit histograms of every pass; then one
-// kernel per pass: each block takes the next tile, ranks its keys with
+// kernel per pass: block b takes tile b, ranks its keys with
 // wave64 ballot matching against per-wave LDS digit counters, publishes its
 // per-digit counts, gets its global digit offsets by decoupled look-back,
 // reorders the tile in LDS so each digit run is contiguous and writes keys
 // (then values) out coalesced.  The order equals the reference's: a stable
 // sort by the transformed key radix(x) (radix_sort.hpp:98-129).
-#include "device.hpp"
+#include "radix.hpp"
 
 #include <cstring>
-#include <type_traits>
 
 namespace bcmi {
 
-constexpr int kSortThreads = 256;
-constexpr int kHistThreads = 256;
-
-// radix() key transform (radix_sort.hpp:98-129) on the W-bit unsigned image
-// x of the key:   key = (neg ? -x : x) ^ (flt ? (-(x >> msb) | signbit) : cx)
-//   ascending:  unsigned cx=0; signed cx=signbit; float flt
-//   descending: unsigned cx=~0 ((T)(-1)-x == ~x); signed neg, cx=signbit;
-//               float neg, flt
+// ------------------------------------------------------------ histogram
+// All passes' digit histograms in one read of the keys.  Each block counts
+// its share in LDS and writes its partial histogram; k_digit_offsets then
+// sums the partials per digit and scans them into exclusive digit offsets
+// (ctrl->offsets).  No global atomics: device-scope atomics on shared
+// addresses serialise across XCDs.
 template <typename KB>
-struct KeyXf {
-    uint32_t neg;
-    uint32_t flt;
-    KB cx;
+struct alignas(16) kchunk {
+    static constexpr int N = 16 / sizeof(KB);
+    KB v[N];
 };
 
+// Each wave counts into its own LDS sub-histogram (fewer same-address LDS
+// atomics); keys are read as 16-byte chunks, two chunks in flight.
 template <typename KB>
-BDEV KB radix_key(KB x, const KeyXf<KB> &f)
-{
-    constexpr int W = sizeof(KB) * 8;
-    using SK = std::make_signed_t<KB>;
-    KB t = f.neg ? (KB)(KB(0) - x) : x;
-    KB sgn = (KB)(KB(1) << (W - 1));
-    KB fm = f.flt ? (KB)((KB)((SK)x >> (W - 1)) | sgn) : f.cx;
-    return (KB)(t ^ fm);
-}
-
-template <typename KB>
-BDEV uint32_t digit_of(KB x, const KeyXf<KB> &f, uint32_t shift)
-{
-    return (uint32_t)(radix_key(x, f) >> shift) & 0xFFu;
-}
-
-template <int VB>
-struct ValT;
-template <> struct ValT<0> { using type = uint8_t; };
-template <> struct ValT<1> { using type = uint8_t; };
-template <> struct ValT<2> { using type = uint16_t; };
-template <> struct ValT<4> { using type = uint32_t; };
-template <> struct ValT<8> { using type = uint64_t; };
-struct alignas(16) v16_t { uint64_t a, b; };
-template <> struct ValT<16> { using type = v16_t; };
-
-// ------------------------------------------------------------ histogram
-// All passes' digit histograms in one read of the keys.  LDS counters per
-// block, global atomics per block, and the last block turns the totals into
-// exclusive digit offsets (ctrl->offsets) and clears the totals.
-template <typename KB>
-__global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, KeyXf<KB> xf,
-                                                            Ctrl *ctrl)
+__global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
+                                                            KeyXf<KB> xf, uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
-    __shared__ uint32_t s_hist[P][256];
-    __shared__ uint32_t s_wsum[kHistThreads / 64];
-    __shared__ bool s_last;
-    for (int i = threadIdx.x; i < P * 256; i += kHistThreads) (&s_hist[0][0])[i] = 0;
+    constexpr int NW = kHistThreads / 64;
+    __shared__ uint32_t s_hist[NW][P][256];
+    for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
     __syncthreads();
-    const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
-    for (uint64_t i = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x; i < n; i += stride) {
-        KB k = radix_key(keys[i], xf);
+    uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
+    auto count = [&](KB x) {
+        KB k = radix_key(x, xf);
 #pragma unroll
-        for (int p = 0; p < P; ++p) atomicAdd(&s_hist[p][(k >> (8 * p)) & 0xFF], 1u);
+        for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+    };
+    // unaligned head and the tail: block 0
+    const uint64_t nchunks = (n - head) / kchunk<KB>::N;
+    const uint64_t tail0 = head + nchunks * kchunk<KB>::N;
+    if (blockIdx.x == 0) {
+        for (uint32_t i = threadIdx.x; i < head; i += kHistThreads) count(keys[i]);
+        for (uint64_t i = tail0 + threadIdx.x; i < n; i += kHistThreads) count(keys[i]);
+    }
+    const kchunk<KB> *body = (const kchunk<KB> *)(keys + head);
+    const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
+    uint64_t c = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x;
+    for (; c + stride < nchunks; c += 2 * stride) {
+        kchunk<KB> a = body[c], b = body[c + stride];
+#pragma unroll
+        for (int k = 0; k < kchunk<KB>::N; ++k) count(a.v[k]);
+#pragma unroll
+        for (int k = 0; k < kchunk<KB>::N; ++k) count(b.v[k]);
+    }
+    if (c < nchunks) {
+        kchunk<KB> a = body[c];
+#pragma unroll
+        for (int k = 0; k < kchunk<KB>::N; ++k) count(a.v[k]);
     }
     __syncthreads();
+    uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
     for (int i = threadIdx.x; i < P * 256; i += kHistThreads) {
-        uint32_t c = (&s_hist[0][0])[i];
-        if (c) __hip_atomic_fetch_add(&(&ctrl->hist[0][0])[i], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += (&s_hist[w][0][0])[i];
+        dst[i] = t;
     }
-    if (!arrive_last(ctrl, gridDim.x, &s_last)) return;
-    const uint32_t d = threadIdx.x, lane = d & 63, w = d >> 6;
-    for (int p = 0; p < P; ++p) {
-        uint32_t c = __hip_atomic_load(&ctrl->hist[p][d], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        uint32_t inc = wave_inclusive_scan(c, 0u, [](uint32_t a, uint32_t b) { return a + b; });
-        if (lane == 63) s_wsum[w] = inc;
-        __syncthreads();
+}
+
+// One block of 1024 threads per pass: thread (slice, d) sums digit d over a
+// quarter of the partial histograms, the slices are combined in LDS and the
+// 256 totals scanned into exclusive digit offsets.
+__global__ __launch_bounds__(1024) void k_digit_offsets(const uint32_t *__restrict__ partial, uint32_t nparts,
+                                                        uint32_t passes, Ctrl *ctrl)
+{
+    __shared__ uint32_t s_part[4][256];
+    __shared__ uint32_t s_wsum[4];
+    const uint32_t p = blockIdx.x, d = threadIdx.x & 255, slice = threadIdx.x >> 8;
+    uint32_t c = 0;
+#pragma unroll 8
+    for (uint32_t b = slice; b < nparts; b += 4) c += partial[((size_t)b * passes + p) * 256 + d];
+    s_part[slice][d] = c;
+    __syncthreads();
+    const uint32_t lane = d & 63, w = d >> 6;
+    c = s_part[0][d] + s_part[1][d] + s_part[2][d] + s_part[3][d];
+    uint32_t inc = wave_inclusive_scan(c, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+    if (slice == 0 && lane == 63) s_wsum[w] = inc;
+    __syncthreads();
+    if (slice == 0) {
         uint32_t pre = 0;
         for (uint32_t i = 0; i < w; ++i) pre += s_wsum[i];
         ctrl->offsets[p][d] = pre + inc - c;
-        __hip_atomic_store(&ctrl->hist[p][d], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __syncthreads();
     }
-    if (threadIdx.x == 0) finish_launch(ctrl, false);
-}
-
-// ------------------------------------------------------------ onesweep pass
-template <typename KB, int VB, int IPT>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep(const KB *__restrict__ kin, KB *__restrict__ kout,
-                                                           const typename ValT<VB>::type *__restrict__ vin,
-                                                           typename ValT<VB>::type *__restrict__ vout, uint32_t n,
-                                                           uint32_t shift, uint32_t pass, KeyXf<KB> xf, Ctrl *ctrl,
-                                                           uint64_t *__restrict__ status)
-{
-    using V = typename ValT<VB>::type;
-    constexpr int TILE = kSortThreads * IPT;
-    constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
-    __shared__ uint32_t s_cnt[4][256];
-    __shared__ uint32_t s_goff[256];
-    __shared__ uint32_t s_wsum[4];
-    __shared__ uint32_t s_tile;
-    __shared__ bool s_last;
-    __shared__ __attribute__((aligned(16))) unsigned char s_stage[TILE * SB];
-    KB *s_keys = (KB *)s_stage;
-    V *s_vals = (V *)s_stage;
-
-    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t epoch = launch_epoch(ctrl);
-    if (tid == 0) s_tile = next_tile(ctrl);
-    for (int i = tid; i < 4 * 256; i += kSortThreads) (&s_cnt[0][0])[i] = 0;
-    __syncthreads();
-    const uint32_t tile = s_tile;
-    const uint32_t base = tile * (uint32_t)TILE;
-    const bool full = (uint64_t)base + TILE <= n;
-    const uint32_t wb = base + w * 64 * IPT + lane;
-
-    // wave-striped load: lane l of wave w holds tile positions w*64*IPT + j*64 + l
-    KB k[IPT];
-    V v[IPT];
-    uint32_t d[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        uint32_t idx = wb + j * 64;
-        bool ok = full || idx < n;
-        k[j] = ok ? kin[idx] : KB(0);
-        if constexpr (VB > 0) {
-            if (ok) v[j] = vin[idx];
-        }
-    }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) d[j] = digit_of(k[j], xf, shift);
-
-    // rank within the wave: peers = lanes whose digit matches (8 ballots);
-    // rank = wave counter + peers below; the lowest peer bumps the counter.
-    uint32_t r[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        const bool ok = full || (wb + j * 64 < n);
-        uint64_t peers = full ? ~0ull : ballot(ok);
-#pragma unroll
-        for (int b = 0; b < 8; ++b) {
-            uint32_t nb = ((d[j] >> b) & 1u) - 1u;  // 0 if the bit is set, ~0 if clear
-            uint64_t m = ballot(nb == 0u);
-            uint64_t sel = m ^ (((uint64_t)nb << 32) | nb);
-            peers &= sel;
-        }
-        uint32_t below = mbcnt(peers);
-        uint32_t old = s_cnt[w][d[j]];
-        r[j] = old + below;
-        if (ok && below == 0) s_cnt[w][d[j]] = old + popc64(peers);
-    }
-    __syncthreads();
-
-    // per digit (thread tid == digit): wave prefixes and the tile count
-    const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
-    const uint32_t total = c0 + c1 + c2 + c3;
-    if (tile == 0) lb_store(&status[tid], lb_word(epoch, LB_P, total));
-    else lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_A, total));
-    uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
-    if (lane == 63) s_wsum[w] = inc;
-    __syncthreads();
-    uint32_t lstart = inc - total;
-    for (uint32_t i = 0; i < w; ++i) lstart += s_wsum[i];
-    s_cnt[0][tid] = lstart;
-    s_cnt[1][tid] = lstart + c0;
-    s_cnt[2][tid] = lstart + c0 + c1;
-    s_cnt[3][tid] = lstart + c0 + c1 + c2;
-    // decoupled look-back for this digit
-    uint32_t excl = 0;
-    if (tile > 0) {
-        size_t kt = (size_t)tile - 1;
-        uint32_t spins = 0;
-        while (true) {
-            uint64_t wd = lb_load(&status[kt * 256 + tid]);
-            if (lb_epoch(wd) != epoch) {
-                if (!spin_expired(ctrl, spins)) continue;
-                break;
-            }
-            excl += lb_payload(wd);
-            if (lb_flag(wd) == LB_P || kt == 0) break;
-            --kt;
-        }
-        lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_P, excl + total));
-    }
-    s_goff[tid] = ctrl->offsets[pass][tid] + excl - lstart;
-    __syncthreads();
-
-    // reorder the tile by digit in LDS, then write out coalesced
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        if (full || wb + j * 64 < n) s_keys[s_cnt[w][d[j]] + r[j]] = k[j];
-    }
-    __syncthreads();
-    const uint32_t tile_n = full ? (uint32_t)TILE : n - base;
-    uint32_t od[IPT];
-#pragma unroll
-    for (int j = 0; j < IPT; ++j) {
-        uint32_t i = j * kSortThreads + tid;
-        if (i < tile_n) {
-            KB key = s_keys[i];
-            od[j] = digit_of(key, xf, shift);
-            uint32_t dst = s_goff[od[j]] + i;
-            if (dst < n) kout[dst] = key;
-            else __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        }
-    }
-    if constexpr (VB > 0) {
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            if (full || wb + j * 64 < n) s_vals[s_cnt[w][d[j]] + r[j]] = v[j];
-        }
-        __syncthreads();
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) {
-            uint32_t i = j * kSortThreads + tid;
-            if (i < tile_n) {
-                uint32_t dst = s_goff[od[j]] + i;
-                if (dst < n) vout[dst] = s_vals[i];
-            }
-        }
-    }
-    if (arrive_last(ctrl, gridDim.x, &s_last) && tid == 0) finish_launch(ctrl, true);
 }
 
 // ------------------------------------------------------- insertion sort
@@ -290,7 +154,7 @@ static KeyXf<KB> make_xf(int kind, int descending)
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <typename KB, int VB, int IPT>
+template <typename KB, int VB, int IPT, int RANK>
 static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int kind, int descending)
 {
     using V = typename ValT<VB>::type;
@@ -300,31 +164,41 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
     V *vals = (V *)vals_;
     const KeyXf<KB> xf = make_xf<KB>(kind, descending);
     const uint32_t tiles = (n + TILE - 1) / TILE;
+    uint32_t hblocks = (n + kHistThreads * 32 - 1) / (kHistThreads * 32);
+    uint32_t hmax = (uint32_t)q->cu_count * 4;
+    if (hblocks > hmax) hblocks = hmax;
+    if (hblocks < 1) hblocks = 1;
+    uint32_t head = 0;
+    if ((uintptr_t)keys & 15) head = (uint32_t)((16 - ((uintptr_t)keys & 15)) / sizeof(KB));
+    if (((uintptr_t)keys & 15) % sizeof(KB)) head = n;  // not element-aligned to a chunk
+    if (head > n) head = n;
     void *scr;
     const size_t kbytes = align256((size_t)n * sizeof(KB));
-    BCMI_TRY(scratch(q, kbytes + (VB ? (size_t)n * VB : 0) + 256, &scr));
+    const size_t vbytes = align256(VB ? (size_t)n * VB : 0);
+    BCMI_TRY(scratch(q, kbytes + vbytes + (size_t)hblocks * P * 256 * 4, &scr));
     KB *ktmp = (KB *)scr;
     V *vtmp = (V *)((char *)scr + kbytes);
+    uint32_t *partial = (uint32_t *)((char *)scr + kbytes + vbytes);
     uint64_t *status;
     BCMI_TRY(status_array(q, (size_t)tiles * 256, &status));
 
-    uint32_t hblocks = (n + kHistThreads * 16 - 1) / (kHistThreads * 16);
-    uint32_t hmax = (uint32_t)q->cu_count * 2;
-    if (hblocks > hmax) hblocks = hmax;
-    if (hblocks < 1) hblocks = 1;
     BCMI_LAUNCH(q, "radix_histogram",
-                hipLaunchKernelGGL((k_histogram<KB>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, xf,
-                                   q->ctrl));
+                hipLaunchKernelGGL((k_histogram<KB>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
+                                   xf, partial));
+    BCMI_LAUNCH(q, "radix_digit_offsets",
+                hipLaunchKernelGGL(k_digit_offsets, dim3(P), dim3(1024), 0, q->stream, partial, hblocks,
+                                   (uint32_t)P, q->ctrl));
+    BCMI_TRY(begin_lookback(q, P));
     KB *ks = keys, *kd = ktmp;
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_onesweep<KB, VB, IPT>), dim3(tiles), dim3(kSortThreads), 0, q->stream, ks,
-                                       kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status));
+                    hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK>), dim3(tiles), dim3(kSortThreads), 0, q->stream, ks,
+                                       kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
+                                       (uint32_t)(P - p)));
         std::swap(ks, kd);
         std::swap(vs, vd);
     }
-    BCMI_TRY(note_lookback_launch(q, P));
     if (ks != keys) {
         // odd number of passes (8-bit keys): the result is in the temporary
         BCMI_HIP(hipMemcpyAsync(keys, ks, (size_t)n * sizeof(KB), hipMemcpyDeviceToDevice, q->stream));
@@ -333,12 +207,28 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
     return 0;
 }
 
+// Tile size (items per thread) and ranking method come from the tuning cache
+// (key "__bcmi_radix_sort", params "ipt" and "rank"); the variants are
+// compiled for the benchmarked 4-byte keys with 0- or 4-byte values.  The
+// default of 32 items per thread with LDS peer-mask ranking was the fastest
+// measured on 2^27 uint32 keys (16: 3.00 ms, 24: 2.39 ms, 32: 2.25 ms).
 template <typename KB, int VB>
 static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc)
 {
     constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
     constexpr int IPT = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
-    return radix_sort<KB, VB, IPT>(q, keys, vals, n, kind, desc);
+    if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
+        const unsigned ipt = tune_get("__bcmi_radix_sort", "ipt", 32);
+        const unsigned rank = tune_get("__bcmi_radix_sort", "rank", kRankLdsMask);
+        if (rank == kRankBallot) {
+            if (ipt == 24) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
+            return radix_sort<KB, VB, 16, kRankBallot>(q, keys, vals, n, kind, desc);
+        }
+        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc);
+        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc);
+        // any other value falls through to the generic tile below
+    }
+    return radix_sort<KB, VB, IPT, kRankLdsMask>(q, keys, vals, n, kind, desc);
 }
 
 template <typename KB>

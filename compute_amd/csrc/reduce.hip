@@ -9,8 +9,8 @@
 // Here: one launch.  A grid of a few blocks per CU streams the input with
 // 128-bit loads; each thread keeps a compensated FP32 pair (TwoSum) for
 // float plus, waves combine pairs with DPP, the block writes one FP64
-// partial, and the last block to finish folds the partials in FP64 in a
-// fixed order (so the result is deterministic).  Integers wrap exactly;
+// partial, and a one-block kernel folds the partials in FP64 in a fixed
+// order (so the result is deterministic).  Integers wrap exactly;
 // min/max/multiplies fold in the element type.
 #include "device.hpp"
 #include "ops.hpp"
@@ -126,15 +126,10 @@ BDEV S block_combine(S v, S *s_buf)
 
 template <typename T, typename P>
 __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in, size_t n, size_t head,
-                                                        size_t nchunks, Ctrl *ctrl,
-                                                        typename P::Part *__restrict__ partials, T init,
-                                                        int has_init, T *out, double *out_f64)
+                                                        size_t nchunks, typename P::Part *__restrict__ partials)
 {
     using S = typename P::State;
-    using Pt = typename P::Part;
     __shared__ S s_buf[kRedThreads];
-    __shared__ Pt s_part[kRedThreads];
-    __shared__ bool s_last;
 
     S acc = P::id();
     // head/tail elements outside the aligned 16-byte body: block 0
@@ -161,11 +156,19 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in
     }
     S blk = block_combine<S, P>(acc, s_buf);
     if (threadIdx.x == 0) partials[blockIdx.x] = P::part(blk);
+}
 
-    if (!arrive_last(ctrl, gridDim.x, &s_last)) return;
-    // Last block: fold the partials in FP64 / the element type, fixed order.
+// One block folds the per-block partials in FP64 / the element type in a
+// fixed order, so the result is deterministic.
+template <typename T, typename P>
+__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const typename P::Part *__restrict__ partials,
+                                                              uint32_t nparts, T init, int has_init, T *out,
+                                                              double *out_f64)
+{
+    using Pt = typename P::Part;
+    __shared__ Pt s_part[kRedThreads];
     Pt p = P::part_id();
-    for (uint32_t i = threadIdx.x; i < gridDim.x; i += kRedThreads) p = P::part_merge(p, partials[i]);
+    for (uint32_t i = threadIdx.x; i < nparts; i += kRedThreads) p = P::part_merge(p, partials[i]);
     s_part[threadIdx.x] = p;
     __syncthreads();
     for (int off = kRedThreads / 2; off > 0; off >>= 1) {
@@ -176,7 +179,6 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in
         Pt total = s_part[0];
         *out = P::finish(total, has_init != 0, init);
         if (out_f64) *out_f64 = P::f64(total) + (has_init ? (double)init : 0.0);
-        finish_launch(ctrl, false);
     }
 }
 
@@ -201,7 +203,10 @@ static int run_reduce(bcmi_queue q, const void *in_, size_t n, const void *init,
     if (init) memcpy(&ini, init, sizeof(T));
     BCMI_LAUNCH(q, "reduce",
                 hipLaunchKernelGGL((k_reduce<T, P>), dim3((unsigned)blocks), dim3(kRedThreads), 0, q->stream, in, n,
-                                   head, nchunks, q->ctrl, (typename P::Part *)scr, ini, init ? 1 : 0, (T *)out,
+                                   head, nchunks, (typename P::Part *)scr));
+    BCMI_LAUNCH(q, "reduce_final",
+                hipLaunchKernelGGL((k_reduce_final<T, P>), dim3(1), dim3(kRedThreads), 0, q->stream,
+                                   (const typename P::Part *)scr, (uint32_t)blocks, ini, init ? 1 : 0, (T *)out,
                                    out_f64));
     return 0;
 }

@@ -110,16 +110,26 @@ int status_array(bcmi_queue q, size_t words, uint64_t **ptr)
     return 0;
 }
 
-int note_lookback_launch(bcmi_queue q, int launches)
+__global__ void k_epoch_advance(Ctrl *ctrl, uint32_t count)
+{
+    if (threadIdx.x == 0) {
+        uint32_t e = __hip_atomic_load(&ctrl->epoch, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&ctrl->epoch, e + count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+}
+
+int begin_lookback(bcmi_queue q, int launches)
 {
     // Tags carry a 30-bit epoch; clear the status words well before an
     // epoch value can repeat.
     if (q->capturing) {
         q->capture_lookback_launches += (uint64_t)launches;
-        return 0;
+    } else {
+        q->lookback_launches += (uint64_t)launches;
+        if (q->lookback_launches > (1ull << 29)) BCMI_TRY(zero_status(q));
     }
-    q->lookback_launches += (uint64_t)launches;
-    if (q->lookback_launches > (1ull << 29)) return zero_status(q);
+    hipLaunchKernelGGL(k_epoch_advance, dim3(1), dim3(64), 0, q->stream, q->ctrl, (uint32_t)launches);
+    BCMI_HIP(hipGetLastError());
     return 0;
 }
 
@@ -285,9 +295,8 @@ int bcmi_queue_finish(bcmi_queue q)
         *fault = 0;
         BCMI_HIP(hipMemcpyAsync(&q->ctrl->fault, fault, sizeof(uint32_t), hipMemcpyHostToDevice, q->stream));
         BCMI_HIP(hipStreamSynchronize(q->stream));
-        set_error(f == 1   ? "a look-back wait exceeded its spin limit; results of the last calls are invalid"
-                  : f == 2 ? "stale tile counter detected; results of the last calls are invalid"
-                           : "an output index fell outside the output range and was dropped; results are invalid");
+        set_error(f == 1 ? "a look-back wait exceeded its spin limit; results of the last calls are invalid"
+                         : "an output index fell outside the output range and was dropped; results are invalid");
         return BCMI_OUT_OF_RESOURCES;
     }
     return 0;
@@ -430,7 +439,8 @@ int bcmi_graph_launch(bcmi_graph g, bcmi_queue q)
     BCMI_TRY(use_queue(q));
     BCMI_CHECK(g && g->exec, BCMI_INVALID_VALUE, "null graph");
     BCMI_HIP(hipGraphLaunch(g->exec, q->stream));
-    BCMI_TRY(note_lookback_launch(q, (int)g->lookback_launches));
+    q->lookback_launches += g->lookback_launches;
+    if (q->lookback_launches > (1ull << 29)) BCMI_TRY(zero_status(q));
     return 0;
 }
 

@@ -11,8 +11,8 @@
 //     digit offsets).  Status words are tagged with an epoch that the last
 //     block of every look-back kernel advances on the device, so the array
 //     is never cleared between calls and graph replays stay correct;
-//   * a small device control block (epoch, tile/done counters, digit
-//     histograms) that every kernel leaves zeroed for the next one.
+//   * a small device control block: the look-back epoch, a fault flag and
+//     the radix digit offsets of the current sort.
 #pragma once
 
 #include <hip/hip_runtime.h>
@@ -29,12 +29,10 @@ namespace bcmi {
 
 // Device control block, zero-initialised at queue creation.
 struct Ctrl {
-    uint32_t epoch;          // look-back epoch counter
-    uint32_t tile_counter;   // dynamic tile id dispenser
-    uint32_t done_counter;   // finished-block counter
-    uint32_t fault;          // set by a kernel whose look-back spin limit expired
-    uint32_t hist[8][256];   // radix digit histograms (self-clearing)
-    uint32_t offsets[8][256];// exclusive digit offsets per pass
+    uint32_t epoch;          // look-back epoch counter (advanced by k_epoch_advance)
+    uint32_t fault;          // set by a kernel whose look-back wait expired or whose output index was out of range
+    uint32_t pad[2];
+    uint32_t offsets[8][256];// exclusive digit offsets per radix pass
 };
 
 constexpr uint32_t kEpochMask = (1u << 30) - 1u;
@@ -109,8 +107,9 @@ int use_queue(bcmi_queue q);
 int scratch(bcmi_queue q, size_t bytes, void **ptr);
 // Look-back status array of at least `words` uint64.
 int status_array(bcmi_queue q, size_t words, uint64_t **ptr);
-// Account for one launch of a look-back kernel (epoch wrap bookkeeping).
-int note_lookback_launch(bcmi_queue q, int launches);
+// Reserve epochs for the next `launches` look-back launches on the queue:
+// enqueues the epoch advance and does the wrap bookkeeping.
+int begin_lookback(bcmi_queue q, int launches);
 
 // Profiling brackets around a kernel launch on q->stream.
 void prof_begin(bcmi_queue q, const char *name, ProfRec *rec);

@@ -6,8 +6,8 @@
 // bytes of traffic per element over 4+ launches; in-place scans first copy
 // the input, :292-303).
 //
-// Here: one launch, single pass, 8 bytes per element.  Each block takes the
-// next tile id from a counter, scans its tile in registers (per-thread
+// Here: one launch, single pass, 8 bytes per element.  Block b scans tile b
+// in registers (per-thread
 // chunk folds, DPP wave scans, an LDS step across the block's 4 waves), and
 // obtains the tile's exclusive prefix by decoupled look-back over the
 // predecessors' published aggregates.  Integer scans wrap exactly.  Float
@@ -22,7 +22,6 @@
 namespace bcmi {
 
 constexpr int kScanThreads = 256;
-constexpr int kScanQ = 4;  // 16-byte chunks per thread
 
 // Integer / plain policy.  Narrow types compute in 32 bits (DPP moves 32 or
 // 64 bits) and truncate on output, which keeps modular results exact.
@@ -70,25 +69,23 @@ struct alignas(16) sc_chunk {
     T v[N];
 };
 
-template <typename T, typename P, bool VEC>
+// Q = 16-byte chunks per thread (tile = 4 waves * 64 lanes * Q chunks).
+template <typename T, typename P, bool VEC, int kScanQ>
 __global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size_t n, Ctrl *ctrl,
-                                                       uint64_t *status, typename P::Cr init, int exclusive)
+                                                       uint64_t *status, typename P::Cr init, int exclusive,
+                                                       uint32_t back)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
     constexpr int V = 16 / sizeof(T);
     constexpr int WSEG = kScanQ * 64 * V;   // elements per wave
     constexpr int TILE = 4 * WSEG;          // 4 waves
-    __shared__ uint32_t s_tile;
     __shared__ S s_wave[4];
     __shared__ Cr s_prefix;
-    __shared__ bool s_last;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t epoch = launch_epoch(ctrl);
-    if (tid == 0) s_tile = next_tile(ctrl);
-    __syncthreads();
-    const uint32_t tile = s_tile;
+    const uint32_t epoch = launch_epoch(ctrl, back);
+    const uint32_t tile = blockIdx.x;
     const size_t tbase = (size_t)tile * TILE;
     const bool full = tbase + TILE <= n;
     const size_t wbase = tbase + (size_t)w * WSEG + (size_t)lane * V;
@@ -166,32 +163,47 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size
                 if (p0 + k < n) out[p0 + k] = y[k];
         }
     }
-    if (arrive_last(ctrl, gridDim.x, &s_last) && tid == 0) finish_launch(ctrl, true);
 }
 
+template <typename T, typename P, int Q>
+static int launch_scan(bcmi_queue q, const T *in, T *out, size_t n, uint64_t *status, typename P::Cr ini,
+                       int exclusive)
+{
+    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
+    size_t tiles = (n + TILE - 1) / TILE;
+    BCMI_CHECK(tiles < (1ull << 31), BCMI_INVALID_BUFFER_SIZE, "scan: n=%zu too large", n);
+    bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
+    BCMI_TRY(begin_lookback(q, 1));
+    if (vec) {
+        BCMI_LAUNCH(q, "scan",
+                    hipLaunchKernelGGL((k_scan<T, P, true, Q>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                                       q->stream, in, out, n, q->ctrl, status, ini, exclusive, 1u));
+    } else {
+        BCMI_LAUNCH(q, "scan",
+                    hipLaunchKernelGGL((k_scan<T, P, false, Q>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                                       q->stream, in, out, n, q->ctrl, status, ini, exclusive, 1u));
+    }
+    return 0;
+}
+
+// Chunks per thread come from the tuning cache (key "__bcmi_scan", param
+// "q": 4, 8 or 16) for 4-byte types; 4 otherwise.
 template <typename T, typename P>
 static int run_scan(bcmi_queue q, const void *in_, void *out_, size_t n, const void *init, int exclusive)
 {
-    constexpr size_t TILE = (size_t)4 * kScanQ * 64 * (16 / sizeof(T));
     const T *in = (const T *)in_;
     T *out = (T *)out_;
-    size_t tiles = (n + TILE - 1) / TILE;
-    BCMI_CHECK(tiles < (1ull << 31), BCMI_INVALID_BUFFER_SIZE, "scan: n=%zu too large", n);
+    const size_t tiles_max = (n + (size_t)4 * 4 * 64 - 1) / ((size_t)4 * 4 * 64) + 1;
     uint64_t *status;
-    BCMI_TRY(status_array(q, tiles * (sizeof(typename P::Cr) / 4), &status));
+    BCMI_TRY(status_array(q, tiles_max * (sizeof(typename P::Cr) / 4), &status));
     typename P::Cr ini = P::cid();
     if (exclusive && init) ini = P::host_init(init);
-    bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
-    if (vec) {
-        BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, true>), dim3((unsigned)tiles), dim3(kScanThreads), 0, q->stream,
-                                       in, out, n, q->ctrl, status, ini, exclusive));
-    } else {
-        BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, false>), dim3((unsigned)tiles), dim3(kScanThreads), 0, q->stream,
-                                       in, out, n, q->ctrl, status, ini, exclusive));
+    if constexpr (sizeof(T) == 4) {
+        const unsigned qq = tune_get("__bcmi_scan", "q", 16);
+        if (qq == 16) return launch_scan<T, P, 16>(q, in, out, n, status, ini, exclusive);
+        if (qq == 8) return launch_scan<T, P, 8>(q, in, out, n, status, ini, exclusive);
     }
-    return note_lookback_launch(q, 1);
+    return launch_scan<T, P, 4>(q, in, out, n, status, ini, exclusive);
 }
 
 template <typename T>

@@ -1,12 +1,528 @@
-// segmented.hip -- batched segmented sort (filled in below).
-#include "device.hpp"
+// segmented.hip -- batched segmented sort.
+//
+// Semantics: every segment [offsets[s], offsets[s+1]) ends up exactly as one
+// stable_sort_by_key call on it would leave it in the reference's GPU
+// column (stable_sort_by_key.hpp:29-77): below 32 elements the one-work-item
+// insertion sort (insertion_sort.hpp:106-133, which only differs from radix
+// order for float keys: it keeps -0.0/+0.0 in input order), otherwise the
+// stable radix order of the radix() key transform (radix_sort.hpp:98-129).
+//
+// The reference has no batched primitive: the per-segment alternative is one
+// sort call per segment (several launches each).  Here a plan built once
+// from the host offsets routes the segments to three paths, each one launch
+// (or one launch per 8-bit pass) for all segments together:
+//   * small (<= 4096 elements): one workgroup per segment sorts it entirely
+//     in LDS -- all digit passes, LDS-mask ranking, LDS scatter/gather --
+//     so the segment's keys are read and written once;
+//   * float keys shorter than 32: the reference's insertion sort, one
+//     thread per segment;
+//   * large: the onesweep pass kernel over a table of tiles of all large
+//     segments, with per-segment digit histograms and look-back chains that
+//     restart at every segment's first tile.
+// A plan owns its tables and temporaries, so running it allocates nothing
+// and can be captured in a hipGraph and replayed.
+#include "radix.hpp"
+
+#include <cstring>
+#include <vector>
+
+namespace bcmi {
+
+constexpr int kSegSmallThreads = 256;
+constexpr int kSegSmallLds = 48 * 1024;  // key + value bytes a small-segment block keeps in LDS
+constexpr uint32_t kSegChunk = 1u << 16;  // keys per histogram chunk of a large segment
+
+struct SegSpan {
+    uint64_t base;
+    uint32_t len;
+    uint32_t idx;  // large-segment index (tiles, chunks) or unused
+};
+
+struct SegTile {
+    uint64_t base;
+    uint32_t len;
+    uint32_t tile;  // tile index within the segment
+    uint32_t seg;   // large-segment index
+    uint32_t pad;
+};
+
+// ------------------------------------------------------------ small path
+template <typename KB, int VB, int IPT>
+__global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__ keys,
+                                                                typename ValT<VB>::type *__restrict__ vals,
+                                                                const SegSpan *__restrict__ segs, KeyXf<KB> xf)
+{
+    using V = typename ValT<VB>::type;
+    constexpr int P = sizeof(KB);
+    constexpr int TILE = kSegSmallThreads * IPT;
+    __shared__ KB s_k[TILE];
+    __shared__ V s_v[VB ? TILE : 1];
+    __shared__ uint32_t s_cnt[4][256];
+    __shared__ uint64_t s_peer[4][256];
+    __shared__ uint32_t s_wsum[4];
+    __shared__ uint32_t s_uniform;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const SegSpan sg = segs[blockIdx.x];
+    KB *kp = keys + sg.base;
+    V *vp = VB ? vals + sg.base : nullptr;
+    const uint32_t len = sg.len;
+    auto pos = [&](int j) { return w * 64 * IPT + j * 64 + lane; };
+    auto valid = [&](int j) { return pos(j) < len; };
+
+    KB k[IPT];
+    V v[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        k[j] = valid(j) ? kp[pos(j)] : KB(0);
+        if constexpr (VB > 0) {
+            if (valid(j)) v[j] = vp[pos(j)];
+        }
+    }
+    for (int i = tid; i < 4 * 256; i += kSegSmallThreads) (&s_peer[0][0])[i] = 0;
+
+    for (int p = 0; p < P; ++p) {
+        uint32_t d[IPT], r[IPT];
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) d[j] = digit_of(k[j], xf, 8 * p);
+        for (int i = lane; i < 256; i += 64) s_cnt[w][i] = 0;
+        __syncthreads();
+        // every thread read last pass's flag before the barrier above
+        if (tid == 0) s_uniform = 0;
+        rank_in_wave<IPT>(d, r, s_cnt[w], s_peer[w], valid);
+        __syncthreads();
+        const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
+        const uint32_t total = c0 + c1 + c2 + c3;
+        if (total == len) s_uniform = 1;
+        const uint32_t lstart = block_exclusive_sum256(total, s_wsum);
+        s_cnt[0][tid] = lstart;
+        s_cnt[1][tid] = lstart + c0;
+        s_cnt[2][tid] = lstart + c0 + c1;
+        s_cnt[3][tid] = lstart + c0 + c1 + c2;
+        __syncthreads();
+        if (s_uniform) continue;  // every key has the same digit: identity pass
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (valid(j)) {
+                const uint32_t dst = s_cnt[w][d[j]] + r[j];
+                s_k[dst] = k[j];
+                if constexpr (VB > 0) s_v[dst] = v[j];
+            }
+        }
+        __syncthreads();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (valid(j)) {
+                k[j] = s_k[pos(j)];
+                if constexpr (VB > 0) v[j] = s_v[pos(j)];
+            }
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        if (valid(j)) {
+            kp[pos(j)] = k[j];
+            if constexpr (VB > 0) vp[pos(j)] = v[j];
+        }
+    }
+}
+
+// ------------------------------------------------- tiny float segments
+// serial_insertion_sort_by_key (insertion_sort.hpp:106-133) on the native
+// type, one thread per segment.
+template <typename T, int VB>
+__global__ void k_seg_insertion(T *keys, typename ValT<VB>::type *vals, const SegSpan *segs, uint32_t nsegs,
+                                int descending)
+{
+    using V = typename ValT<VB>::type;
+    const uint32_t s = blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= nsegs) return;
+    T *kp = keys + segs[s].base;
+    V *vp = VB ? vals + segs[s].base : nullptr;
+    const uint32_t n = segs[s].len;
+    for (uint32_t i = 1; i < n; ++i) {
+        const T key = kp[i];
+        V val{};
+        if constexpr (VB > 0) val = vp[i];
+        uint32_t q = i;
+        while (q > 0 && (descending ? (key > kp[q - 1]) : (key < kp[q - 1]))) {
+            kp[q] = kp[q - 1];
+            if constexpr (VB > 0) vp[q] = vp[q - 1];
+            q--;
+        }
+        kp[q] = key;
+        if constexpr (VB > 0) vp[q] = val;
+    }
+}
+
+// ------------------------------------------------------------ large path
+// One block per histogram chunk (<= kSegChunk keys of one segment).
+template <typename KB>
+__global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__restrict__ keys,
+                                                                const SegSpan *__restrict__ chunks, KeyXf<KB> xf,
+                                                                uint32_t *__restrict__ partial)
+{
+    constexpr int P = sizeof(KB);
+    constexpr int NW = kHistThreads / 64;
+    __shared__ uint32_t s_hist[NW][P][256];
+    for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
+    __syncthreads();
+    const SegSpan c = chunks[blockIdx.x];
+    uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
+    for (uint32_t i = threadIdx.x; i < c.len; i += kHistThreads) {
+        KB k = radix_key(keys[c.base + i], xf);
+#pragma unroll
+        for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+    }
+    __syncthreads();
+    uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
+    for (int i = threadIdx.x; i < P * 256; i += kHistThreads) {
+        uint32_t t = 0;
+#pragma unroll
+        for (int w = 0; w < NW; ++w) t += (&s_hist[w][0][0])[i];
+        dst[i] = t;
+    }
+}
+
+// grid (large segments, passes): digit d of segment s summed over its chunks
+// [chunk_begin[s], chunk_begin[s+1]) and scanned into exclusive offsets.
+__global__ __launch_bounds__(256) void k_seg_digit_offsets(const uint32_t *__restrict__ partial,
+                                                           const uint32_t *__restrict__ chunk_begin, uint32_t passes,
+                                                           uint32_t *__restrict__ seg_off)
+{
+    __shared__ uint32_t s_wsum[4];
+    const uint32_t s = blockIdx.x, p = blockIdx.y, d = threadIdx.x;
+    uint32_t c = 0;
+    for (uint32_t b = chunk_begin[s]; b < chunk_begin[s + 1]; ++b) c += partial[((size_t)b * passes + p) * 256 + d];
+    seg_off[((size_t)s * passes + p) * 256 + d] = block_exclusive_sum256(c, s_wsum);
+}
+
+template <typename KB, int VB, int IPT>
+__global__ __launch_bounds__(kSortThreads) void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
+                                                               const typename ValT<VB>::type *__restrict__ vin,
+                                                               typename ValT<VB>::type *__restrict__ vout,
+                                                               const SegTile *__restrict__ tiles, uint32_t shift,
+                                                               uint32_t pass, uint32_t passes,
+                                                               const uint32_t *__restrict__ seg_off, KeyXf<KB> xf,
+                                                               Ctrl *ctrl, uint64_t *__restrict__ status,
+                                                               uint32_t back)
+{
+    const SegTile t = tiles[blockIdx.x];
+    const size_t b = t.base;
+    onesweep_tile<KB, VB, IPT, kRankLdsMask>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
+                                             t.tile, shift, seg_off + ((size_t)t.seg * passes + pass) * 256, xf,
+                                             ctrl, status + (size_t)(blockIdx.x - t.tile) * 256,
+                                             launch_epoch(ctrl, back));
+}
+
+// odd pass count (8-bit keys): copy the large segments back from the temporaries
+template <typename KB, int VB>
+__global__ void k_seg_copyback(KB *keys, const KB *ktmp, typename ValT<VB>::type *vals,
+                               const typename ValT<VB>::type *vtmp, const SegSpan *segs)
+{
+    const SegSpan s = segs[blockIdx.x];
+    for (uint32_t i = threadIdx.x; i < s.len; i += blockDim.x) {
+        keys[s.base + i] = ktmp[s.base + i];
+        if constexpr (VB > 0) vals[s.base + i] = vtmp[s.base + i];
+    }
+}
+
+static size_t al256(size_t x) { return (x + 255) & ~(size_t)255; }
+
+}  // namespace bcmi
+
+// A plan: host-side routing of the segments, uploaded once.
+struct bcmi_segplan_s {
+    int device = 0;
+    int key_dtype = 0;
+    size_t value_bytes = 0;
+    int descending = 0;
+    uint64_t total = 0;  // elements covered by the offsets
+    int small_ipt_max = 16;
+    uint32_t nsmall[5] = {0, 0, 0, 0, 0};  // classes IPT 1, 2, 4, 8, 16
+    uint32_t ninsert = 0;
+    uint32_t nlarge = 0, ntiles = 0, nchunks = 0;
+    // device memory (one allocation)
+    void *mem = nullptr;
+    bcmi::SegSpan *small[5] = {nullptr, nullptr, nullptr, nullptr, nullptr};
+    bcmi::SegSpan *insert = nullptr;
+    bcmi::SegSpan *large = nullptr;
+    bcmi::SegSpan *chunks = nullptr;
+    uint32_t *chunk_begin = nullptr;
+    bcmi::SegTile *tiles = nullptr;
+    uint32_t *partial = nullptr;
+    uint32_t *seg_off = nullptr;
+    void *ktmp = nullptr;
+    void *vtmp = nullptr;
+};
+
+namespace bcmi {
+
+static int large_ipt(int kb, size_t vb)
+{
+    size_t sb = (size_t)kb > vb ? (size_t)kb : vb;
+    return sb >= 16 ? 8 : (sb >= 8 ? 12 : 16);
+}
+
+template <typename KB, int VB>
+static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
+{
+    using V = typename ValT<VB>::type;
+    constexpr int P = sizeof(KB);
+    constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+    constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
+    KB *keys = (KB *)keys_;
+    V *vals = (V *)vals_;
+    const int kind = pl->key_dtype == BCMI_F32 || pl->key_dtype == BCMI_F64 ? 2
+                   : (pl->key_dtype == BCMI_I8 || pl->key_dtype == BCMI_I16 || pl->key_dtype == BCMI_I32 ||
+                      pl->key_dtype == BCMI_I64) ? 1 : 0;
+    KeyXf<KB> xf;
+    {
+        const KB sgn = (KB)((KB)1 << (sizeof(KB) * 8 - 1));
+        xf.neg = 0;
+        xf.flt = 0;
+        xf.cx = 0;
+        if (!pl->descending) {
+            if (kind == 1) xf.cx = sgn;
+            if (kind == 2) xf.flt = 1;
+        } else {
+            if (kind == 0) xf.cx = (KB)~(KB)0;
+            if (kind == 1) { xf.neg = 1; xf.cx = sgn; }
+            if (kind == 2) { xf.neg = 1; xf.flt = 1; }
+        }
+    }
+    // small classes
+#define BCMI_SMALL(CLS, IPTV)                                                                                   \
+    if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
+        if (pl->nsmall[CLS])                                                                                    \
+            BCMI_LAUNCH(q, "segsort_small",                                                                     \
+                        hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV>), dim3(pl->nsmall[CLS]),                  \
+                                           dim3(kSegSmallThreads), 0, q->stream, keys, vals, pl->small[CLS], xf)); \
+    }
+    BCMI_SMALL(0, 1)
+    BCMI_SMALL(1, 2)
+    BCMI_SMALL(2, 4)
+    BCMI_SMALL(3, 8)
+    BCMI_SMALL(4, 16)
+#undef BCMI_SMALL
+    if (pl->ninsert) {
+        const unsigned blocks = (pl->ninsert + 63) / 64;
+        if (pl->key_dtype == BCMI_F32) {
+            if constexpr (sizeof(KB) == 4)
+                BCMI_LAUNCH(q, "segsort_insertion",
+                            hipLaunchKernelGGL((k_seg_insertion<float, VB>), dim3(blocks), dim3(64), 0, q->stream,
+                                               (float *)keys, vals, pl->insert, pl->ninsert, pl->descending));
+        } else {
+            if constexpr (sizeof(KB) == 8)
+                BCMI_LAUNCH(q, "segsort_insertion",
+                            hipLaunchKernelGGL((k_seg_insertion<double, VB>), dim3(blocks), dim3(64), 0, q->stream,
+                                               (double *)keys, vals, pl->insert, pl->ninsert, pl->descending));
+        }
+    }
+    if (pl->nlarge) {
+        BCMI_LAUNCH(q, "segsort_histogram",
+                    hipLaunchKernelGGL((k_seg_histogram<KB>), dim3(pl->nchunks), dim3(kHistThreads), 0, q->stream,
+                                       keys, pl->chunks, xf, pl->partial));
+        BCMI_LAUNCH(q, "segsort_digit_offsets",
+                    hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, P), dim3(256), 0, q->stream, pl->partial,
+                                       pl->chunk_begin, (uint32_t)P, pl->seg_off));
+        uint64_t *status;
+        BCMI_TRY(status_array(q, (size_t)pl->ntiles * 256, &status));
+        BCMI_TRY(begin_lookback(q, P));
+        KB *ks = keys, *kd = (KB *)pl->ktmp;
+        V *vs = vals, *vd = (V *)pl->vtmp;
+        for (int p = 0; p < P; ++p) {
+            BCMI_LAUNCH(q, "segsort_onesweep",
+                        hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL>), dim3(pl->ntiles), dim3(kSortThreads), 0,
+                                           q->stream, ks, kd, vs, vd, pl->tiles, (uint32_t)(8 * p), (uint32_t)p,
+                                           (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p)));
+            std::swap(ks, kd);
+            std::swap(vs, vd);
+        }
+        if (ks != keys)
+            BCMI_LAUNCH(q, "segsort_copyback",
+                        hipLaunchKernelGGL((k_seg_copyback<KB, VB>), dim3(pl->nlarge), dim3(256), 0, q->stream, keys,
+                                           ks, vals, vs, pl->large));
+    }
+    return 0;
+}
+
+template <typename KB>
+static int run_plan_vb(bcmi_queue q, bcmi_segplan pl, void *keys, void *vals)
+{
+    switch (vals ? pl->value_bytes : 0) {
+    case 0: return run_plan<KB, 0>(q, pl, keys, nullptr);
+    case 1: return run_plan<KB, 1>(q, pl, keys, vals);
+    case 2: return run_plan<KB, 2>(q, pl, keys, vals);
+    case 4: return run_plan<KB, 4>(q, pl, keys, vals);
+    case 8: return run_plan<KB, 8>(q, pl, keys, vals);
+    case 16: return run_plan<KB, 16>(q, pl, keys, vals);
+    default: set_error("segmented sort: value size %zu unsupported", pl->value_bytes); return BCMI_INVALID_VALUE;
+    }
+}
+
+}  // namespace bcmi
 
 using namespace bcmi;
 
-extern "C" int bcmi_segmented_sort(bcmi_queue q, int key_dtype, void *keys, void *values, size_t value_bytes,
-                                   const uint64_t *offsets, const uint64_t *host_offsets, size_t nseg, int descending)
+extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t value_bytes,
+                                          const uint64_t *host_offsets, size_t nseg, int descending,
+                                          bcmi_segplan *plan)
 {
     BCMI_TRY(use_queue(q));
-    set_error("segmented sort: not available yet");
-    return BCMI_INVALID_OPERATION;
+    BCMI_CHECK(plan && (host_offsets || nseg == 0), BCMI_INVALID_VALUE, "segmented plan: null argument");
+    BCMI_CHECK(dtype_valid(key_dtype), BCMI_INVALID_VALUE, "segmented plan: bad key dtype %d", key_dtype);
+    BCMI_CHECK(value_bytes == 0 || value_bytes == 1 || value_bytes == 2 || value_bytes == 4 || value_bytes == 8 ||
+                   value_bytes == 16,
+               BCMI_INVALID_VALUE, "segmented plan: value size %zu unsupported", value_bytes);
+    const int kb = dtype_size(key_dtype);
+    const bool flt = key_dtype == BCMI_F32 || key_dtype == BCMI_F64;
+    const size_t eb = (size_t)kb + value_bytes;
+    // the LDS-resident path holds (key + value) bytes x capacity in LDS
+    int cmax = 4;
+    while (cmax > 0 && eb * (size_t)kSegSmallThreads * ((size_t)1 << cmax) > (size_t)kSegSmallLds) --cmax;
+    const uint64_t small_max = (uint64_t)kSegSmallThreads << cmax;
+    const int ipt = large_ipt(kb, value_bytes);
+    const uint64_t tile = (uint64_t)kSortThreads * ipt;
+    std::vector<SegSpan> small[5], insert, large, chunks;
+    std::vector<uint32_t> chunk_begin(1, 0);
+    std::vector<SegTile> tiles;
+    uint64_t total = nseg ? host_offsets[nseg] : 0;
+    for (size_t s = 0; s < nseg; ++s) {
+        BCMI_CHECK(host_offsets[s + 1] >= host_offsets[s], BCMI_INVALID_VALUE,
+                   "segmented plan: offsets decrease at segment %zu", s);
+        const uint64_t base = host_offsets[s], len = host_offsets[s + 1] - host_offsets[s];
+        BCMI_CHECK(len <= (1ull << 32) - (1ull << 20), BCMI_INVALID_BUFFER_SIZE,
+                   "segmented plan: segment %zu has %llu elements", s, (unsigned long long)len);
+        if (len < 2) continue;
+        if (flt && len < 32) {
+            insert.push_back(SegSpan{base, (uint32_t)len, 0});
+        } else if (len <= small_max) {
+            int c = 0;
+            while (((uint64_t)kSegSmallThreads << c) < len) ++c;
+            small[c].push_back(SegSpan{base, (uint32_t)len, 0});
+        } else {
+            const uint32_t li = (uint32_t)large.size();
+            large.push_back(SegSpan{base, (uint32_t)len, li});
+            for (uint64_t t = 0; t * tile < len; ++t) tiles.push_back(SegTile{base, (uint32_t)len, (uint32_t)t, li, 0});
+            for (uint64_t c = 0; c < len; c += kSegChunk) {
+                uint64_t cl = len - c < kSegChunk ? len - c : kSegChunk;
+                chunks.push_back(SegSpan{base + c, (uint32_t)cl, li});
+            }
+            chunk_begin.push_back((uint32_t)chunks.size());
+        }
+    }
+    bcmi_segplan pl = new bcmi_segplan_s();
+    pl->device = q->device;
+    pl->key_dtype = key_dtype;
+    pl->value_bytes = value_bytes;
+    pl->descending = descending ? 1 : 0;
+    pl->total = total;
+    pl->nlarge = (uint32_t)large.size();
+    pl->ntiles = (uint32_t)tiles.size();
+    pl->nchunks = (uint32_t)chunks.size();
+    pl->ninsert = (uint32_t)insert.size();
+    // one allocation: tables, histogram partials, digit offsets, temporaries
+    size_t off = 0, o_small[5], o_insert, o_large, o_chunks, o_cb, o_tiles, o_partial, o_segoff, o_k, o_v;
+    for (int c = 0; c < 5; ++c) {
+        pl->nsmall[c] = (uint32_t)small[c].size();
+        o_small[c] = off;
+        off += al256(small[c].size() * sizeof(SegSpan));
+    }
+    o_insert = off; off += al256(insert.size() * sizeof(SegSpan));
+    o_large = off; off += al256(large.size() * sizeof(SegSpan));
+    o_chunks = off; off += al256(chunks.size() * sizeof(SegSpan));
+    o_cb = off; off += al256(chunk_begin.size() * sizeof(uint32_t));
+    o_tiles = off; off += al256(tiles.size() * sizeof(SegTile));
+    o_partial = off; off += al256(chunks.size() * (size_t)kb * 256 * 4);
+    o_segoff = off; off += al256(large.size() * (size_t)kb * 256 * 4);
+    // temporaries cover the elements of large segments at their own offsets
+    const uint64_t span = large.empty() ? 0 : (large.back().base + large.back().len);
+    o_k = off; off += al256(span * (size_t)kb);
+    o_v = off; off += al256(span * value_bytes);
+    hipError_t e = hipMalloc(&pl->mem, off + 256);
+    if (e != hipSuccess) {
+        delete pl;
+        return hip_fail(e, "segmented plan allocation", __FILE__, __LINE__);
+    }
+    char *m = (char *)pl->mem;
+    for (int c = 0; c < 5; ++c) pl->small[c] = (SegSpan *)(m + o_small[c]);
+    pl->insert = (SegSpan *)(m + o_insert);
+    pl->large = (SegSpan *)(m + o_large);
+    pl->chunks = (SegSpan *)(m + o_chunks);
+    pl->chunk_begin = (uint32_t *)(m + o_cb);
+    pl->tiles = (SegTile *)(m + o_tiles);
+    pl->partial = (uint32_t *)(m + o_partial);
+    pl->seg_off = (uint32_t *)(m + o_segoff);
+    pl->ktmp = m + o_k;
+    pl->vtmp = m + o_v;
+    auto up = [&](void *dst, const void *src, size_t bytes) -> int {
+        if (bytes) BCMI_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return 0;
+    };
+    int st = 0;
+    for (int c = 0; c < 5 && !st; ++c) st = up(pl->small[c], small[c].data(), small[c].size() * sizeof(SegSpan));
+    if (!st) st = up(pl->insert, insert.data(), insert.size() * sizeof(SegSpan));
+    if (!st) st = up(pl->large, large.data(), large.size() * sizeof(SegSpan));
+    if (!st) st = up(pl->chunks, chunks.data(), chunks.size() * sizeof(SegSpan));
+    if (!st) st = up(pl->chunk_begin, chunk_begin.data(), chunk_begin.size() * sizeof(uint32_t));
+    if (!st) st = up(pl->tiles, tiles.data(), tiles.size() * sizeof(SegTile));
+    if (st) {
+        (void)hipFree(pl->mem);
+        delete pl;
+        return st;
+    }
+    // look-back words for the large tiles, so that running the plan allocates nothing
+    uint64_t *status;
+    if (pl->ntiles) {
+        int s2 = status_array(q, (size_t)pl->ntiles * 256, &status);
+        if (s2) {
+            (void)hipFree(pl->mem);
+            delete pl;
+            return s2;
+        }
+    }
+    *plan = pl;
+    return 0;
+}
+
+extern "C" int bcmi_segmented_plan_destroy(bcmi_segplan plan)
+{
+    if (!plan) return 0;
+    (void)hipSetDevice(plan->device);
+    if (plan->mem) BCMI_HIP(hipFree(plan->mem));
+    delete plan;
+    return 0;
+}
+
+extern "C" int bcmi_segmented_sort_plan(bcmi_queue q, bcmi_segplan plan, void *keys, void *values)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(plan, BCMI_INVALID_VALUE, "segmented sort: null plan");
+    BCMI_CHECK(plan->device == q->device, BCMI_INVALID_COMMAND_QUEUE, "segmented sort: plan is for another device");
+    if (plan->total == 0) return 0;
+    BCMI_CHECK(keys, BCMI_INVALID_VALUE, "segmented sort: null keys");
+    if (!values) values = nullptr;
+    switch (dtype_size(plan->key_dtype)) {
+    case 1: return run_plan_vb<uint8_t>(q, plan, keys, values);
+    case 2: return run_plan_vb<uint16_t>(q, plan, keys, values);
+    case 4: return run_plan_vb<uint32_t>(q, plan, keys, values);
+    default: return run_plan_vb<uint64_t>(q, plan, keys, values);
+    }
+}
+
+extern "C" int bcmi_segmented_sort(bcmi_queue q, int key_dtype, void *keys, void *values, size_t value_bytes,
+                                   const uint64_t *host_offsets, size_t nseg, int descending)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(!q->capturing, BCMI_INVALID_OPERATION,
+               "segmented sort: build a plan (bcmi_segmented_plan_create) before capturing a graph");
+    bcmi_segplan plan = nullptr;
+    BCMI_TRY(bcmi_segmented_plan_create(q, key_dtype, values ? value_bytes : 0, host_offsets, nseg, descending, &plan));
+    int st = bcmi_segmented_sort_plan(q, plan, keys, values);
+    hipError_t e = hipStreamSynchronize(q->stream);
+    bcmi_segmented_plan_destroy(plan);
+    if (st) return st;
+    if (e != hipSuccess) return hip_fail(e, "segmented sort", __FILE__, __LINE__);
+    return 0;
 }

@@ -182,15 +182,25 @@ int bcmi_saxpy(bcmi_queue queue, float a, const float *x, const float *y,
                float *out, size_t n);
 
 /* ------------------------------------------------- batched / segmented */
-/* Stable sort of nseg independent segments in one batch: segment s is
- * [offsets[s], offsets[s+1]) of keys/values.  offsets: device pointer to
- * nseg+1 uint64; host_offsets: the same array in host memory (needed to
- * plan the launch).  No reference counterpart as one call: it replaces one
- * stable_sort_by_key call per segment. */
+/* Batched segmented sort.  Segment s is [host_offsets[s], host_offsets[s+1])
+ * of keys/values (value_bytes: 0, 1, 2, 4, 8 or 16); every segment ends up
+ * exactly as one stable_sort_by_key call on it would leave it
+ * (stable_sort_by_key.hpp:29-77: below 32 elements the insertion sort, else
+ * radix order).  No reference counterpart as one call: it replaces one
+ * stable_sort_by_key call per segment.  bcmi_segmented_sort plans, runs and
+ * waits (blocking).  For repeated or graph-captured use, build the plan once:
+ * running a plan does no host work and allocates nothing. */
+typedef struct bcmi_segplan_s *bcmi_segplan;
 int bcmi_segmented_sort(bcmi_queue queue, int key_dtype, void *keys,
                         void *values, size_t value_bytes,
-                        const uint64_t *offsets, const uint64_t *host_offsets,
-                        size_t nseg, int descending);
+                        const uint64_t *host_offsets, size_t nseg,
+                        int descending);
+int bcmi_segmented_plan_create(bcmi_queue queue, int key_dtype,
+                               size_t value_bytes, const uint64_t *host_offsets,
+                               size_t nseg, int descending, bcmi_segplan *plan);
+int bcmi_segmented_sort_plan(bcmi_queue queue, bcmi_segplan plan, void *keys,
+                             void *values);
+int bcmi_segmented_plan_destroy(bcmi_segplan plan);
 
 /* Segmented scan over runs of adjacent equal keys (the structure of
  * reduce_by_key_with_scan.hpp:104-293 exposed as a scan): exclusive runs

@@ -1,0 +1,76 @@
+//---------------------------------------------------------------------------//
+// copy / copy_n between host ranges and device ranges (reference:
+// algorithm/copy.hpp; host<->device copies are blocking, device->device
+// copies are enqueued).
+//---------------------------------------------------------------------------//
+#ifndef BOOST_COMPUTE_ALGORITHM_COPY_HPP
+#define BOOST_COMPUTE_ALGORITHM_COPY_HPP
+
+#include <algorithm>
+#include <iterator>
+#include <vector>
+
+#include <boost/compute/system.hpp>
+#include <boost/compute/type_traits/is_device_iterator.hpp>
+
+namespace boost {
+namespace compute {
+namespace detail {
+
+// host -> device
+template <class InputIterator, class T>
+inline buffer_iterator<T> copy_dispatch(InputIterator first, InputIterator last, buffer_iterator<T> result,
+                                        command_queue &queue, std::false_type, std::true_type)
+{
+    std::vector<T> tmp(first, last);
+    if (!tmp.empty())
+        bcmi_check(bcmi_memcpy(queue.get(), result.device_ptr(), tmp.data(), tmp.size() * sizeof(T), BCMI_H2D, 1));
+    return result + (std::ptrdiff_t)tmp.size();
+}
+
+// device -> host
+template <class T, class OutputIterator>
+inline OutputIterator copy_dispatch(buffer_iterator<T> first, buffer_iterator<T> last, OutputIterator result,
+                                    command_queue &queue, std::true_type, std::false_type)
+{
+    const size_t n = (size_t)(last - first);
+    std::vector<T> tmp(n);
+    if (n) bcmi_check(bcmi_memcpy(queue.get(), tmp.data(), first.device_ptr(), n * sizeof(T), BCMI_D2H, 1));
+    return std::copy(tmp.begin(), tmp.end(), result);
+}
+
+// device -> device
+template <class T>
+inline buffer_iterator<T> copy_dispatch(buffer_iterator<T> first, buffer_iterator<T> last, buffer_iterator<T> result,
+                                        command_queue &queue, std::true_type, std::true_type)
+{
+    const size_t n = (size_t)(last - first);
+    if (n)
+        bcmi_check(bcmi_memcpy(queue.get(), result.device_ptr(), first.device_ptr(), n * sizeof(T), BCMI_D2D, 0));
+    return result + (std::ptrdiff_t)n;
+}
+
+}  // namespace detail
+
+template <class InputIterator, class OutputIterator>
+inline OutputIterator copy(InputIterator first, InputIterator last, OutputIterator result,
+                           command_queue &queue = system::default_queue())
+{
+    return detail::copy_dispatch(first, last, result, queue,
+                                 std::integral_constant<bool, is_device_iterator<InputIterator>::value>(),
+                                 std::integral_constant<bool, is_device_iterator<OutputIterator>::value>());
+}
+
+template <class InputIterator, class Size, class OutputIterator>
+inline OutputIterator copy_n(InputIterator first, Size count, OutputIterator result,
+                             command_queue &queue = system::default_queue())
+{
+    InputIterator last = first;
+    std::advance(last, count);
+    return ::boost::compute::copy(first, last, result, queue);
+}
+
+}  // namespace compute
+}  // namespace boost
+
+#endif

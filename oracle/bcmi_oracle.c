@@ -437,17 +437,17 @@ void orc_compare_stable_sort(void *keys, void *vals, size_t n, int kbytes,
     free(kb);
 }
 
-/* Segmented sort: one stable radix-order sort per segment (the per-call
- * baseline: one sort_by_key call per segment). */
+/* Segmented sort: each segment as one stable_sort_by_key call would leave
+ * it (the per-call baseline: stable_sort_by_key.hpp:29-77 per segment). */
 void orc_segmented_sort(void *keys, void *vals, const uint64_t *offsets,
                         size_t nseg, int kbytes, int kind, int descending,
                         int vbytes)
 {
     for (size_t s = 0; s < nseg; ++s) {
         size_t b = (size_t)offsets[s], e = (size_t)offsets[s + 1];
-        orc_radix_order_sort((char *)keys + b * kbytes,
-                             vals ? (char *)vals + b * vbytes : NULL, e - b,
-                             kbytes, kind, descending, vbytes);
+        orc_sort_dispatch(3, (char *)keys + b * kbytes,
+                          vals ? (char *)vals + b * vbytes : NULL, e - b,
+                          kbytes, kind, descending, vbytes);
     }
 }
 

@@ -68,3 +68,37 @@ def test_version():
     a, b = C.c_int(), C.c_int()
     ca.check(ca.lib().bcmi_get_version(C.byref(a), C.byref(b)))
     assert (a.value, b.value) == (0, 5)
+
+
+CPP_BUILD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "build")
+
+
+def test_cpp_headers_link_in_two_translation_units():
+    """The header-only boost::compute templates, included by two objects of
+    one program, link without duplicate symbols (the reference's
+    test/extra/test_multiple_objects1.cpp / test_multiple_objects2.cpp)."""
+    exe = os.path.join(CPP_BUILD, "odr")
+    if not os.path.exists(exe):
+        pytest.skip("tests/cpp not built (run __graft_entry__.build())")
+    assert subprocess.run([exe], timeout=60).returncode == 0
+
+
+def test_cpp_headers_compile_standalone():
+    """Every public header compiles on its own (include-what-you-use)."""
+    inc = os.path.join(os.path.dirname(CPP_BUILD), "..", "..", "include")
+    inc = os.path.normpath(inc)
+    headers = []
+    for d, _, files in os.walk(os.path.join(inc, "boost")):
+        for f in files:
+            if f.endswith(".hpp"):
+                headers.append(os.path.relpath(os.path.join(d, f), inc))
+    assert len(headers) > 20
+    src = "".join('#include <%s>\n' % h for h in sorted(headers))
+    for h in sorted(headers):
+        r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I", inc, "-x", "c++", "-"],
+                           input="#include <%s>\nint main(){return 0;}\n" % h, text=True,
+                           capture_output=True, timeout=120)
+        assert r.returncode == 0, (h, r.stderr[:2000])
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I", inc, "-x", "c++", "-"],
+                       input=src + "int main(){return 0;}\n", text=True, capture_output=True, timeout=120)
+    assert r.returncode == 0, r.stderr[:2000]

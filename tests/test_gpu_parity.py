@@ -506,3 +506,85 @@ def test_graph_replay_matches_eager(queue):
         # eager work between replays
         ca.scan(queue, dx, out, False)
         queue.finish()
+
+
+# ========================================================= segmented sort
+def _segments(lens):
+    return np.r_[0, np.cumsum(lens)].astype(np.uint64)
+
+
+@pytest.mark.parametrize("dt", ["uint64", "uint32", "int32", "float32", "float64", "uint8", "int16"])
+@pytest.mark.parametrize("vdt", [None, "uint32", "uint64"])
+def test_segmented_sort_matches_per_segment_calls(queue, dt, vdt):
+    # every size class: empty, singletons, < 32, LDS classes up to 4096, and
+    # multi-tile segments for the batched onesweep path
+    lens = [0, 1, 2, 5, 31, 32, 33, 200, 256, 257, 700, 1500, 2049, 4096, 4097, 9000, 30000, 0, 3]
+    lens += list(RNG.integers(1, 3000, 60))
+    off = _segments(lens)
+    n = int(off[-1])
+    k = random_keys(dt, n)
+    v = np.arange(n).astype(vdt) if vdt else None
+    for desc in (False, True):
+        dk = dev(queue, k)
+        dv = dev(queue, v) if vdt else None
+        ca.segmented_sort(queue, dk, dv, off, descending=desc)
+        exp = O.segmented_sort(k, v, off, descending=desc)
+        if vdt:
+            ek, ev = exp
+            assert np.array_equal(dv.to_host(queue), ev), desc
+        else:
+            ek = exp
+        assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), desc
+
+
+def test_segmented_sort_float_small_segments_keep_signed_zero_order(queue):
+    # below 32 elements stable_sort_by_key is the insertion sort: +0.0 and
+    # -0.0 compare equal and keep input order; from 32 on, radix order puts
+    # -0.0 first
+    k = np.array([0.0, -0.0, 1.0] * 11, np.float32)  # 33 keys
+    off = np.array([0, 3, 33], np.uint64)
+    v = np.arange(33, dtype=np.uint32)
+    dk, dv = dev(queue, k), dev(queue, v)
+    ca.segmented_sort(queue, dk, dv, off)
+    ek, ev = O.segmented_sort(k, v, off)
+    assert np.array_equal(dv.to_host(queue), ev)
+    assert list(ev[:3]) == [0, 1, 2]
+
+
+def test_segmented_sort_workload_sample(queue):
+    """The segmented workload's generator at 2048 segments (lognormal lengths,
+    median 256, one 2^20 segment): bit-identical to per-segment sorts."""
+    off = O.gen_segments(2048)
+    n = int(off[-1])
+    k, v = O.gen_segment_data(n)
+    dk, dv = dev(queue, k), dev(queue, v)
+    ca.segmented_sort(queue, dk, dv, off)
+    ek, ev = O.segmented_sort(k, v, off)
+    assert np.array_equal(dk.to_host(queue), ek)
+    assert np.array_equal(dv.to_host(queue), ev)
+
+
+def test_segmented_plan_graph_replay(queue):
+    lens = list(RNG.integers(1, 6000, 300)) + [50000]
+    off = _segments(lens)
+    n = int(off[-1])
+    k = random_keys("uint64", n)
+    v = np.arange(n, dtype=np.uint32)
+    dk, dv = dev(queue, k), dev(queue, v)
+    plan = ca.SegmentedPlan(queue, np.uint64, np.uint32, off)
+    plan.run(queue, dk, dv)  # eager
+    ek, ev = O.segmented_sort(k, v, off)
+    assert np.array_equal(dk.to_host(queue), ek) and np.array_equal(dv.to_host(queue), ev)
+    g = ca.Graph(queue)
+    with g:
+        plan.run(queue, dk, dv)
+    for rep in range(3):
+        ca.check(ca.lib().bcmi_memcpy(queue.h, dk.ptr, k.ctypes.data_as(ca.C.c_void_p), k.nbytes, ca.H2D, 1))
+        ca.check(ca.lib().bcmi_memcpy(queue.h, dv.ptr, v.ctypes.data_as(ca.C.c_void_p), v.nbytes, ca.H2D, 1))
+        g.launch()
+        queue.finish()
+        assert np.array_equal(dk.to_host(queue), ek), rep
+        assert np.array_equal(dv.to_host(queue), ev), rep
+        x = dev(queue, np.arange(1000, dtype=np.int32))  # eager work between replays
+        ca.scan(queue, x, x, False)
+        queue.finish()
