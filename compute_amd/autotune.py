@@ -1,0 +1,187 @@
+"""Autotuner for the launch parameters the kernels read from the tuning cache.
+
+The reference keeps tunables per (device, algorithm key, parameter) in a
+JSON file per device that its perf programs fill in
+(detail/parameter_cache.hpp:38-217, e.g. the "__boost_reduce_on_gpu"
+"vpt"/"tpb" entries read at reduce_on_gpu.hpp:194-197).  Here the same
+role: each tunable below is swept on its workload, every candidate's output
+is checked, the fastest correct candidate is kept, and the result is written
+with bcmi_tuning_cache_save.  libbcmi.so loads the file at its first queue:
+the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
+
+Tunables (the digit width is fixed at 8 bits and the block size at 256
+threads by the kernels' LDS layout; what varies is the work per thread):
+  __bcmi_radix_sort|ipt, |rank         keys-only 4-byte radix sort
+  __bcmi_radix_sort_by_key|ipt, |rank  4-byte keys with 4-byte values
+  __bcmi_scan|q                        4-byte scans, items per lane
+  __bcmi_segscan|q                     reduce_by_key / segmented scan
+
+    python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
+"""
+import argparse
+import json
+import os
+import sys
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
+
+SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 16), (1, 24)]  # (rank, ipt)
+SCAN_Q = (4, 8, 16)
+SEGSCAN_Q = (2, 4, 8)
+
+
+def _median_ms(ca, q, fn, restore=None, reps=7):
+    ts = []
+    for _ in range(reps):
+        if restore:
+            restore()
+        e0, e1 = ca.Event(), ca.Event()
+        e0.record(q)
+        fn()
+        e1.record(q)
+        q.finish()
+        ts.append(e0.elapsed_ms(e1))
+    return float(np.median(ts[1:]))
+
+
+def _set(ca, key, param, value):
+    ca.check(ca.lib().bcmi_tune_set(key.encode(), param.encode(), int(value)))
+
+
+def tune_sort(ca, W, q, log2n, log):
+    n = 1 << log2n
+    L = ca.lib()
+    keys = W.sort_u32(n)
+    expect = np.sort(keys)
+    src = ca.DeviceArray.from_host(keys, q)
+    work = ca.DeviceArray(n, np.uint32, q)
+    restore = lambda: ca.check(L.bcmi_memcpy(q.h, work.ptr, src.ptr, src.nbytes, ca.D2D, 0))
+    best = None
+    for rank, ipt in SORT_CANDIDATES:
+        _set(ca, "__bcmi_radix_sort", "rank", rank)
+        _set(ca, "__bcmi_radix_sort", "ipt", ipt)
+        ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
+        ok = bool(np.array_equal(work.to_host(q), expect))
+        log({"tunable": "__bcmi_radix_sort", "rank": rank, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, rank, ipt)
+    if best is None:
+        raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
+    _set(ca, "__bcmi_radix_sort", "rank", best[1])
+    _set(ca, "__bcmi_radix_sort", "ipt", best[2])
+
+    k, v = W.sort_by_key_f32(n)
+    sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+    wk, wv = ca.DeviceArray(n, np.float32, q), ca.DeviceArray(n, np.uint32, q)
+
+    def restore2():
+        ca.check(L.bcmi_memcpy(q.h, wk.ptr, sk.ptr, sk.nbytes, ca.D2D, 0))
+        ca.check(L.bcmi_memcpy(q.h, wv.ptr, sv.ptr, sv.nbytes, ca.D2D, 0))
+
+    first = None
+    best = None
+    for rank, ipt in SORT_CANDIDATES:
+        _set(ca, "__bcmi_radix_sort_by_key", "rank", rank)
+        _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
+        ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)
+        got = (wk.to_host(q).view(np.uint32), wv.to_host(q))
+        if first is None:
+            # stable order by the radix key: the payload is the input index,
+            # so ties must keep increasing indices
+            rk = got[0] ^ np.where(got[0] >> 31, np.uint32(0xFFFFFFFF), np.uint32(0x80000000))
+            ok = bool(np.all(rk[1:] >= rk[:-1]) and np.all((rk[1:] > rk[:-1]) | (got[1][1:] > got[1][:-1])))
+            first = got
+        else:
+            ok = bool(np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]))
+        log({"tunable": "__bcmi_radix_sort_by_key", "rank": rank, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, rank, ipt)
+    if best is None:
+        raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
+    _set(ca, "__bcmi_radix_sort_by_key", "rank", best[1])
+    _set(ca, "__bcmi_radix_sort_by_key", "ipt", best[2])
+
+
+def tune_scan(ca, W, q, log2n, log):
+    n = 1 << log2n
+    x = W.scan_i32(n)
+    expect = np.empty(n, np.int32)
+    expect[0] = 0
+    np.cumsum(x[:-1], dtype=np.int32, out=expect[1:])
+    dx = ca.DeviceArray.from_host(x, q)
+    do = ca.DeviceArray(n, np.int32, q)
+    best = None
+    for qq in SCAN_Q:
+        _set(ca, "__bcmi_scan", "q", qq)
+        ms = _median_ms(ca, q, lambda: ca.scan(q, dx, do, True, init=0))
+        ok = bool(np.array_equal(do.to_host(q), expect))
+        log({"tunable": "__bcmi_scan", "q": qq, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, qq)
+    if best is None:
+        raise RuntimeError("autotune: no scan candidate matched")
+    _set(ca, "__bcmi_scan", "q", best[1])
+
+
+def tune_segscan(ca, W, q, log2n, log):
+    n = 1 << log2n
+    L = ca.lib()
+    k, v = W.runs_i32_f32(n)
+    heads = np.flatnonzero(np.r_[True, k[1:] != k[:-1]])
+    runs = len(heads)
+    last_keys = k[np.r_[heads[1:] - 1, n - 1]]
+    v64 = v.astype(np.float64)
+    sums = np.add.reduceat(v64, heads)
+    mags = np.add.reduceat(np.abs(v64), heads)
+    dk, dv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+    ko, vo = ca.DeviceArray(n, np.int32, q), ca.DeviceArray(n, np.float32, q)
+    cnt = ca.DeviceArray(1, np.uint64, q)
+    best = None
+    for qq in SEGSCAN_Q:
+        _set(ca, "__bcmi_segscan", "q", qq)
+        ms = _median_ms(ca, q, lambda: ca.check(L.bcmi_reduce_by_key(q.h, ca.I32, ca.F32, ca.PLUS, dk.ptr, dv.ptr, n,
+                                                                        ko.ptr, vo.ptr, cnt.ptr, 0)))
+        ok = int(cnt.to_host(q)[0]) == runs and bool(np.array_equal(ko.to_host(q, runs), last_keys))
+        got = vo.to_host(q, runs).astype(np.float64)
+        ok = ok and bool(np.all(np.abs(got - sums) <= 2.0 ** -23 * np.abs(sums) + 2.0 ** -40 * mags))
+        ms2 = _median_ms(ca, q, lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0))
+        log({"tunable": "__bcmi_segscan", "q": qq, "rbk_ms": round(ms, 4), "segscan_ms": round(ms2, 4), "ok": ok})
+        if ok and (best is None or ms + ms2 < best[0]):
+            best = (ms + ms2, qq)
+    if best is None:
+        raise RuntimeError("autotune: no segmented scan candidate matched")
+    _set(ca, "__bcmi_segscan", "q", best[1])
+
+
+def main(argv=None):
+    ap = argparse.ArgumentParser(description=__doc__.split("\n")[0])
+    ap.add_argument("--out", default=DEFAULT_OUT)
+    ap.add_argument("--log2n-sort", type=int, default=27)
+    ap.add_argument("--log2n-scan", type=int, default=28)
+    ap.add_argument("--log2n-segscan", type=int, default=27)
+    ap.add_argument("--only", default="sort,scan,segscan")
+    args = ap.parse_args(argv)
+    sys.path.insert(0, os.path.dirname(HERE))
+    import compute_amd as ca
+    import compute_amd.workloads as W
+
+    q = ca.Queue(0)
+    log = lambda rec: print(json.dumps(rec), flush=True)
+    only = args.only.split(",")
+    if "sort" in only:
+        tune_sort(ca, W, q, args.log2n_sort, log)
+    if "scan" in only:
+        tune_scan(ca, W, q, args.log2n_scan, log)
+    if "segscan" in only:
+        tune_segscan(ca, W, q, args.log2n_segscan, log)
+    q.finish()
+    os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
+    ca.check(ca.lib().bcmi_tuning_cache_save(args.out.encode()))
+    print(json.dumps({"saved": args.out}), flush=True)
+
+
+if __name__ == "__main__":
+    main()

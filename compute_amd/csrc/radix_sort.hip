@@ -208,7 +208,8 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
 }
 
 // Tile size (items per thread) and ranking method come from the tuning cache
-// (key "__bcmi_radix_sort", params "ipt" and "rank"); the variants are
+// (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
+// params "ipt" and "rank"); the variants are
 // compiled for the benchmarked 4-byte keys with 0- or 4-byte values.  The
 // default of 32 items per thread with LDS peer-mask ranking was the fastest
 // measured on 2^27 uint32 keys (16: 3.00 ms, 24: 2.39 ms, 32: 2.25 ms).
@@ -218,8 +219,9 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
     constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
     constexpr int IPT = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
-        const unsigned ipt = tune_get("__bcmi_radix_sort", "ipt", 32);
-        const unsigned rank = tune_get("__bcmi_radix_sort", "rank", kRankLdsMask);
+        const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
+        const unsigned ipt = tune_get(key, "ipt", 32);
+        const unsigned rank = tune_get(key, "rank", kRankLdsMask);
         if (rank == kRankBallot) {
             if (ipt == 24) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
             return radix_sort<KB, VB, 16, kRankBallot>(q, keys, vals, n, kind, desc);

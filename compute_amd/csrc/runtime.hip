@@ -6,6 +6,7 @@
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
+#include <dlfcn.h>
 #include <fstream>
 #include <mutex>
 #include <sstream>
@@ -181,6 +182,8 @@ unsigned tune_get(const std::string &key, const char *param, unsigned def)
     return it == g_tune.end() ? def : it->second;
 }
 
+static int tune_autoload();
+
 template <typename T>
 __global__ void k_fill(T *dst, T v, size_t n)
 {
@@ -245,6 +248,7 @@ int bcmi_queue_create(int device, bcmi_queue *queue)
     BCMI_HIP(hipGetDeviceProperties(&p, device));
     BCMI_CHECK(std::strncmp(p.gcnArchName, "gfx950", 6) == 0, BCMI_DEVICE_NOT_FOUND,
                "device %d is %s; this library is built for gfx950 only", device, p.gcnArchName);
+    BCMI_TRY(tune_autoload());
     bcmi_queue q = new bcmi_queue_s();
     q->device = device;
     q->cu_count = p.multiProcessorCount;
@@ -473,6 +477,17 @@ int bcmi_tune_set(const char *key, const char *param, unsigned value)
 // (detail/parameter_cache.hpp:106-210).
 int bcmi_tuning_cache_load(const char *path)
 {
+    return tune_load(path, true);
+}
+
+}  // extern "C"
+
+namespace bcmi {
+
+// Parses the cache file; `override` false keeps entries already set (the
+// automatic load at the first queue must not undo an explicit bcmi_tune_set).
+int tune_load(const char *path, bool override)
+{
     BCMI_CHECK(path, BCMI_INVALID_VALUE, "null path");
     std::ifstream f(path);
     BCMI_CHECK(f.good(), BCMI_INVALID_VALUE, "cannot open tuning cache %s", path);
@@ -495,9 +510,40 @@ int bcmi_tuning_cache_load(const char *path)
         i = (size_t)(end - s.c_str());
     }
     std::lock_guard<std::mutex> lk(g_tune_mu);
-    for (auto &kv : parsed) g_tune[kv.first] = kv.second;
+    for (auto &kv : parsed)
+        if (override || !g_tune.count(kv.first)) g_tune[kv.first] = kv.second;
     return 0;
 }
+
+// Once per process, at the first queue: the file BCMI_TUNING_CACHE names
+// (an error if it cannot be read), else tuning/gfx950.json beside this
+// library when present -- the autotuner's output (compute_amd/autotune.py),
+// in the role of the reference's per-device cache file loaded by
+// parameter_cache (detail/parameter_cache.hpp:106-138).
+static int tune_autoload()
+{
+    static std::once_flag once;
+    static int status = 0;
+    std::call_once(once, [] {
+        const char *env = std::getenv("BCMI_TUNING_CACHE");
+        if (env && *env) {
+            status = tune_load(env, false);
+            return;
+        }
+        Dl_info info;
+        if (!dladdr((void *)&tune_autoload, &info) || !info.dli_fname) return;
+        std::string dir(info.dli_fname);
+        const size_t slash = dir.rfind('/');
+        dir = slash == std::string::npos ? std::string(".") : dir.substr(0, slash);
+        const std::string path = dir + "/tuning/gfx950.json";
+        if (std::ifstream(path).good()) status = tune_load(path.c_str(), false);
+    });
+    return status;
+}
+
+}  // namespace bcmi
+
+extern "C" {
 
 int bcmi_tuning_cache_save(const char *path)
 {

@@ -121,6 +121,7 @@ inline int cu_blocks(bcmi_queue q, int per_cu) { return q->cu_count * per_cu; }
 
 // Tuning cache (parameter_cache analogue).
 unsigned tune_get(const std::string &key, const char *param, unsigned def);
+int tune_load(const char *path, bool override);
 
 }  // namespace bcmi
 
