@@ -21,8 +21,8 @@
  *     comment says "blocking", mirroring which reference calls block
  *     (reduce.hpp:225, accumulate.hpp:53-55,
  *     reduce_by_key_with_scan.hpp:485-487).
- *   - Sizes are size_t; sorts accept n < 2^32 (radix_sort.hpp:131-137 uses
- *     uint counts).
+ *   - Sizes are size_t; sorts accept n <= 2^32 - 2^20 (32-bit tile and
+ *     digit counts, as radix_sort.hpp:131-137 uses uint counts).
  */
 #ifndef BCMI_H
 #define BCMI_H
@@ -221,7 +221,10 @@ int bcmi_graph_destroy(bcmi_graph graph);
 
 /* ------------------------------------------------------------ tuning */
 /* parameter_cache (detail/parameter_cache.hpp:63-82): per-device
- * (key, param) -> value, persisted as JSON. */
+ * (key, param) -> value, persisted as JSON.  The first bcmi_queue_create
+ * loads the file named by the environment variable BCMI_TUNING_CACHE, else
+ * tuning/gfx950.json beside libbcmi.so when present, without overriding
+ * values already set with bcmi_tune_set. */
 int bcmi_tune_get(const char *key, const char *param, unsigned default_value,
                   unsigned *value);
 int bcmi_tune_set(const char *key, const char *param, unsigned value);
