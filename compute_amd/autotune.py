@@ -28,7 +28,7 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
-SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 16), (1, 24)]  # (rank, ipt)
+SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 24)]  # (rank, ipt)
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 

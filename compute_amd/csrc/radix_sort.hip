@@ -36,9 +36,9 @@ struct alignas(16) kchunk {
 
 // Each wave counts into its own LDS sub-histogram (fewer same-address LDS
 // atomics); keys are read as 16-byte chunks, two chunks in flight.
-template <typename KB>
+template <typename KB, typename XF>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
-                                                            KeyXf<KB> xf, uint32_t *__restrict__ partial)
+                                                            XF xf, uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
     constexpr int NW = kHistThreads / 64;
@@ -47,7 +47,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
     __syncthreads();
     uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
     auto count = [&](KB x) {
-        KB k = radix_key(x, xf);
+        KB k = xf.key(x);
 #pragma unroll
         for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
     };
@@ -133,36 +133,16 @@ __global__ void k_insertion(T *keys, typename ValT<VB>::type *vals, uint32_t n, 
     }
 }
 
-template <typename KB>
-static KeyXf<KB> make_xf(int kind, int descending)
-{
-    KeyXf<KB> f;
-    f.neg = 0;
-    f.flt = 0;
-    f.cx = 0;
-    const KB sgn = (KB)((KB)1 << (sizeof(KB) * 8 - 1));
-    if (!descending) {
-        if (kind == 1) f.cx = sgn;
-        if (kind == 2) f.flt = 1;
-    } else {
-        if (kind == 0) f.cx = (KB)~(KB)0;
-        if (kind == 1) { f.neg = 1; f.cx = sgn; }
-        if (kind == 2) { f.neg = 1; f.flt = 1; }
-    }
-    return f;
-}
-
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <typename KB, int VB, int IPT, int RANK>
-static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int kind, int descending)
+template <typename KB, int VB, int IPT, int RANK, typename XF>
+static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
     constexpr uint32_t TILE = kSortThreads * IPT;
     KB *keys = (KB *)keys_;
     V *vals = (V *)vals_;
-    const KeyXf<KB> xf = make_xf<KB>(kind, descending);
     const uint32_t tiles = (n + TILE - 1) / TILE;
     uint32_t hblocks = (n + kHistThreads * 32 - 1) / (kHistThreads * 32);
     uint32_t hmax = (uint32_t)q->cu_count * 4;
@@ -183,7 +163,7 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
     BCMI_TRY(status_array(q, (size_t)tiles * 256, &status));
 
     BCMI_LAUNCH(q, "radix_histogram",
-                hipLaunchKernelGGL((k_histogram<KB>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
+                hipLaunchKernelGGL((k_histogram<KB, XF>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
                                    xf, partial));
     BCMI_LAUNCH(q, "radix_digit_offsets",
                 hipLaunchKernelGGL(k_digit_offsets, dim3(P), dim3(1024), 0, q->stream, partial, hblocks,
@@ -193,7 +173,7 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK>), dim3(tiles), dim3(kSortThreads), 0, q->stream, ks,
+                    hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK, XF>), dim3(tiles), dim3(kSortThreads), 0, q->stream, ks,
                                        kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
                                        (uint32_t)(P - p)));
         std::swap(ks, kd);
@@ -205,6 +185,13 @@ static int radix_sort(bcmi_queue q, void *keys_, void *vals_, uint32_t n, int ki
         if (VB) BCMI_HIP(hipMemcpyAsync(vals, vs, (size_t)n * VB, hipMemcpyDeviceToDevice, q->stream));
     }
     return 0;
+}
+
+template <typename KB, int VB, int IPT, int RANK>
+static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending)
+{
+    return with_key_xf<KB>(kind, descending,
+                           [&](auto xf) { return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf); });
 }
 
 // Tile size (items per thread) and ranking method come from the tuning cache
@@ -222,10 +209,7 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
         const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
         const unsigned ipt = tune_get(key, "ipt", 32);
         const unsigned rank = tune_get(key, "rank", kRankLdsMask);
-        if (rank == kRankBallot) {
-            if (ipt == 24) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
-            return radix_sort<KB, VB, 16, kRankBallot>(q, keys, vals, n, kind, desc);
-        }
+        if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
         if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc);
         // any other value falls through to the generic tile below

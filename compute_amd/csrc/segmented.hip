@@ -47,10 +47,10 @@ struct SegTile {
 };
 
 // ------------------------------------------------------------ small path
-template <typename KB, int VB, int IPT>
+template <typename KB, int VB, int IPT, typename XF>
 __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__ keys,
                                                                 typename ValT<VB>::type *__restrict__ vals,
-                                                                const SegSpan *__restrict__ segs, KeyXf<KB> xf)
+                                                                const SegSpan *__restrict__ segs, XF xf)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
@@ -69,6 +69,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__
     const uint32_t len = sg.len;
     auto pos = [&](int j) { return w * 64 * IPT + j * 64 + lane; };
     auto valid = [&](int j) { return pos(j) < len; };
+    const int32_t lim = (int32_t)len - (int32_t)(w * 64 * IPT);
 
     KB k[IPT];
     V v[IPT];
@@ -82,14 +83,12 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__
     for (int i = tid; i < 4 * 256; i += kSegSmallThreads) (&s_peer[0][0])[i] = 0;
 
     for (int p = 0; p < P; ++p) {
-        uint32_t d[IPT], r[IPT];
-#pragma unroll
-        for (int j = 0; j < IPT; ++j) d[j] = digit_of(k[j], xf, 8 * p);
+        Ranks<IPT> r;
         for (int i = lane; i < 256; i += 64) s_cnt[w][i] = 0;
         __syncthreads();
         // every thread read last pass's flag before the barrier above
         if (tid == 0) s_uniform = 0;
-        rank_in_wave<IPT>(d, r, s_cnt[w], s_peer[w], valid);
+        rank_in_wave<IPT, false, kRankLdsMask>(k, r, xf, 8 * p, s_cnt[w], s_peer[w], lim);
         __syncthreads();
         const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
         const uint32_t total = c0 + c1 + c2 + c3;
@@ -104,7 +103,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             if (valid(j)) {
-                const uint32_t dst = s_cnt[w][d[j]] + r[j];
+                const uint32_t dst = s_cnt[w][xf.digit(k[j], 8 * p)] + r[j];
                 s_k[dst] = k[j];
                 if constexpr (VB > 0) s_v[dst] = v[j];
             }
@@ -157,9 +156,9 @@ __global__ void k_seg_insertion(T *keys, typename ValT<VB>::type *vals, const Se
 
 // ------------------------------------------------------------ large path
 // One block per histogram chunk (<= kSegChunk keys of one segment).
-template <typename KB>
+template <typename KB, typename XF>
 __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__restrict__ keys,
-                                                                const SegSpan *__restrict__ chunks, KeyXf<KB> xf,
+                                                                const SegSpan *__restrict__ chunks, XF xf,
                                                                 uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
@@ -170,7 +169,7 @@ __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__rest
     const SegSpan c = chunks[blockIdx.x];
     uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
     for (uint32_t i = threadIdx.x; i < c.len; i += kHistThreads) {
-        KB k = radix_key(keys[c.base + i], xf);
+        KB k = xf.key(keys[c.base + i]);
 #pragma unroll
         for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
     }
@@ -197,19 +196,20 @@ __global__ __launch_bounds__(256) void k_seg_digit_offsets(const uint32_t *__res
     seg_off[((size_t)s * passes + p) * 256 + d] = block_exclusive_sum256(c, s_wsum);
 }
 
-template <typename KB, int VB, int IPT>
-__global__ __launch_bounds__(kSortThreads) void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
+template <typename KB, int VB, int IPT, typename XF>
+__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(onesweep_waves_per_eu<VB, IPT>())))
+void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                                const typename ValT<VB>::type *__restrict__ vin,
                                                                typename ValT<VB>::type *__restrict__ vout,
                                                                const SegTile *__restrict__ tiles, uint32_t shift,
                                                                uint32_t pass, uint32_t passes,
-                                                               const uint32_t *__restrict__ seg_off, KeyXf<KB> xf,
+                                                               const uint32_t *__restrict__ seg_off, XF xf,
                                                                Ctrl *ctrl, uint64_t *__restrict__ status,
                                                                uint32_t back)
 {
     const SegTile t = tiles[blockIdx.x];
     const size_t b = t.base;
-    onesweep_tile<KB, VB, IPT, kRankLdsMask>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
+    onesweep_tile<KB, VB, IPT, kRankLdsMask, XF>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
                                              t.tile, shift, seg_off + ((size_t)t.seg * passes + pass) * 256, xf,
                                              ctrl, status + (size_t)(blockIdx.x - t.tile) * 256,
                                              launch_epoch(ctrl, back));
@@ -264,8 +264,8 @@ static int large_ipt(int kb, size_t vb)
     return sb >= 16 ? 8 : (sb >= 8 ? 12 : 16);
 }
 
-template <typename KB, int VB>
-static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
+template <typename KB, int VB, typename XF>
+static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, XF xf)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
@@ -273,30 +273,12 @@ static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
     constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     KB *keys = (KB *)keys_;
     V *vals = (V *)vals_;
-    const int kind = pl->key_dtype == BCMI_F32 || pl->key_dtype == BCMI_F64 ? 2
-                   : (pl->key_dtype == BCMI_I8 || pl->key_dtype == BCMI_I16 || pl->key_dtype == BCMI_I32 ||
-                      pl->key_dtype == BCMI_I64) ? 1 : 0;
-    KeyXf<KB> xf;
-    {
-        const KB sgn = (KB)((KB)1 << (sizeof(KB) * 8 - 1));
-        xf.neg = 0;
-        xf.flt = 0;
-        xf.cx = 0;
-        if (!pl->descending) {
-            if (kind == 1) xf.cx = sgn;
-            if (kind == 2) xf.flt = 1;
-        } else {
-            if (kind == 0) xf.cx = (KB)~(KB)0;
-            if (kind == 1) { xf.neg = 1; xf.cx = sgn; }
-            if (kind == 2) { xf.neg = 1; xf.flt = 1; }
-        }
-    }
     // small classes
 #define BCMI_SMALL(CLS, IPTV)                                                                                   \
     if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
         if (pl->nsmall[CLS])                                                                                    \
             BCMI_LAUNCH(q, "segsort_small",                                                                     \
-                        hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV>), dim3(pl->nsmall[CLS]),                  \
+                        hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV, XF>), dim3(pl->nsmall[CLS]),                  \
                                            dim3(kSegSmallThreads), 0, q->stream, keys, vals, pl->small[CLS], xf)); \
     }
     BCMI_SMALL(0, 1)
@@ -321,7 +303,7 @@ static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
     }
     if (pl->nlarge) {
         BCMI_LAUNCH(q, "segsort_histogram",
-                    hipLaunchKernelGGL((k_seg_histogram<KB>), dim3(pl->nchunks), dim3(kHistThreads), 0, q->stream,
+                    hipLaunchKernelGGL((k_seg_histogram<KB, XF>), dim3(pl->nchunks), dim3(kHistThreads), 0, q->stream,
                                        keys, pl->chunks, xf, pl->partial));
         BCMI_LAUNCH(q, "segsort_digit_offsets",
                     hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, P), dim3(256), 0, q->stream, pl->partial,
@@ -333,7 +315,7 @@ static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
         V *vs = vals, *vd = (V *)pl->vtmp;
         for (int p = 0; p < P; ++p) {
             BCMI_LAUNCH(q, "segsort_onesweep",
-                        hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL>), dim3(pl->ntiles), dim3(kSortThreads), 0,
+                        hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
                                            q->stream, ks, kd, vs, vd, pl->tiles, (uint32_t)(8 * p), (uint32_t)p,
                                            (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p)));
             std::swap(ks, kd);
@@ -345,6 +327,16 @@ static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_)
                                            ks, vals, vs, pl->large));
     }
     return 0;
+}
+
+template <typename KB, int VB>
+static int run_plan(bcmi_queue q, bcmi_segplan pl, void *keys, void *vals)
+{
+    const int kind = pl->key_dtype == BCMI_F32 || pl->key_dtype == BCMI_F64 ? 2
+                   : (pl->key_dtype == BCMI_I8 || pl->key_dtype == BCMI_I16 || pl->key_dtype == BCMI_I32 ||
+                      pl->key_dtype == BCMI_I64) ? 1 : 0;
+    return with_key_xf<KB>(kind, pl->descending,
+                           [&](auto xf) { return run_plan_xf<KB, VB>(q, pl, keys, vals, xf); });
 }
 
 template <typename KB>
