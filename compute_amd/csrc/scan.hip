@@ -69,100 +69,111 @@ struct alignas(16) sc_chunk {
     T v[N];
 };
 
-// Q = 16-byte chunks per thread (tile = 4 waves * 64 lanes * Q chunks).
-template <typename T, typename P, bool VEC, int kScanQ>
-__global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size_t n, Ctrl *ctrl,
-                                                       uint64_t *status, typename P::Cr init, int exclusive,
-                                                       uint32_t back)
+template <typename P>
+struct ScanLds {
+    typename P::S wave[4];
+    typename P::Cr prefix;
+};
+
+// One tile: Q 16-byte chunks per thread (tile = 4 waves * 64 lanes * Q
+// chunks), wave-striped: chunk q of lane l of wave w covers elements
+// w*WSEG + q*64*V + l*V .. +V-1 of the tile.  FULL: the tile lies below n, so
+// no element needs a bounds test.  VEC: input and output are 16-byte
+// aligned, so full chunks move as one 16-byte load / store.
+template <typename T, typename P, bool VEC, int Q, bool FULL>
+BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
+                    typename P::Cr init, int exclusive, uint32_t epoch)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
     constexpr int V = 16 / sizeof(T);
-    constexpr int WSEG = kScanQ * 64 * V;   // elements per wave
-    constexpr int TILE = 4 * WSEG;          // 4 waves
-    __shared__ S s_wave[4];
-    __shared__ Cr s_prefix;
+    constexpr int WSEG = Q * 64 * V;  // elements per wave
+    constexpr int TILE = 4 * WSEG;    // 4 waves
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t epoch = launch_epoch(ctrl, back);
     const uint32_t tile = blockIdx.x;
-    const size_t tbase = (size_t)tile * TILE;
-    const bool full = tbase + TILE <= n;
-    const size_t wbase = tbase + (size_t)w * WSEG + (size_t)lane * V;
+    // wave-uniform base, so every access is one 32-bit lane offset + immediates
+    const size_t wstart = (size_t)tile * TILE + (size_t)__builtin_amdgcn_readfirstlane(w) * WSEG;
+    const T *win = in + wstart;
+    T *wout = out + wstart;
+    const int64_t rem = FULL ? (int64_t)WSEG : (int64_t)n - (int64_t)wstart;
+    auto ok = [&](int q, int k) { return FULL || (int64_t)(q * 64 * V + lane * V + k) < rem; };
 
-    T x[kScanQ][V];
+    T x[Q][V];
 #pragma unroll
-    for (int q = 0; q < kScanQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
-        if (VEC && full) {
-            sc_chunk<T> c = *(const sc_chunk<T> *)(in + p0);
+    for (int q = 0; q < Q; ++q) {
+        const T *src = win + q * 64 * V + lane * V;
+        if (VEC && FULL) {
+            const sc_chunk<T> c = *(const sc_chunk<T> *)src;
 #pragma unroll
             for (int k = 0; k < V; ++k) x[q][k] = c.v[k];
         } else {
 #pragma unroll
-            for (int k = 0; k < V; ++k) x[q][k] = (p0 + k < n) ? in[p0 + k] : T(0);
+            for (int k = 0; k < V; ++k) x[q][k] = ok(q, k) ? src[k] : T(0);
         }
     }
-    // chunk folds
-    S tot[kScanQ];
-#pragma unroll
-    for (int q = 0; q < kScanQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
-        S s = P::id();
-#pragma unroll
-        for (int k = 0; k < V; ++k)
-            if (full || p0 + k < n) s = P::add(s, x[q][k]);
-        tot[q] = s;
-    }
-    // wave scans of the chunk folds, carried across q
+    // chunk folds, then wave scans of the folds carried across q
     auto mrg = [](S a, S b) { return P::merge(a, b); };
     S carry = P::id();
-    S pre[kScanQ];
+    S pre[Q];
 #pragma unroll
-    for (int q = 0; q < kScanQ; ++q) {
-        S inc = wave_inclusive_scan(tot[q], P::id(), mrg);
-        S exc = wave_shr1(inc, P::id());
-        pre[q] = P::merge(carry, exc);
+    for (int q = 0; q < Q; ++q) {
+        S t = P::id();
+#pragma unroll
+        for (int k = 0; k < V; ++k)
+            if (ok(q, k)) t = P::add(t, x[q][k]);
+        const S inc = wave_inclusive_scan(t, P::id(), mrg);
+        pre[q] = P::merge(carry, wave_shr1(inc, P::id()));
         carry = P::merge(carry, readlane(inc, 63));
     }
-    if (lane == 0) s_wave[w] = carry;
+    if (lane == 0) L.wave[w] = carry;
     __syncthreads();
     S wpre = P::id();
-    for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, s_wave[i]);
+    for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, L.wave[i]);
     if (w == 0) {
-        S total = P::merge(P::merge(s_wave[0], s_wave[1]), P::merge(s_wave[2], s_wave[3]));
-        Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
-                                      [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
-        if (lane == 0) s_prefix = prefix;
+        const S total = P::merge(P::merge(L.wave[0], L.wave[1]), P::merge(L.wave[2], L.wave[3]));
+        const Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
+                                            [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
+        if (lane == 0) L.prefix = prefix;
     }
     __syncthreads();
-    const Cr prefix = s_prefix;
+    const Cr prefix = L.prefix;
 #pragma unroll
-    for (int q = 0; q < kScanQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
+    for (int q = 0; q < Q; ++q) {
         S s = P::merge(wpre, pre[q]);
         T y[V];
 #pragma unroll
         for (int k = 0; k < V; ++k) {
-            if (exclusive) {
-                y[k] = P::out(prefix, s);
-                s = P::add(s, x[q][k]);
-            } else {
-                s = P::add(s, x[q][k]);
-                y[k] = P::out(prefix, s);
-            }
+            const S before = s;
+            s = P::add(s, x[q][k]);
+            y[k] = P::out(prefix, exclusive ? before : s);
         }
-        if (VEC && full) {
+        T *dst = wout + q * 64 * V + lane * V;
+        if (VEC && FULL) {
             sc_chunk<T> c;
 #pragma unroll
             for (int k = 0; k < V; ++k) c.v[k] = y[k];
-            *(sc_chunk<T> *)(out + p0) = c;
+            *(sc_chunk<T> *)dst = c;
         } else {
 #pragma unroll
             for (int k = 0; k < V; ++k)
-                if (p0 + k < n) out[p0 + k] = y[k];
+                if (ok(q, k)) dst[k] = y[k];
         }
     }
+}
+
+template <typename T, typename P, bool VEC, int Q>
+__global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size_t n, Ctrl *ctrl,
+                                                       uint64_t *status, typename P::Cr init, int exclusive,
+                                                       uint32_t back)
+{
+    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
+    __shared__ ScanLds<P> lds;
+    const uint32_t epoch = launch_epoch(ctrl, back);
+    if ((size_t)(blockIdx.x + 1) * TILE <= n)
+        scan_body<T, P, VEC, Q, true>(lds, in, out, n, ctrl, status, init, exclusive, epoch);
+    else
+        scan_body<T, P, VEC, Q, false>(lds, in, out, n, ctrl, status, init, exclusive, epoch);
 }
 
 template <typename T, typename P, int Q>

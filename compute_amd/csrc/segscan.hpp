@@ -86,120 +86,151 @@ BDEV bool key_eq(KT a, KT b) { return a == b; }  // equal_to<T>: -0 == +0, NaN !
 
 enum { SEG_RBK = 0, SEG_INCL = 1, SEG_EXCL = 2 };
 
-// kSegQ = 16-byte value chunks per thread.
-template <typename KT, typename VT, typename P, int MODE, int kSegQ>
-__global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
-                                                         KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
-                                                         uint64_t *status, VT init, uint64_t *count_out,
-                                                         uint32_t back)
+template <typename P>
+struct SegLds {
+    Seg<typename P::A> wave[4];
+    Seg<typename P::CA> prefix;
+};
+
+// V consecutive keys, read or written as one access (16 or 32 bytes here;
+// 8 when 8-byte values pair with 4-byte keys)
+template <typename KT, int V>
+struct alignas(V * sizeof(KT) >= 16 ? 16 : V * sizeof(KT)) key_chunk {
+    KT v[V];
+};
+
+// One tile: Q 16-byte value chunks per thread, wave-striped as in the scan
+// (chunk q of lane l of wave w covers tile elements w*WSEG + q*64*V + l*V ..
+// +V-1).  FULL: the tile lies below n.  VEC: keys and values (and, for the
+// scans, the output) are 16-byte aligned, so full chunks move as vectors.
+// The key before each chunk comes from the lane below (DPP), from lane 63
+// of the previous chunk, or, for the wave's first chunk, from memory.
+template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
+BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
+                       KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
+                       uint64_t *count_out, uint32_t epoch)
 {
     using A = typename P::A;
     using CA = typename P::CA;
     using S = Seg<A>;
     using C = Seg<CA>;
     constexpr int V = 16 / sizeof(VT);
-    constexpr int WSEG = kSegQ * 64 * V;
+    constexpr int WSEG = Q * 64 * V;
     constexpr int TILE = 4 * WSEG;
-    __shared__ S s_wave[4];
-    __shared__ C s_prefix;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
-    const uint32_t epoch = launch_epoch(ctrl, back);
     const uint32_t tile = blockIdx.x;
-    const size_t tbase = (size_t)tile * TILE;
-    const size_t wbase = tbase + (size_t)w * WSEG + (size_t)lane * V;
+    const size_t wstart = (size_t)tile * TILE + (size_t)__builtin_amdgcn_readfirstlane(w) * WSEG;
+    const KT *wk = keys + wstart;
+    const VT *wv = vals + wstart;
+    const int64_t rem = FULL ? (int64_t)WSEG : (int64_t)n - (int64_t)wstart;
+    auto ok = [&](int q, int k) { return FULL || (int64_t)(q * 64 * V + lane * V + k) < rem; };
 
-    VT x[kSegQ][V];
-    KT kk[kSegQ][V + 1];  // kk[q][0] = key before the chunk
-    uint32_t headm[kSegQ];
+    VT x[Q][V];
+    KT kc[Q][V];
 #pragma unroll
-    for (int q = 0; q < kSegQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
-        kk[q][0] = p0 > 0 && p0 - 1 < n ? keys[p0 - 1] : KT(0);
+    for (int q = 0; q < Q; ++q) {
+        const int o = q * 64 * V + lane * V;
+        if (VEC && FULL) {
+            const key_chunk<KT, V> kq = *(const key_chunk<KT, V> *)(wk + o);
+            const key_chunk<VT, V> vq = *(const key_chunk<VT, V> *)(wv + o);
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            bool ok = p0 + k < n;
-            x[q][k] = ok ? vals[p0 + k] : VT(0);
-            kk[q][k + 1] = ok ? keys[p0 + k] : KT(0);
-        }
-        uint32_t m = 0;
+            for (int k = 0; k < V; ++k) {
+                kc[q][k] = kq.v[k];
+                x[q][k] = vq.v[k];
+            }
+        } else {
 #pragma unroll
-        for (int k = 0; k < V; ++k)
-            if (p0 + k == 0 || !key_eq(kk[q][k], kk[q][k + 1])) m |= 1u << k;
-        headm[q] = m;
-    }
-    // chunk folds of (head, value) pairs
-    S tot[kSegQ];
-#pragma unroll
-    for (int q = 0; q < kSegQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
-        S s{P::id(), 0u};
-#pragma unroll
-        for (int k = 0; k < V; ++k) {
-            if (p0 + k < n) {
-                if (headm[q] >> k & 1u) s = S{P::add(P::id(), x[q][k]), s.n + 1};
-                else s.v = P::add(s.v, x[q][k]);
+            for (int k = 0; k < V; ++k) {
+                kc[q][k] = ok(q, k) ? wk[o + k] : KT(0);
+                x[q][k] = ok(q, k) ? wv[o + k] : VT(0);
             }
         }
-        tot[q] = s;
     }
+    // head flags: element i starts a run iff i == 0 or !(key[i-1] == key[i])
+    const KT kfirst = (wstart > 0 && wstart - 1 < n) ? keys[wstart - 1] : KT(0);
+    uint32_t headm[Q];
+    KT kprev[Q];  // the key before each chunk
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        KT prev = wave_shr1(kc[q][V - 1], KT(0));
+        if (lane == 0) prev = q == 0 ? kfirst : readlane(kc[q - 1][V - 1], 63);
+        kprev[q] = prev;
+        uint32_t m = 0;
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            const KT before = k == 0 ? prev : kc[q][k - 1];
+            if (!key_eq(before, kc[q][k])) m |= 1u << k;
+        }
+        if (q == 0 && lane == 0 && wstart == 0) m |= 1u;
+        headm[q] = m;
+    }
+    // chunk folds of (head, value) pairs, wave scans carried across q
     const S sid{P::id(), 0u};
     auto mrg = [](S a, S b) { return seg_merge<A, P>(a, b); };
     S carry = sid;
-    S pre[kSegQ];
+    S pre[Q];
 #pragma unroll
-    for (int q = 0; q < kSegQ; ++q) {
-        S inc = wave_inclusive_scan(tot[q], sid, mrg);
-        S exc = wave_shr1(inc, sid);
-        pre[q] = mrg(carry, exc);
-        carry = mrg(carry, readlane(inc, 63));
-    }
-    if (lane == 0) s_wave[w] = carry;
-    __syncthreads();
-    S wpre = sid;
-    for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, s_wave[i]);
-    if (w == 0) {
-        S total = mrg(mrg(s_wave[0], s_wave[1]), mrg(s_wave[2], s_wave[3]));
-        C agg{P::carry(total.v), total.n};
-        C cid{P::cid(), 0u};
-        C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
-            return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
-        }, ctrl);
-        if (lane == 0) s_prefix = prefix;
-    }
-    __syncthreads();
-    const C prefix = s_prefix;
-#pragma unroll
-    for (int q = 0; q < kSegQ; ++q) {
-        size_t p0 = wbase + (size_t)q * 64 * V;
-        S s = mrg(wpre, pre[q]);  // exclusive local pair before the chunk
+    for (int q = 0; q < Q; ++q) {
+        S t = sid;
 #pragma unroll
         for (int k = 0; k < V; ++k) {
-            const size_t i = p0 + k;
-            if (i >= n) break;
+            if (ok(q, k)) {
+                if (headm[q] >> k & 1u) t = S{P::add(P::id(), x[q][k]), t.n + 1};
+                else t.v = P::add(t.v, x[q][k]);
+            }
+        }
+        const S inc = wave_inclusive_scan(t, sid, mrg);
+        pre[q] = mrg(carry, wave_shr1(inc, sid));
+        carry = mrg(carry, readlane(inc, 63));
+    }
+    if (lane == 0) L.wave[w] = carry;
+    __syncthreads();
+    S wpre = sid;
+    for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, L.wave[i]);
+    if (w == 0) {
+        const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
+        const C agg{P::carry(total.v), total.n};
+        const C cid{P::cid(), 0u};
+        const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
+            return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
+        }, ctrl);
+        if (lane == 0) L.prefix = prefix;
+    }
+    __syncthreads();
+    const C prefix = L.prefix;
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int o = q * 64 * V + lane * V;
+        S s = mrg(wpre, pre[q]);  // exclusive local pair before the chunk
+        VT y[V];
+#pragma unroll
+        for (int k = 0; k < V; ++k) {
+            if (!ok(q, k)) continue;
+            const size_t i = wstart + o + k;
             const bool head = headm[q] >> k & 1u;
             if constexpr (MODE == SEG_RBK) {
                 if (head && i > 0) {
-                    // finished run i-1: exclusive pair at i
-                    uint32_t nb = prefix.n + s.n;
+                    // finished run i-1 = the exclusive pair at i
+                    const uint32_t nb = prefix.n + s.n;
                     if (nb >= 1 && nb <= n) {
-                        keys_out[nb - 1] = kk[q][k];
+                        keys_out[nb - 1] = k == 0 ? kprev[q] : kc[q][k - 1];
                         out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
                     } else {
                         __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                     }
                 }
             } else if constexpr (MODE == SEG_EXCL) {
-                out[i] = head ? init : P::fin_init(init, prefix.v, s.v, s.n == 0);
+                y[k] = head ? init : P::fin_init(init, prefix.v, s.v, s.n == 0);
             }
             if (head) s = S{P::add(P::id(), x[q][k]), s.n + 1};
             else s.v = P::add(s.v, x[q][k]);
-            if constexpr (MODE == SEG_INCL) out[i] = P::fin(prefix.v, s.v, s.n == 0);
+            if constexpr (MODE == SEG_INCL) y[k] = P::fin(prefix.v, s.v, s.n == 0);
             if constexpr (MODE == SEG_RBK) {
                 if (i == n - 1) {
-                    uint32_t nb = prefix.n + s.n;
+                    const uint32_t nb = prefix.n + s.n;
                     if (nb >= 1 && nb <= n) {
-                        keys_out[nb - 1] = kk[q][k + 1];
+                        keys_out[nb - 1] = kc[q][k];
                         out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
                     } else {
                         __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -208,7 +239,37 @@ __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ 
                 }
             }
         }
+        if constexpr (MODE != SEG_RBK) {
+            VT *dst = out + wstart + o;
+            if (VEC && FULL) {
+                key_chunk<VT, V> c;
+#pragma unroll
+                for (int k = 0; k < V; ++k) c.v[k] = y[k];
+                *(key_chunk<VT, V> *)dst = c;
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k)
+                    if (ok(q, k)) dst[k] = y[k];
+            }
+        }
     }
+}
+
+template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC>
+__global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
+                                                         KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
+                                                         uint64_t *status, VT init, uint64_t *count_out,
+                                                         uint32_t back)
+{
+    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
+    __shared__ SegLds<P> lds;
+    const uint32_t epoch = launch_epoch(ctrl, back);
+    if ((size_t)(blockIdx.x + 1) * TILE <= n)
+        segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                    count_out, epoch);
+    else
+        segscan_body<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                     count_out, epoch);
 }
 
 template <typename KT, typename VT, typename P, int MODE, int Q>
@@ -217,11 +278,18 @@ static int launch_seg(bcmi_queue q, const void *keys, const void *vals, void *ou
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
     size_t tiles = (n + TILE - 1) / TILE;
+    const bool vec = (((uintptr_t)keys | (uintptr_t)vals | (MODE == SEG_RBK ? 0 : (uintptr_t)out)) & 15) == 0;
     BCMI_TRY(begin_lookback(q, 1));
-    BCMI_LAUNCH(q, MODE == SEG_RBK ? "reduce_by_key" : "segmented_scan",
-                hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q>), dim3((unsigned)tiles), dim3(kSegThreads), 0,
-                                   q->stream, (const KT *)keys, (const VT *)vals, (VT *)out, (KT *)keys_out, n,
-                                   q->ctrl, status, ini, count_out, 1u));
+    if (vec)
+        BCMI_LAUNCH(q, MODE == SEG_RBK ? "reduce_by_key" : "segmented_scan",
+                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, true>), dim3((unsigned)tiles), dim3(kSegThreads),
+                                       0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out, (KT *)keys_out, n,
+                                       q->ctrl, status, ini, count_out, 1u));
+    else
+        BCMI_LAUNCH(q, MODE == SEG_RBK ? "reduce_by_key" : "segmented_scan",
+                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, false>), dim3((unsigned)tiles),
+                                       dim3(kSegThreads), 0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out,
+                                       (KT *)keys_out, n, q->ctrl, status, ini, count_out, 1u));
     return 0;
 }
 
