@@ -67,6 +67,24 @@ def kernel_avg_ms(q, fn, kernel, reps=3, restore=None):
     return (tot / n if n else float("nan")), n // reps if reps else n
 
 
+def kernel_breakdown(q, fn, kernels, reps=3, restore=None):
+    """Per-call milliseconds of each named kernel (queue event profiler)."""
+    q.profile_reset()
+    q.profile(True)
+    for _ in range(reps):
+        if restore:
+            restore()
+        fn()
+    q.profile(False)
+    out = {}
+    for k in kernels:
+        n, tot = q.profile_query(k)
+        if n:
+            out[k] = round(tot / reps, 4)
+    q.profile_reset()
+    return out
+
+
 def d2d(ca, q, dst, src):
     ca.check(ca.lib().bcmi_memcpy(q.h, dst.ptr, src.ptr, src.nbytes, ca.D2D, 0))
 
@@ -283,6 +301,10 @@ def main():
                 if ln > 1:
                     L.bcmi_sort(q.h, ca.STABLE_SORT_BY_KEY, ca.U64, wk.at(s_), wv.at(s_), 4, ln, 0)
 
+        parts = kernel_breakdown(q, fn, ["segsort_small", "segsort_insertion", "segsort_histogram",
+                                         "segsort_digit_offsets", "segsort_route", "segsort_onesweep_msd",
+                                         "segsort_groups", "segsort_onesweep", "segsort_copyback"],
+                                 restore=restore)
         ms, wall = timed_steps(ca, q, per_call, 1, 1, restore)
         t_call = float(ms[0])
         sec["segmented_sort_u64_u32_65536seg"] = {
@@ -290,6 +312,7 @@ def main():
             "batched_graph_ms": round(t_graph, 4), "batched_eager_ms": round(t_eager, 4),
             "per_call_ms": round(t_call, 2),
             "speedup_graph_vs_per_call": round(t_call / t_graph, 1),
+            "kernels_ms_per_call": parts,
             "Gpairs_per_s": round(n / (t_graph * 1e-3) / 1e9, 3),
             "GBps_alg_24B": round(gbs(24 * n, t_graph), 1),
             "frac_24B": round(gbs(24 * n, t_graph) / PEAK_GBS, 4)}

@@ -14,7 +14,8 @@ threads by the kernels' LDS layout; what varies is the work per thread):
   __bcmi_radix_sort|ipt, |rank         keys-only 4-byte radix sort
   __bcmi_radix_sort_by_key|ipt, |rank  4-byte keys with 4-byte values
   __bcmi_scan|q                        4-byte scans, items per lane
-  __bcmi_segscan|q                     reduce_by_key / segmented scan
+  __bcmi_reduce_by_key|q               reduce_by_key, items per lane
+  __bcmi_segscan|q                     segmented scans, items per lane
 
     python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
 """
@@ -29,6 +30,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
 SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 24)]  # (rank, ipt)
+SORT_KEYS_ONLY_EXTRA = [(0, 40), (0, 48), (0, 64)]
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 
@@ -60,7 +62,7 @@ def tune_sort(ca, W, q, log2n, log):
     work = ca.DeviceArray(n, np.uint32, q)
     restore = lambda: ca.check(L.bcmi_memcpy(q.h, work.ptr, src.ptr, src.nbytes, ca.D2D, 0))
     best = None
-    for rank, ipt in SORT_CANDIDATES:
+    for rank, ipt in SORT_CANDIDATES + SORT_KEYS_ONLY_EXTRA:
         _set(ca, "__bcmi_radix_sort", "rank", rank)
         _set(ca, "__bcmi_radix_sort", "ipt", ipt)
         ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
@@ -141,16 +143,44 @@ def tune_segscan(ca, W, q, log2n, log):
     cnt = ca.DeviceArray(1, np.uint64, q)
     best = None
     for qq in SEGSCAN_Q:
-        _set(ca, "__bcmi_segscan", "q", qq)
+        _set(ca, "__bcmi_reduce_by_key", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.check(L.bcmi_reduce_by_key(q.h, ca.I32, ca.F32, ca.PLUS, dk.ptr, dv.ptr, n,
                                                                         ko.ptr, vo.ptr, cnt.ptr, 0)))
         ok = int(cnt.to_host(q)[0]) == runs and bool(np.array_equal(ko.to_host(q, runs), last_keys))
         got = vo.to_host(q, runs).astype(np.float64)
         ok = ok and bool(np.all(np.abs(got - sums) <= 2.0 ** -23 * np.abs(sums) + 2.0 ** -40 * mags))
-        ms2 = _median_ms(ca, q, lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0))
-        log({"tunable": "__bcmi_segscan", "q": qq, "rbk_ms": round(ms, 4), "segscan_ms": round(ms2, 4), "ok": ok})
-        if ok and (best is None or ms + ms2 < best[0]):
-            best = (ms + ms2, qq)
+        log({"tunable": "__bcmi_reduce_by_key", "q": qq, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, qq)
+    if best is None:
+        raise RuntimeError("autotune: no reduce_by_key candidate matched")
+    _set(ca, "__bcmi_reduce_by_key", "q", best[1])
+    # segmented exclusive scan of the same runs (each run restarts at 0),
+    # accumulated run by run in float64: step t adds the t-th element of
+    # every run longer than t (runs ordered by length, so a prefix of them)
+    lens = np.diff(np.r_[heads, n])
+    by_len = np.argsort(-lens, kind="stable")
+    sl = lens[by_len]
+    acc = np.zeros(runs)
+    amag = np.zeros(runs)
+    ex = np.empty(n)
+    emag = np.empty(n)
+    for t in range(int(sl[0])):
+        c = int(np.searchsorted(-sl, -t, side="left"))  # runs with length > t
+        pos = heads[by_len[:c]] + t
+        ex[pos] = acc[:c]
+        emag[pos] = amag[:c]
+        acc[:c] += v64[pos]
+        amag[:c] += np.abs(v64[pos])
+    best = None
+    for qq in SEGSCAN_Q:
+        _set(ca, "__bcmi_segscan", "q", qq)
+        ms = _median_ms(ca, q, lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0))
+        got = vo.to_host(q).astype(np.float64)
+        ok = bool(np.all(np.abs(got - ex) <= 2.0 ** -23 * np.abs(ex) + 2.0 ** -36 * emag))
+        log({"tunable": "__bcmi_segscan", "q": qq, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, qq)
     if best is None:
         raise RuntimeError("autotune: no segmented scan candidate matched")
     _set(ca, "__bcmi_segscan", "q", best[1])

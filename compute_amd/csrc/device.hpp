@@ -217,7 +217,10 @@ BDEV uint32_t lb_read(const uint64_t *status, size_t tile, uint32_t epoch, Cr &v
 // prefix and returns its exclusive prefix (init folded in at tile 0).
 // `cop(a, b)` must be associative; a is the earlier operand.  Lane l reads
 // tile (base - l), so the lanes are folded with the operands swapped to keep
-// tile order.
+// tile order.  When a word before the first inclusive prefix is not yet
+// published, only that one word is polled (by lane 0) until it appears, and
+// then the 64-wide read is repeated: a thousand waiting tiles re-reading 64
+// words each would load the memory system with coherent reads.
 template <typename Cr, typename Op>
 BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, Cr init, Cr cid, Op cop,
                       Ctrl *ctrl)
@@ -231,16 +234,32 @@ BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, C
     if (lane == 0) lb_publish(status, tile, epoch, LB_A, agg);
     Cr acc = cid;
     int64_t base = (int64_t)tile - 1;
+    bool gave_up = false;
     while (true) {
-        int64_t idx = base - (int64_t)lane;
+        const int64_t idx = base - (int64_t)lane;
         Cr v = cid;
-        uint32_t flag = LB_P;
+        uint32_t flag = LB_P;  // before tile 0: the identity, as an inclusive prefix
         if (idx >= 0) flag = lb_read(status, (size_t)idx, epoch, v);
-        uint64_t pm = ballot(flag == LB_P);
-        uint64_t xm = ballot(flag == LB_X);
-        uint32_t first = pm ? (uint32_t)__builtin_ctzll(pm) : 64u;
-        uint64_t rel = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
-        if ((xm & rel) && !spin_expired(ctrl, spins)) continue;
+        const uint64_t pm = ballot(flag == LB_P);
+        const uint64_t xm = ballot(flag == LB_X);
+        const uint32_t first = pm ? (uint32_t)__builtin_ctzll(pm) : 64u;
+        const uint64_t rel = first >= 63 ? ~0ull : ((2ull << first) - 1ull);
+        if ((xm & rel) && !gave_up) {
+            const int64_t widx = base - (int64_t)__builtin_ctzll(xm & rel);
+            while (true) {
+                uint32_t f = LB_X;
+                if (lane == 0) {
+                    Cr t;
+                    f = lb_read(status, (size_t)widx, epoch, t);
+                }
+                if (__builtin_amdgcn_readfirstlane(f) != LB_X) break;
+                if (spin_expired(ctrl, spins)) {
+                    gave_up = true;  // recorded as a fault; finish with what is there
+                    break;
+                }
+            }
+            continue;
+        }
         Cr mine = lane <= first ? v : cid;
         Cr older = wave_reduce(mine, cid, [&](Cr a, Cr b) { return cop(b, a); });
         acc = cop(older, acc);

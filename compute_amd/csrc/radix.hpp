@@ -61,7 +61,9 @@ template <> struct ValT<1> { using type = uint8_t; };
 template <> struct ValT<2> { using type = uint16_t; };
 template <> struct ValT<4> { using type = uint32_t; };
 template <> struct ValT<8> { using type = uint64_t; };
-struct alignas(16) v16_t { uint64_t a, b; };
+// 16-byte payloads as a native 4 x u32 vector: it lives in registers (a
+// struct of two u64 was kept in scratch memory by the compiler)
+typedef uint32_t v16_t __attribute__((ext_vector_type(4)));
 template <> struct ValT<16> { using type = v16_t; };
 
 // ------------------------------------------------------------ ranking
@@ -230,7 +232,8 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     L.cnt[3][tid] = lstart + c0 + c1 + c2;
     // decoupled look-back for this digit: fetch kLookback predecessors'
     // status words at once (one round trip), fold them in tile order until
-    // an inclusive prefix; stop and re-poll at a word not yet published.
+    // an inclusive prefix; at a word not yet published, poll that word alone
+    // until it is, then continue.
     uint32_t excl = 0;
     if (tile > 0) {
         int64_t kt = (int64_t)tile - 1;
@@ -255,7 +258,13 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
             }
             if (found) break;
             kt -= used;
-            if (stalled && used == 0 && spin_expired(ctrl, spins)) break;
+            if (stalled && used == 0) {
+                // poll just the unpublished word, then read 8-wide again
+                bool expired = false;
+                while (lb_epoch(lb_load(&status[(size_t)kt * 256 + tid])) != epoch)
+                    if ((expired = spin_expired(ctrl, spins))) break;
+                if (expired) break;
+            }
         }
         lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_P, excl + total));
     }

@@ -47,9 +47,12 @@ struct SegTile {
 };
 
 // ------------------------------------------------------------ small path
+// Sorts each span of segs entirely in LDS.  The keys (values) are read from
+// ksrc (vsrc) and the sorted span written to keys (vals) at the same offset;
+// the small-segment route sorts in place (ksrc == keys).
 template <typename KB, int VB, int IPT, typename XF>
-__global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__ keys,
-                                                                typename ValT<VB>::type *__restrict__ vals,
+__global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, const typename ValT<VB>::type *vsrc,
+                                                                KB *keys, typename ValT<VB>::type *vals,
                                                                 const SegSpan *__restrict__ segs, XF xf)
 {
     using V = typename ValT<VB>::type;
@@ -64,9 +67,12 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const SegSpan sg = segs[blockIdx.x];
+    const uint32_t len = sg.len;
+    if (len == 0) return;  // an unused group slot
+    const KB *ks = ksrc + sg.base;
+    const V *vs = VB ? vsrc + sg.base : nullptr;
     KB *kp = keys + sg.base;
     V *vp = VB ? vals + sg.base : nullptr;
-    const uint32_t len = sg.len;
     auto pos = [&](int j) { return w * 64 * IPT + j * 64 + lane; };
     auto valid = [&](int j) { return pos(j) < len; };
     const int32_t lim = (int32_t)len - (int32_t)(w * 64 * IPT);
@@ -75,9 +81,9 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(KB *__restrict__
     V v[IPT];
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
-        k[j] = valid(j) ? kp[pos(j)] : KB(0);
+        k[j] = valid(j) ? ks[pos(j)] : KB(0);
         if constexpr (VB > 0) {
-            if (valid(j)) v[j] = vp[pos(j)];
+            if (valid(j)) v[j] = vs[pos(j)];
         }
     }
     for (int i = tid; i < 4 * 256; i += kSegSmallThreads) (&s_peer[0][0])[i] = 0;
@@ -205,9 +211,11 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                                uint32_t pass, uint32_t passes,
                                                                const uint32_t *__restrict__ seg_off, XF xf,
                                                                Ctrl *ctrl, uint64_t *__restrict__ status,
-                                                               uint32_t back)
+                                                               uint32_t back, const uint32_t *__restrict__ route,
+                                                               uint32_t want)
 {
     const SegTile t = tiles[blockIdx.x];
+    if (route[t.seg] != want) return;  // the segment takes the other route
     const size_t b = t.base;
     onesweep_tile<KB, VB, IPT, kRankLdsMask, XF>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
                                              t.tile, shift, seg_off + ((size_t)t.seg * passes + pass) * 256, xf,
@@ -215,11 +223,71 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                              launch_epoch(ctrl, back));
 }
 
+// Route of a large segment, decided on the device from its histogram:
+//   kRouteLsd: all P onesweep passes over the segment (any distribution);
+//   kRouteMsd: one onesweep pass on the most significant digit into the
+//     temporaries, which partitions the segment stably into its 256 digit
+//     buckets; consecutive buckets are packed greedily into groups of at
+//     most `cap` keys, and each group is sorted in LDS by the whole key and
+//     written back (a group is a contiguous range of the sorted segment, so
+//     sorting it by the whole key is exact, and stable).  Taken when every
+//     bucket fits one group and the groups fit the segment's slots.
+enum : uint32_t { kRouteLsd = 0, kRouteMsd = 1 };
+
+// One block (256 threads, thread = digit) per large segment.
+__global__ __launch_bounds__(256) void k_seg_route(const SegSpan *__restrict__ large,
+                                                   const uint32_t *__restrict__ seg_off, uint32_t passes,
+                                                   const uint32_t *__restrict__ slot_begin, SegSpan *groups,
+                                                   uint32_t *route, uint32_t cap, uint32_t enable)
+{
+    __shared__ uint32_t s_start[257];
+    const uint32_t s = blockIdx.x, d = threadIdx.x;
+    const SegSpan sg = large[s];
+    const uint32_t *off = seg_off + ((size_t)s * passes + passes - 1) * 256;
+    s_start[d] = off[d];
+    if (d == 0) s_start[256] = sg.len;
+    __syncthreads();
+    const uint32_t size = s_start[d + 1] - s_start[d];
+    const int big = __syncthreads_or(size > cap);
+    const uint32_t b0 = slot_begin[s], b1 = slot_begin[s + 1];
+    if (d != 0) return;
+    // count the greedy groups first, so the route is decided before anything is written
+    uint32_t ngroups = 0, gl = 0;
+    for (int b = 0; b < 256; ++b) {
+        const uint32_t z = s_start[b + 1] - s_start[b];
+        if (gl > 0 && gl + z > cap) {
+            ++ngroups;
+            gl = 0;
+        }
+        gl += z;
+    }
+    if (gl > 0) ++ngroups;
+    const bool msd = enable && !big && ngroups <= b1 - b0;
+    route[s] = msd ? kRouteMsd : kRouteLsd;
+    uint32_t slot = b0;
+    if (msd) {
+        uint32_t gs = 0;
+        gl = 0;
+        for (int b = 0; b < 256; ++b) {
+            const uint32_t z = s_start[b + 1] - s_start[b];
+            if (gl > 0 && gl + z > cap) {
+                groups[slot++] = SegSpan{sg.base + gs, gl, 0};
+                gl = 0;
+            }
+            if (gl == 0) gs = s_start[b];
+            gl += z;
+        }
+        if (gl > 0) groups[slot++] = SegSpan{sg.base + gs, gl, 0};
+    }
+    for (; slot < b1; ++slot) groups[slot] = SegSpan{0, 0, 0};
+}
+
 // odd pass count (8-bit keys): copy the large segments back from the temporaries
 template <typename KB, int VB>
 __global__ void k_seg_copyback(KB *keys, const KB *ktmp, typename ValT<VB>::type *vals,
-                               const typename ValT<VB>::type *vtmp, const SegSpan *segs)
+                               const typename ValT<VB>::type *vtmp, const SegSpan *segs, const uint32_t *route)
 {
+    if (route[blockIdx.x] != kRouteLsd) return;
     const SegSpan s = segs[blockIdx.x];
     for (uint32_t i = threadIdx.x; i < s.len; i += blockDim.x) {
         keys[s.base + i] = ktmp[s.base + i];
@@ -238,7 +306,8 @@ struct bcmi_segplan_s {
     size_t value_bytes = 0;
     int descending = 0;
     uint64_t total = 0;  // elements covered by the offsets
-    int small_ipt_max = 16;
+    int small_cls = 4;      // largest small class: capacity kSegSmallThreads << small_cls
+    uint32_t nslots = 0;     // group slots of the MSD route, over all large segments
     uint32_t nsmall[5] = {0, 0, 0, 0, 0};  // classes IPT 1, 2, 4, 8, 16
     uint32_t ninsert = 0;
     uint32_t nlarge = 0, ntiles = 0, nchunks = 0;
@@ -248,6 +317,9 @@ struct bcmi_segplan_s {
     bcmi::SegSpan *insert = nullptr;
     bcmi::SegSpan *large = nullptr;
     bcmi::SegSpan *chunks = nullptr;
+    uint32_t *route = nullptr;       // per large segment: kRouteLsd / kRouteMsd (set on the device)
+    uint32_t *slot_begin = nullptr;  // per large segment: first group slot (nlarge + 1 entries)
+    bcmi::SegSpan *groups = nullptr; // MSD route groups (written on the device)
     uint32_t *chunk_begin = nullptr;
     bcmi::SegTile *tiles = nullptr;
     uint32_t *partial = nullptr;
@@ -273,13 +345,14 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
     constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     KB *keys = (KB *)keys_;
     V *vals = (V *)vals_;
-    // small classes
+    // small classes (sorted in place)
 #define BCMI_SMALL(CLS, IPTV)                                                                                   \
     if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
         if (pl->nsmall[CLS])                                                                                    \
             BCMI_LAUNCH(q, "segsort_small",                                                                     \
                         hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV, XF>), dim3(pl->nsmall[CLS]),                  \
-                                           dim3(kSegSmallThreads), 0, q->stream, keys, vals, pl->small[CLS], xf)); \
+                                           dim3(kSegSmallThreads), 0, q->stream, keys, vals, keys, vals,          \
+                                           pl->small[CLS], xf));                                                \
     }
     BCMI_SMALL(0, 1)
     BCMI_SMALL(1, 2)
@@ -308,23 +381,51 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
         BCMI_LAUNCH(q, "segsort_digit_offsets",
                     hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, P), dim3(256), 0, q->stream, pl->partial,
                                        pl->chunk_begin, (uint32_t)P, pl->seg_off));
+        // route each large segment (MSD needs two digits or more)
+        BCMI_LAUNCH(q, "segsort_route",
+                    hipLaunchKernelGGL(k_seg_route, dim3(pl->nlarge), dim3(256), 0, q->stream, pl->large, pl->seg_off,
+                                       (uint32_t)P, pl->slot_begin, pl->groups, pl->route,
+                                       (uint32_t)(kSegSmallThreads << pl->small_cls), (uint32_t)(P >= 2)));
         uint64_t *status;
         BCMI_TRY(status_array(q, (size_t)pl->ntiles * 256, &status));
-        BCMI_TRY(begin_lookback(q, P));
+        BCMI_TRY(begin_lookback(q, P + 1));
+        // MSD route: top-digit partition into the temporaries ...
+        BCMI_LAUNCH(q, "segsort_onesweep_msd",
+                    hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
+                                       q->stream, keys, (KB *)pl->ktmp, vals, (V *)pl->vtmp, pl->tiles,
+                                       (uint32_t)(8 * (P - 1)), (uint32_t)(P - 1), (uint32_t)P, pl->seg_off, xf,
+                                       q->ctrl, status, (uint32_t)(P + 1), pl->route, (uint32_t)kRouteMsd));
+        // ... then every group sorted in LDS from the temporaries back into place
+#define BCMI_GROUPS(CLS, IPTV)                                                                                  \
+    if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
+        if (pl->small_cls == CLS && pl->nslots)                                                                 \
+            BCMI_LAUNCH(q, "segsort_groups",                                                                    \
+                        hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV, XF>), dim3(pl->nslots), dim3(kSegSmallThreads), \
+                                           0, q->stream, (const KB *)pl->ktmp, (const V *)pl->vtmp, keys, vals,   \
+                                           pl->groups, xf));                                                    \
+    }
+        BCMI_GROUPS(0, 1)
+        BCMI_GROUPS(1, 2)
+        BCMI_GROUPS(2, 4)
+        BCMI_GROUPS(3, 8)
+        BCMI_GROUPS(4, 16)
+#undef BCMI_GROUPS
+        // LSD route: all P passes
         KB *ks = keys, *kd = (KB *)pl->ktmp;
         V *vs = vals, *vd = (V *)pl->vtmp;
         for (int p = 0; p < P; ++p) {
             BCMI_LAUNCH(q, "segsort_onesweep",
                         hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
                                            q->stream, ks, kd, vs, vd, pl->tiles, (uint32_t)(8 * p), (uint32_t)p,
-                                           (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p)));
+                                           (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p),
+                                           pl->route, (uint32_t)kRouteLsd));
             std::swap(ks, kd);
             std::swap(vs, vd);
         }
         if (ks != keys)
             BCMI_LAUNCH(q, "segsort_copyback",
                         hipLaunchKernelGGL((k_seg_copyback<KB, VB>), dim3(pl->nlarge), dim3(256), 0, q->stream, keys,
-                                           ks, vals, vs, pl->large));
+                                           ks, vals, vs, pl->large, pl->route));
     }
     return 0;
 }
@@ -377,7 +478,7 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     const int ipt = large_ipt(kb, value_bytes);
     const uint64_t tile = (uint64_t)kSortThreads * ipt;
     std::vector<SegSpan> small[5], insert, large, chunks;
-    std::vector<uint32_t> chunk_begin(1, 0);
+    std::vector<uint32_t> chunk_begin(1, 0), slot_begin(1, 0);
     std::vector<SegTile> tiles;
     uint64_t total = nseg ? host_offsets[nseg] : 0;
     for (size_t s = 0; s < nseg; ++s) {
@@ -402,6 +503,10 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
                 chunks.push_back(SegSpan{base + c, (uint32_t)cl, li});
             }
             chunk_begin.push_back((uint32_t)chunks.size());
+            // greedy groups of <= small_max keys: any two consecutive ones
+            // hold more than small_max, so at most 2*len/small_max + 1
+            const uint64_t gmax = std::min<uint64_t>(256, 2 * (len / small_max) + 2);
+            slot_begin.push_back(slot_begin.back() + (uint32_t)gmax);
         }
     }
     bcmi_segplan pl = new bcmi_segplan_s();
@@ -428,6 +533,9 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     o_tiles = off; off += al256(tiles.size() * sizeof(SegTile));
     o_partial = off; off += al256(chunks.size() * (size_t)kb * 256 * 4);
     o_segoff = off; off += al256(large.size() * (size_t)kb * 256 * 4);
+    const size_t o_route = off; off += al256(large.size() * sizeof(uint32_t));
+    const size_t o_sb = off; off += al256(slot_begin.size() * sizeof(uint32_t));
+    const size_t o_groups = off; off += al256((size_t)slot_begin.back() * sizeof(SegSpan));
     // temporaries cover the elements of large segments at their own offsets
     const uint64_t span = large.empty() ? 0 : (large.back().base + large.back().len);
     o_k = off; off += al256(span * (size_t)kb);
@@ -446,6 +554,11 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     pl->tiles = (SegTile *)(m + o_tiles);
     pl->partial = (uint32_t *)(m + o_partial);
     pl->seg_off = (uint32_t *)(m + o_segoff);
+    pl->route = (uint32_t *)(m + o_route);
+    pl->slot_begin = (uint32_t *)(m + o_sb);
+    pl->groups = (SegSpan *)(m + o_groups);
+    pl->nslots = slot_begin.back();
+    pl->small_cls = cmax;
     pl->ktmp = m + o_k;
     pl->vtmp = m + o_v;
     auto up = [&](void *dst, const void *src, size_t bytes) -> int {
@@ -459,6 +572,7 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     if (!st) st = up(pl->chunks, chunks.data(), chunks.size() * sizeof(SegSpan));
     if (!st) st = up(pl->chunk_begin, chunk_begin.data(), chunk_begin.size() * sizeof(uint32_t));
     if (!st) st = up(pl->tiles, tiles.data(), tiles.size() * sizeof(SegTile));
+    if (!st) st = up(pl->slot_begin, slot_begin.data(), slot_begin.size() * sizeof(uint32_t));
     if (st) {
         (void)hipFree(pl->mem);
         delete pl;

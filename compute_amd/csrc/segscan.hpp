@@ -293,8 +293,9 @@ static int launch_seg(bcmi_queue q, const void *keys, const void *vals, void *ou
     return 0;
 }
 
-// Chunks per thread from the tuning cache (key "__bcmi_segscan", param
-// "q": 2, 4 or 8) for int32 keys with 4-byte values; 4 otherwise.
+// Chunks per thread from the tuning cache (key "__bcmi_reduce_by_key" or
+// "__bcmi_segscan", param "q": 2, 4 or 8) for 4-byte keys with 4-byte
+// values; 4 otherwise.
 template <typename KT, typename VT, typename P, int MODE>
 static int run_seg(bcmi_queue q, const void *keys, const void *vals, void *out, void *keys_out, size_t n,
                    const void *init, uint64_t *count_out)
@@ -307,7 +308,7 @@ static int run_seg(bcmi_queue q, const void *keys, const void *vals, void *out, 
     VT ini{};
     if (init) memcpy(&ini, init, sizeof(VT));
     if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4) {
-        const unsigned qq = tune_get("__bcmi_segscan", "q", 8);
+        const unsigned qq = tune_get(MODE == SEG_RBK ? "__bcmi_reduce_by_key" : "__bcmi_segscan", "q", 8);
         if (qq == 8) return launch_seg<KT, VT, P, MODE, 8>(q, keys, vals, out, keys_out, n, status, ini, count_out);
         if (qq == 2) return launch_seg<KT, VT, P, MODE, 2>(q, keys, vals, out, keys_out, n, status, ini, count_out);
     }

@@ -551,6 +551,35 @@ def test_segmented_sort_float_small_segments_keep_signed_zero_order(queue):
     assert list(ev[:3]) == [0, 1, 2]
 
 
+@pytest.mark.parametrize("dt", ["uint32", "uint64", "float32", "int16"])
+def test_segmented_sort_routes_follow_the_data(queue, dt):
+    """Large segments whose top digit splits them into buckets that fit a
+    group take the top-digit route; a segment whose keys share the top digit
+    (one bucket larger than a group) falls back to all digit passes.  The
+    route is chosen on the device per run, so one plan run on two inputs
+    with the roles swapped must match the per-segment sorts both times."""
+    lens = [20000, 20000, 50, 9000, 0, 4097]
+    off = _segments(lens)
+    n = int(off[-1])
+    uni = random_keys(dt, n)
+    ubits = np.dtype("u%d" % uni.itemsize)
+    low = np.iinfo(ubits).max >> 8
+
+    def one_bucket(sl):  # clear the top byte of the keys in sl
+        k = uni.copy()
+        k.view(ubits)[sl] &= low
+        return k
+
+    v = np.arange(n, dtype=np.uint32)
+    plan = ca.SegmentedPlan(queue, np.dtype(dt), np.uint32, off)
+    for keys in (one_bucket(slice(0, 20000)), one_bucket(slice(20000, 40000))):
+        dk, dv = dev(queue, keys), dev(queue, v)
+        plan.run(queue, dk, dv)
+        ek, ev = O.segmented_sort(keys, v, off)
+        assert np.array_equal(bits(dk.to_host(queue)), bits(ek))
+        assert np.array_equal(dv.to_host(queue), ev)
+
+
 def test_segmented_sort_workload_sample(queue):
     """The segmented workload's generator at 2048 segments (lognormal lengths,
     median 256, one 2^20 segment): bit-identical to per-segment sorts."""
