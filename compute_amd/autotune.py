@@ -31,6 +31,7 @@ DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
 SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 24)]  # (rank, ipt)
 SORT_KEYS_ONLY_EXTRA = [(0, 40), (0, 48), (0, 64)]
+SORT_PAIRS_EXTRA = [(0, 40), (0, 48)]
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 
@@ -85,7 +86,7 @@ def tune_sort(ca, W, q, log2n, log):
 
     first = None
     best = None
-    for rank, ipt in SORT_CANDIDATES:
+    for rank, ipt in SORT_CANDIDATES + SORT_PAIRS_EXTRA:
         _set(ca, "__bcmi_radix_sort_by_key", "rank", rank)
         _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
         ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)

@@ -102,16 +102,16 @@ struct Ranks {
     BDEV uint32_t operator[](int j) const { return (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xFFFFu); }
 };
 
-template <int IPT, bool FULL, int RANK, typename KB, typename XF>
-BDEV void rank_in_wave(const KB (&k)[IPT], Ranks<IPT> &r, const XF &xf, uint32_t shift, uint32_t *cnt,
-                       uint64_t *peer, int32_t lim)
+// dig(key) gives the digit in [0, 256).
+template <int IPT, bool FULL, int RANK, typename KB, typename DigitF>
+BDEV void rank_in_wave_by(const KB (&k)[IPT], Ranks<IPT> &r, DigitF dig, uint32_t *cnt, uint64_t *peer, int32_t lim)
 {
     const uint32_t lane = threadIdx.x & 63;
     const uint64_t lanebit = 1ull << lane;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const bool ok = FULL || (int32_t)(j * 64 + lane) < lim;
-        const uint32_t d = xf.digit(k[j], shift);
+        const uint32_t d = dig(k[j]);
         uint64_t peers;
         if constexpr (RANK == 0) {
             if (ok) __hip_atomic_fetch_or(&peer[d], lanebit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -133,6 +133,13 @@ BDEV void rank_in_wave(const KB (&k)[IPT], Ranks<IPT> &r, const XF &xf, uint32_t
                 __hip_atomic_store(&peer[d], 0ull, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         }
     }
+}
+
+template <int IPT, bool FULL, int RANK, typename KB, typename XF>
+BDEV void rank_in_wave(const KB (&k)[IPT], Ranks<IPT> &r, const XF &xf, uint32_t shift, uint32_t *cnt,
+                       uint64_t *peer, int32_t lim)
+{
+    rank_in_wave_by<IPT, FULL, RANK>(k, r, [&](KB x) { return xf.digit(x, shift); }, cnt, peer, lim);
 }
 
 // Block-wide exclusive scan of one value per thread (256 threads).

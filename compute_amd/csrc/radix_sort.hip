@@ -212,9 +212,9 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
         if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
         if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc);
+        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc);
+        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if constexpr (VB == 0) {
-            if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc);
-            if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc);
             if (ipt == 64) return radix_sort<KB, VB, 64, kRankLdsMask>(q, keys, vals, n, kind, desc);
         }
         // any other value falls through to the generic tile below

@@ -47,9 +47,20 @@ struct SegTile {
 };
 
 // ------------------------------------------------------------ small path
+constexpr uint32_t kBucketMax = 64;  // largest bucket the one-pass sort finishes by insertion
+
 // Sorts each span of segs entirely in LDS.  The keys (values) are read from
 // ksrc (vsrc) and the sorted span written to keys (vals) at the same offset;
 // the small-segment route sorts in place (ksrc == keys).
+//
+// First range-adaptive passes: with [mn, mx] the span's range of transformed
+// keys, rel(k) = (key(k) - mn) >> shift, shift chosen so rel fits 8 bits
+// (spans up to 512 keys) or 16 bits, is monotone in the key.  One or two
+// stable 8-bit LDS passes order the keys by rel; then each run of equal rel
+// (a few keys for spread-out keys: 4096 keys over 2^16 values) is finished
+// by a stable insertion sort on the whole key, owned by the thread holding
+// the run's first slot.  If a run exceeds kBucketMax keys (clustered keys)
+// the span takes all 8-bit LSD passes in LDS instead.
 template <typename KB, int VB, int IPT, typename XF>
 __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, const typename ValT<VB>::type *vsrc,
                                                                 KB *keys, typename ValT<VB>::type *vals,
@@ -64,6 +75,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
     __shared__ uint64_t s_peer[4][256];
     __shared__ uint32_t s_wsum[4];
     __shared__ uint32_t s_uniform;
+    __shared__ KB s_mn[4], s_mx[4];
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const SegSpan sg = segs[blockIdx.x];
@@ -86,15 +98,21 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
             if (valid(j)) v[j] = vs[pos(j)];
         }
     }
-    for (int i = tid; i < 4 * 256; i += kSegSmallThreads) (&s_peer[0][0])[i] = 0;
+    for (int i = tid; i < 4 * 256; i += kSegSmallThreads) {
+        (&s_peer[0][0])[i] = 0;
+        (&s_cnt[0][0])[i] = 0;
+    }
 
-    for (int p = 0; p < P; ++p) {
+    // one stable LDS counting pass by dig() over the register-held keys:
+    // in-wave ranks, the block's digit offsets, scatter to LDS, read back in
+    // position order (a pass whose digit is the same for every key is skipped)
+    auto lds_pass = [&](auto dig) {
         Ranks<IPT> r;
         for (int i = lane; i < 256; i += 64) s_cnt[w][i] = 0;
         __syncthreads();
-        // every thread read last pass's flag before the barrier above
+        // every thread read the previous pass's flag before the barrier above
         if (tid == 0) s_uniform = 0;
-        rank_in_wave<IPT, false, kRankLdsMask>(k, r, xf, 8 * p, s_cnt[w], s_peer[w], lim);
+        rank_in_wave_by<IPT, false, kRankLdsMask>(k, r, dig, s_cnt[w], s_peer[w], lim);
         __syncthreads();
         const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
         const uint32_t total = c0 + c1 + c2 + c3;
@@ -105,11 +123,11 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
         s_cnt[2][tid] = lstart + c0 + c1;
         s_cnt[3][tid] = lstart + c0 + c1 + c2;
         __syncthreads();
-        if (s_uniform) continue;  // every key has the same digit: identity pass
+        if (s_uniform) return;
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             if (valid(j)) {
-                const uint32_t dst = s_cnt[w][xf.digit(k[j], 8 * p)] + r[j];
+                const uint32_t dst = s_cnt[w][dig(k[j])] + r[j];
                 s_k[dst] = k[j];
                 if constexpr (VB > 0) s_v[dst] = v[j];
             }
@@ -122,7 +140,111 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
                 if constexpr (VB > 0) v[j] = s_v[pos(j)];
             }
         }
+    };
+
+    // ---- range-adaptive passes
+    {
+        KB mn = (KB)~KB(0), mx = KB(0);
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (valid(j)) {
+                const KB t = xf.key(k[j]);
+                mn = t < mn ? t : mn;
+                mx = t > mx ? t : mx;
+            }
+        }
+        {  // wave reductions move 32-bit words: widen narrow keys
+            using WK = std::conditional_t<(sizeof(KB) < 4), uint32_t, KB>;
+            mn = (KB)wave_reduce((WK)mn, (WK)~WK(0), [](WK a, WK b) { return a < b ? a : b; });
+            mx = (KB)wave_reduce((WK)mx, WK(0), [](WK a, WK b) { return a > b ? a : b; });
+        }
+        if (lane == 0) {
+            s_mn[w] = mn;
+            s_mx[w] = mx;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            mn = s_mn[i] < mn ? s_mn[i] : mn;
+            mx = s_mx[i] > mx ? s_mx[i] : mx;
+        }
+        if (mn == mx) {  // one key value: the input order is the sorted order
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (valid(j)) {
+                    kp[pos(j)] = k[j];
+                    if constexpr (VB > 0) vp[pos(j)] = v[j];
+                }
+            }
+            return;
+        }
+        // the leading 8 (spans up to 512 keys) or 16 bits of key - mn, as
+        // one or two 8-bit passes, least significant first
+        const uint32_t bits = len > 512 ? 16u : 8u;
+        const uint32_t bitlen = 64u - (uint32_t)__builtin_clzll((unsigned long long)(KB)(mx - mn));
+        const uint32_t shift = bitlen > bits ? bitlen - bits : 0u;
+        auto rel = [&](KB x) { return (KB)((KB)(xf.key(x) - mn) >> shift); };  // < 2^bits
+        if (bits == 16) lds_pass([&](KB x) { return (uint32_t)rel(x) & 0xFFu; });
+        lds_pass([&](KB x) { return (uint32_t)(rel(x) >> (bits - 8)); });
+        // keys are now ordered by rel(); finish each run of equal rel() with a
+        // stable insertion sort on the whole key (owner: the run's first slot)
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            if (valid(j)) {
+                s_k[pos(j)] = k[j];
+                if constexpr (VB > 0) s_v[pos(j)] = v[j];
+            }
+        }
+        __syncthreads();
+        uint32_t rlen[IPT];
+        bool big = false;
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            rlen[j] = 0;
+            const uint32_t i = pos(j);
+            if (!valid(j)) continue;
+            const KB here = rel(s_k[i]);
+            if (i > 0 && rel(s_k[i - 1]) == here) continue;  // not a run start
+            uint32_t e = i + 1;
+            while (e < len && e - i <= kBucketMax && rel(s_k[e]) == here) ++e;
+            rlen[j] = e - i;
+            big = big || rlen[j] > kBucketMax;
+        }
+        if (!__syncthreads_or(big)) {
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t b0 = pos(j);
+                for (uint32_t i = 1; i < rlen[j]; ++i) {
+                    const KB key = s_k[b0 + i];
+                    const KB tk = xf.key(key);
+                    V val{};
+                    if constexpr (VB > 0) val = s_v[b0 + i];
+                    uint32_t q = i;
+                    while (q > 0 && xf.key(s_k[b0 + q - 1]) > tk) {
+                        s_k[b0 + q] = s_k[b0 + q - 1];
+                        if constexpr (VB > 0) s_v[b0 + q] = s_v[b0 + q - 1];
+                        --q;
+                    }
+                    s_k[b0 + q] = key;
+                    if constexpr (VB > 0) s_v[b0 + q] = val;
+                }
+            }
+            __syncthreads();
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                if (valid(j)) {
+                    kp[pos(j)] = s_k[pos(j)];
+                    if constexpr (VB > 0) vp[pos(j)] = s_v[pos(j)];
+                }
+            }
+            return;
+        }
+        // a run too long to finish by insertion (clustered keys): all LSD
+        // passes follow.  The order so far is a stable sort by a monotone
+        // function of the key, so equal keys are still in input order.
     }
+
+    for (int p = 0; p < P; ++p) lds_pass([&](KB x) { return xf.digit(x, 8 * p); });
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         if (valid(j)) {

@@ -580,6 +580,28 @@ def test_segmented_sort_routes_follow_the_data(queue, dt):
         assert np.array_equal(dv.to_host(queue), ev)
 
 
+@pytest.mark.parametrize("dt", ["uint64", "int32", "float64", "uint16"])
+def test_segmented_sort_in_lds_paths(queue, dt):
+    """Spans sorted in LDS take one range-bucketing pass plus per-bucket
+    insertion sorts when the keys spread over their range, all LSD passes
+    when they cluster (a bucket over 64 keys), and a plain copy when every
+    key is equal; each must match the per-segment sort."""
+    rng = np.random.default_rng(7)
+    spread = random_keys(dt, 3000, rng)
+    clustered = np.repeat(random_keys(dt, 4, rng), 750)
+    clustered[rng.integers(0, 3000, 40)] = random_keys(dt, 40, rng)
+    equal = np.full(2500, spread[0])
+    keys = np.concatenate([spread, clustered, equal, spread[:700], clustered[:100]]).astype(dt)
+    off = _segments([3000, 3000, 2500, 700, 100])
+    v = np.arange(keys.size, dtype=np.uint32)
+    for desc in (False, True):
+        dk, dv = dev(queue, keys), dev(queue, v)
+        ca.segmented_sort(queue, dk, dv, off, descending=desc)
+        ek, ev = O.segmented_sort(keys, v, off, descending=desc)
+        assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), desc
+        assert np.array_equal(dv.to_host(queue), ev), desc
+
+
 def test_segmented_sort_workload_sample(queue):
     """The segmented workload's generator at 2048 segments (lognormal lengths,
     median 256, one 2^20 segment): bit-identical to per-segment sorts."""
