@@ -283,27 +283,54 @@ __global__ void k_seg_insertion(T *keys, typename ValT<VB>::type *vals, const Se
 }
 
 // ------------------------------------------------------------ large path
-// One block per histogram chunk (<= kSegChunk keys of one segment).
+// Route of a large segment, decided on the device from its histogram:
+//   kRouteLsd: all P onesweep passes over the segment (any distribution);
+//   kRouteMsd: one onesweep pass on the most significant digit into the
+//     temporaries, which partitions the segment stably into its 256 digit
+//     buckets; consecutive buckets are packed greedily into groups of at
+//     most `cap` keys, and each group is sorted in LDS by the whole key and
+//     written back (a group is a contiguous range of the sorted segment, so
+//     sorting it by the whole key is exact, and stable).  Taken when every
+//     bucket fits one group and the groups fit the segment's slots.
+enum : uint32_t { kRouteLsd = 0, kRouteMsd = 1 };
+
+// One block per histogram chunk (<= kSegChunk keys of one segment): the
+// histograms of digits [p0, p1).  With route != nullptr only chunks of
+// segments on the LSD route are counted (the others need just the top digit).
 template <typename KB, typename XF>
 __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__restrict__ keys,
                                                                 const SegSpan *__restrict__ chunks, XF xf,
-                                                                uint32_t *__restrict__ partial)
+                                                                uint32_t *__restrict__ partial, uint32_t p0,
+                                                                uint32_t p1, const uint32_t *__restrict__ route)
 {
     constexpr int P = sizeof(KB);
     constexpr int NW = kHistThreads / 64;
     __shared__ uint32_t s_hist[NW][P][256];
+    const SegSpan c = chunks[blockIdx.x];
+    if (route && route[c.idx] != kRouteLsd) return;
     for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
     __syncthreads();
-    const SegSpan c = chunks[blockIdx.x];
     uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
-    for (uint32_t i = threadIdx.x; i < c.len; i += kHistThreads) {
-        KB k = xf.key(keys[c.base + i]);
+    const KB *kc = keys + c.base;
+    constexpr int U = 8;  // loads in flight per thread
+    for (uint32_t i0 = threadIdx.x; i0 < c.len; i0 += kHistThreads * U) {
+        KB kk[U];
 #pragma unroll
-        for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+        for (int u = 0; u < U; ++u) {
+            const uint32_t i = i0 + u * kHistThreads;
+            kk[u] = i < c.len ? kc[i] : KB(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 + u * kHistThreads < c.len) {
+                const KB k = xf.key(kk[u]);
+                for (uint32_t p = p0; p < p1; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+            }
+        }
     }
     __syncthreads();
     uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
-    for (int i = threadIdx.x; i < P * 256; i += kHistThreads) {
+    for (uint32_t i = p0 * 256 + threadIdx.x; i < p1 * 256; i += kHistThreads) {
         uint32_t t = 0;
 #pragma unroll
         for (int w = 0; w < NW; ++w) t += (&s_hist[w][0][0])[i];
@@ -311,14 +338,17 @@ __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__rest
     }
 }
 
-// grid (large segments, passes): digit d of segment s summed over its chunks
-// [chunk_begin[s], chunk_begin[s+1]) and scanned into exclusive offsets.
+// grid (large segments, passes p0 + blockIdx.y): digit d of segment s summed
+// over its chunks [chunk_begin[s], chunk_begin[s+1]) and scanned into
+// exclusive offsets; with route != nullptr, LSD-route segments only.
 __global__ __launch_bounds__(256) void k_seg_digit_offsets(const uint32_t *__restrict__ partial,
                                                            const uint32_t *__restrict__ chunk_begin, uint32_t passes,
-                                                           uint32_t *__restrict__ seg_off)
+                                                           uint32_t p0, uint32_t *__restrict__ seg_off,
+                                                           const uint32_t *__restrict__ route)
 {
     __shared__ uint32_t s_wsum[4];
-    const uint32_t s = blockIdx.x, p = blockIdx.y, d = threadIdx.x;
+    const uint32_t s = blockIdx.x, p = p0 + blockIdx.y, d = threadIdx.x;
+    if (route && route[s] != kRouteLsd) return;
     uint32_t c = 0;
     for (uint32_t b = chunk_begin[s]; b < chunk_begin[s + 1]; ++b) c += partial[((size_t)b * passes + p) * 256 + d];
     seg_off[((size_t)s * passes + p) * 256 + d] = block_exclusive_sum256(c, s_wsum);
@@ -344,17 +374,6 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                              ctrl, status + (size_t)(blockIdx.x - t.tile) * 256,
                                              launch_epoch(ctrl, back));
 }
-
-// Route of a large segment, decided on the device from its histogram:
-//   kRouteLsd: all P onesweep passes over the segment (any distribution);
-//   kRouteMsd: one onesweep pass on the most significant digit into the
-//     temporaries, which partitions the segment stably into its 256 digit
-//     buckets; consecutive buckets are packed greedily into groups of at
-//     most `cap` keys, and each group is sorted in LDS by the whole key and
-//     written back (a group is a contiguous range of the sorted segment, so
-//     sorting it by the whole key is exact, and stable).  Taken when every
-//     bucket fits one group and the groups fit the segment's slots.
-enum : uint32_t { kRouteLsd = 0, kRouteMsd = 1 };
 
 // One block (256 threads, thread = digit) per large segment.
 __global__ __launch_bounds__(256) void k_seg_route(const SegSpan *__restrict__ large,
@@ -497,17 +516,31 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
         }
     }
     if (pl->nlarge) {
+        // the top digit's histogram and offsets decide each segment's route
+        // (MSD needs two digits or more); the other digits are counted only
+        // for segments on the LSD route
         BCMI_LAUNCH(q, "segsort_histogram",
                     hipLaunchKernelGGL((k_seg_histogram<KB, XF>), dim3(pl->nchunks), dim3(kHistThreads), 0, q->stream,
-                                       keys, pl->chunks, xf, pl->partial));
+                                       keys, pl->chunks, xf, pl->partial, (uint32_t)(P - 1), (uint32_t)P,
+                                       (const uint32_t *)nullptr));
         BCMI_LAUNCH(q, "segsort_digit_offsets",
-                    hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, P), dim3(256), 0, q->stream, pl->partial,
-                                       pl->chunk_begin, (uint32_t)P, pl->seg_off));
-        // route each large segment (MSD needs two digits or more)
+                    hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, 1), dim3(256), 0, q->stream, pl->partial,
+                                       pl->chunk_begin, (uint32_t)P, (uint32_t)(P - 1), pl->seg_off,
+                                       (const uint32_t *)nullptr));
         BCMI_LAUNCH(q, "segsort_route",
                     hipLaunchKernelGGL(k_seg_route, dim3(pl->nlarge), dim3(256), 0, q->stream, pl->large, pl->seg_off,
                                        (uint32_t)P, pl->slot_begin, pl->groups, pl->route,
                                        (uint32_t)(kSegSmallThreads << pl->small_cls), (uint32_t)(P >= 2)));
+        if (P >= 2) {
+            BCMI_LAUNCH(q, "segsort_histogram",
+                        hipLaunchKernelGGL((k_seg_histogram<KB, XF>), dim3(pl->nchunks), dim3(kHistThreads), 0,
+                                           q->stream, keys, pl->chunks, xf, pl->partial, 0u, (uint32_t)(P - 1),
+                                           (const uint32_t *)pl->route));
+            BCMI_LAUNCH(q, "segsort_digit_offsets",
+                        hipLaunchKernelGGL(k_seg_digit_offsets, dim3(pl->nlarge, P - 1), dim3(256), 0, q->stream,
+                                           pl->partial, pl->chunk_begin, (uint32_t)P, 0u, pl->seg_off,
+                                           (const uint32_t *)pl->route));
+        }
         uint64_t *status;
         BCMI_TRY(status_array(q, (size_t)pl->ntiles * 256, &status));
         BCMI_TRY(begin_lookback(q, P + 1));
