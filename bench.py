@@ -109,6 +109,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--only", default="all")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-per-call", action="store_true",
+                    help="skip the segmented sort's one-call-per-segment baseline (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
@@ -305,13 +307,16 @@ def main():
                                          "segsort_digit_offsets", "segsort_route", "segsort_onesweep_msd",
                                          "segsort_groups", "segsort_onesweep", "segsort_copyback"],
                                  restore=restore)
-        ms, wall = timed_steps(ca, q, per_call, 1, 1, restore)
-        t_call = float(ms[0])
+        if args.no_per_call:
+            t_call = float("nan")
+        else:
+            ms, wall = timed_steps(ca, q, per_call, 1, 1, restore)
+            t_call = float(ms[0])
         sec["segmented_sort_u64_u32_65536seg"] = {
             "pairs": n, "segments": 65536,
             "batched_graph_ms": round(t_graph, 4), "batched_eager_ms": round(t_eager, 4),
-            "per_call_ms": round(t_call, 2),
-            "speedup_graph_vs_per_call": round(t_call / t_graph, 1),
+            "per_call_ms": round(t_call, 2) if t_call == t_call else None,
+            "speedup_graph_vs_per_call": round(t_call / t_graph, 1) if t_call == t_call else None,
             "kernels_ms_per_call": parts,
             "Gpairs_per_s": round(n / (t_graph * 1e-3) / 1e9, 3),
             "GBps_alg_24B": round(gbs(24 * n, t_graph), 1),
