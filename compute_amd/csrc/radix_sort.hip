@@ -207,9 +207,25 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
     constexpr int IPT = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
         const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
-        const unsigned ipt = tune_get(key, "ipt", 32);
-        const unsigned rank = tune_get(key, "rank", kRankLdsMask);
+        unsigned ipt = tune_get(key, "ipt", 32);
+        unsigned rank = tune_get(key, "rank", kRankLdsMask);
+        // The tuned tile is chosen on 2^27 keys; smaller sorts need more,
+        // smaller tiles to fill the chip.  Measured on MI355X (u32 keys; f32
+        // keys + u32 values): 16 keys per thread is fastest up to 2^19 keys
+        // (2^20 with values), 32 with LDS ranking up to 2^23 (2^25 with
+        // values).  Param "fit_size" = 0 keeps the tuned tile at every size.
+        if (tune_get(key, "fit_size", 1)) {
+            const uint32_t small = VB ? (1u << 20) : (1u << 19), mid = VB ? (1u << 25) : (1u << 23);
+            if (n <= small) {
+                ipt = 16;
+                rank = kRankLdsMask;
+            } else if (n <= mid && (VB || ipt > 32)) {
+                ipt = 32;
+                rank = kRankLdsMask;
+            }
+        }
         if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
+        if (ipt == 16) return radix_sort<KB, VB, 16, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc);
         if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc);

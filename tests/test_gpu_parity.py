@@ -508,6 +508,46 @@ def test_graph_replay_matches_eager(queue):
         queue.finish()
 
 
+# ================================================ tunable radix variants
+def _tune_get(key, param):
+    v = ca.C.c_uint()
+    ca.check(ca.lib().bcmi_tune_get(key.encode(), param.encode(), 0xFFFFFFFF, ca.C.byref(v)))
+    return v.value
+
+
+@pytest.mark.parametrize("rank,ipt", [(0, 16), (0, 24), (0, 32), (1, 24), (0, 40), (0, 48), (0, 64)])
+def test_sort_tunable_tile_variants(queue, rank, ipt):
+    """Every tile size / ranking the tuner may select sorts exactly, keys
+    only and with values, including partial last tiles and descending."""
+    params = ("ipt", "rank", "fit_size")
+    saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
+    L = ca.lib()
+    try:
+        for key in saved:
+            ca.check(L.bcmi_tune_set(key.encode(), b"ipt", ipt))
+            ca.check(L.bcmi_tune_set(key.encode(), b"rank", rank))
+            ca.check(L.bcmi_tune_set(key.encode(), b"fit_size", 0))  # the variant at every size
+        for n in (70001, (1 << 20) + 333):
+            k = random_keys("uint32", n)
+            d = dev(queue, k)
+            ca.sort(queue, d, api=ca.RADIX_SORT)
+            assert np.array_equal(d.to_host(queue), np.sort(k)), n
+            kf = random_keys("float32", n)
+            v = np.arange(n, dtype=np.uint32)
+            for desc in (False, True):
+                dk, dv = dev(queue, kf), dev(queue, v)
+                ca.sort(queue, dk, dv, api=ca.SORT_BY_KEY, descending=desc)
+                ek, ev = O.radix_order_sort(kf, v, descending=desc)
+                assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), (n, desc)
+                assert np.array_equal(dv.to_host(queue), ev), (n, desc)
+    finally:
+        for key, vals in saved.items():
+            for p, val in zip(params, vals):
+                # an absent entry reads back as the sentinel: restore the built-in default
+                ca.check(L.bcmi_tune_set(key.encode(), p.encode(),
+                                         val if val != 0xFFFFFFFF else {"ipt": 32, "rank": 0, "fit_size": 1}[p]))
+
+
 # ========================================================= segmented sort
 def _segments(lens):
     return np.r_[0, np.cumsum(lens)].astype(np.uint64)
