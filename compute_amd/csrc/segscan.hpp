@@ -255,6 +255,232 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
     }
 }
 
+// ---------------------------------------------------------- blocked form
+// For 4-byte keys and values: the tile is loaded wave-striped (coalesced),
+// then transposed through LDS so that lane l of wave w holds E = Q*V
+// consecutive elements (wave positions l*E .. l*E+E-1).  Each lane folds its
+// E elements in order and the wave needs one scan of the folds, instead of
+// one per chunk: for the compensated pairs that is most of the arithmetic.
+// The transposition buffer has one row of E words per lane, padded by 4
+// words so that the 16-byte accesses of both directions spread over banks.
+template <int Q>
+struct SegBlkBuf {
+    static constexpr int E = Q * 4;
+    static constexpr int ROW = E + 4;
+    uint32_t w[4][64 * ROW];
+};
+
+template <typename P, int Q>
+struct SegLdsBlk {
+    Seg<typename P::A> wave[4];
+    Seg<typename P::CA> prefix;
+    SegBlkBuf<Q> t;
+};
+
+// striped chunks of one wave -> this lane's row (all four waves call it)
+template <int Q, typename T>
+BDEV void seg_to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t lane)
+{
+    constexpr int E = Q * 4, ROW = E + 4;
+    static_assert(sizeof(T) == 4, "4-byte elements");
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int p = q * 256 + (int)lane * 4;  // first of the chunk's 4 wave positions
+        uint4 c;
+        c.x = __builtin_bit_cast(uint32_t, s[q][0]);
+        c.y = __builtin_bit_cast(uint32_t, s[q][1]);
+        c.z = __builtin_bit_cast(uint32_t, s[q][2]);
+        c.w = __builtin_bit_cast(uint32_t, s[q][3]);
+        *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e += 4) {
+        const uint4 c = *(const uint4 *)(buf + lane * ROW + e);
+        b[e] = __builtin_bit_cast(T, c.x);
+        b[e + 1] = __builtin_bit_cast(T, c.y);
+        b[e + 2] = __builtin_bit_cast(T, c.z);
+        b[e + 3] = __builtin_bit_cast(T, c.w);
+    }
+    __syncthreads();
+}
+
+// this lane's row -> striped chunks of one wave (all four waves call it)
+template <int Q, typename T>
+BDEV void seg_to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t lane)
+{
+    constexpr int E = Q * 4, ROW = E + 4;
+#pragma unroll
+    for (int e = 0; e < E; e += 4) {
+        uint4 c;
+        c.x = __builtin_bit_cast(uint32_t, b[e]);
+        c.y = __builtin_bit_cast(uint32_t, b[e + 1]);
+        c.z = __builtin_bit_cast(uint32_t, b[e + 2]);
+        c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
+        *(uint4 *)(buf + lane * ROW + e) = c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int p = q * 256 + (int)lane * 4;
+        const uint4 c = *(const uint4 *)(buf + (p / E) * ROW + p % E);
+        s[q][0] = __builtin_bit_cast(T, c.x);
+        s[q][1] = __builtin_bit_cast(T, c.y);
+        s[q][2] = __builtin_bit_cast(T, c.z);
+        s[q][3] = __builtin_bit_cast(T, c.w);
+    }
+    __syncthreads();
+}
+
+template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
+BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
+                           KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
+                           uint64_t *count_out, uint32_t epoch)
+{
+    static_assert(sizeof(KT) == 4 && sizeof(VT) == 4, "blocked form: 4-byte keys and values");
+    using A = typename P::A;
+    using CA = typename P::CA;
+    using S = Seg<A>;
+    using C = Seg<CA>;
+    constexpr int V = 4;
+    constexpr int E = Q * V;  // elements per lane
+    constexpr int WSEG = Q * 64 * V;
+    constexpr int TILE = 4 * WSEG;
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+    const size_t wstart = (size_t)tile * TILE + (size_t)__builtin_amdgcn_readfirstlane(w) * WSEG;
+    const KT *wk = keys + wstart;
+    const VT *wv = vals + wstart;
+    const int64_t rem = FULL ? (int64_t)WSEG : (int64_t)n - (int64_t)wstart;
+    auto sok = [&](int q, int k) { return FULL || (int64_t)(q * 64 * V + lane * V + k) < rem; };
+    auto ok = [&](int e) { return FULL || (int64_t)(lane * E + e) < rem; };
+    uint32_t *buf = L.t.w[w];
+
+    KT kb[E];
+    VT xb[E];
+    {
+        KT kc[Q][V];
+        VT x[Q][V];
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int o = q * 64 * V + lane * V;
+            if (VEC && FULL) {
+                const key_chunk<KT, V> kq = *(const key_chunk<KT, V> *)(wk + o);
+                const key_chunk<VT, V> vq = *(const key_chunk<VT, V> *)(wv + o);
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    kc[q][k] = kq.v[k];
+                    x[q][k] = vq.v[k];
+                }
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    kc[q][k] = sok(q, k) ? wk[o + k] : KT(0);
+                    x[q][k] = sok(q, k) ? wv[o + k] : VT(0);
+                }
+            }
+        }
+        seg_to_blocked<Q>(kc, kb, buf, lane);
+        seg_to_blocked<Q>(x, xb, buf, lane);
+    }
+    // head flags: element i starts a run iff i == 0 or !(key[i-1] == key[i])
+    const KT kfirst = (wstart > 0 && wstart - 1 < n) ? keys[wstart - 1] : KT(0);
+    KT kprev = wave_shr1(kb[E - 1], KT(0));
+    if (lane == 0) kprev = kfirst;
+    uint32_t headm = 0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        const KT before = e == 0 ? kprev : kb[e - 1];
+        if (!key_eq(before, kb[e])) headm |= 1u << e;
+    }
+    if (lane == 0 && wstart == 0) headm |= 1u;
+    // fold of this lane's (head, value) pairs, one wave scan
+    const S sid{P::id(), 0u};
+    auto mrg = [](S a, S b) { return seg_merge<A, P>(a, b); };
+    S t = sid;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (ok(e)) {
+            if (headm >> e & 1u) t = S{P::add(P::id(), xb[e]), t.n + 1};
+            else t.v = P::add(t.v, xb[e]);
+        }
+    }
+    const S inc = wave_inclusive_scan(t, sid, mrg);
+    const S pre = wave_shr1(inc, sid);
+    if (lane == 63) L.wave[w] = inc;
+    __syncthreads();
+    S wpre = sid;
+    for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, L.wave[i]);
+    if (w == 0) {
+        const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
+        const C agg{P::carry(total.v), total.n};
+        const C cid{P::cid(), 0u};
+        const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
+            return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
+        }, ctrl);
+        if (lane == 0) L.prefix = prefix;
+    }
+    __syncthreads();
+    const C prefix = L.prefix;
+    S s = mrg(wpre, pre);  // exclusive local pair before this lane's first element
+    VT y[E];
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (!ok(e)) continue;
+        const size_t i = wstart + (size_t)lane * E + e;
+        const bool head = headm >> e & 1u;
+        if constexpr (MODE == SEG_RBK) {
+            if (head && i > 0) {
+                // finished run i-1 = the exclusive pair at i
+                const uint32_t nb = prefix.n + s.n;
+                if (nb >= 1 && nb <= n) {
+                    keys_out[nb - 1] = e == 0 ? kprev : kb[e - 1];
+                    out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
+                } else {
+                    __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+            }
+        } else if constexpr (MODE == SEG_EXCL) {
+            y[e] = head ? init : P::fin_init(init, prefix.v, s.v, s.n == 0);
+        }
+        if (head) s = S{P::add(P::id(), xb[e]), s.n + 1};
+        else s.v = P::add(s.v, xb[e]);
+        if constexpr (MODE == SEG_INCL) y[e] = P::fin(prefix.v, s.v, s.n == 0);
+        if constexpr (MODE == SEG_RBK) {
+            if (i == n - 1) {
+                const uint32_t nb = prefix.n + s.n;
+                if (nb >= 1 && nb <= n) {
+                    keys_out[nb - 1] = kb[e];
+                    out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
+                } else {
+                    __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                }
+                *count_out = nb;
+            }
+        }
+    }
+    if constexpr (MODE != SEG_RBK) {
+        VT ys[Q][V];
+        seg_to_striped<Q>(y, ys, buf, lane);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            const int o = q * 64 * V + lane * V;
+            VT *dst = out + wstart + o;
+            if (VEC && FULL) {
+                key_chunk<VT, V> c;
+#pragma unroll
+                for (int k = 0; k < V; ++k) c.v[k] = ys[q][k];
+                *(key_chunk<VT, V> *)dst = c;
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k)
+                    if (sok(q, k)) dst[k] = ys[q][k];
+            }
+        }
+    }
+}
+
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC>
 __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
                                                          KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
@@ -262,14 +488,25 @@ __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ 
                                                          uint32_t back)
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
-    __shared__ SegLds<P> lds;
     const uint32_t epoch = launch_epoch(ctrl, back);
-    if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                    count_out, epoch);
-    else
-        segscan_body<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                     count_out, epoch);
+    const bool full = (size_t)(blockIdx.x + 1) * TILE <= n;
+    if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4 && Q <= 8) {
+        __shared__ SegLdsBlk<P, Q> lds;
+        if (full)
+            segscan_body_blk<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                            count_out, epoch);
+        else
+            segscan_body_blk<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                             count_out, epoch);
+    } else {
+        __shared__ SegLds<P> lds;
+        if (full)
+            segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                        count_out, epoch);
+        else
+            segscan_body<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
+                                                         count_out, epoch);
+    }
 }
 
 template <typename KT, typename VT, typename P, int MODE, int Q>
