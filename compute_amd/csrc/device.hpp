@@ -181,6 +181,75 @@ BDEV double cf_value(cf32 a) { return (double)a.s + (double)a.c; }
 
 namespace bcmi {
 
+// ------------------------------------------- striped <-> blocked transposes
+// A wave-striped tile (chunk q of lane l = wave positions q*256 + l*4 .. +3,
+// loaded and stored coalesced) transposed through LDS so that lane l holds
+// E = Q*4 consecutive positions l*E .. l*E+E-1, and back.  The buffer has
+// one row of E words per lane, padded by 4 words so that the 16-byte
+// accesses of both directions spread over the banks.  All four waves of the
+// block call these together (they contain block barriers).
+template <int Q>
+struct BlkBuf {
+    static constexpr int E = Q * 4;
+    static constexpr int ROW = E + 4;
+    uint32_t w[4][64 * ROW];
+};
+
+// striped chunks of one wave -> this lane's row
+template <int Q, typename T>
+BDEV void to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t lane)
+{
+    constexpr int E = Q * 4, ROW = E + 4;
+    static_assert(sizeof(T) == 4, "4-byte elements");
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int p = q * 256 + (int)lane * 4;  // first of the chunk's 4 wave positions
+        uint4 c;
+        c.x = __builtin_bit_cast(uint32_t, s[q][0]);
+        c.y = __builtin_bit_cast(uint32_t, s[q][1]);
+        c.z = __builtin_bit_cast(uint32_t, s[q][2]);
+        c.w = __builtin_bit_cast(uint32_t, s[q][3]);
+        *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int e = 0; e < E; e += 4) {
+        const uint4 c = *(const uint4 *)(buf + lane * ROW + e);
+        b[e] = __builtin_bit_cast(T, c.x);
+        b[e + 1] = __builtin_bit_cast(T, c.y);
+        b[e + 2] = __builtin_bit_cast(T, c.z);
+        b[e + 3] = __builtin_bit_cast(T, c.w);
+    }
+    __syncthreads();
+}
+
+// this lane's row -> striped chunks of one wave
+template <int Q, typename T>
+BDEV void to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t lane)
+{
+    constexpr int E = Q * 4, ROW = E + 4;
+#pragma unroll
+    for (int e = 0; e < E; e += 4) {
+        uint4 c;
+        c.x = __builtin_bit_cast(uint32_t, b[e]);
+        c.y = __builtin_bit_cast(uint32_t, b[e + 1]);
+        c.z = __builtin_bit_cast(uint32_t, b[e + 2]);
+        c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
+        *(uint4 *)(buf + lane * ROW + e) = c;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int q = 0; q < Q; ++q) {
+        const int p = q * 256 + (int)lane * 4;
+        const uint4 c = *(const uint4 *)(buf + (p / E) * ROW + p % E);
+        s[q][0] = __builtin_bit_cast(T, c.x);
+        s[q][1] = __builtin_bit_cast(T, c.y);
+        s[q][2] = __builtin_bit_cast(T, c.z);
+        s[q][3] = __builtin_bit_cast(T, c.w);
+    }
+    __syncthreads();
+}
+
 // ------------------------------------------- wave-parallel decoupled look-back
 // A carry of 4*N bytes occupies N consecutive status words per tile.
 template <typename Cr>

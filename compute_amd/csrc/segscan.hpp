@@ -256,81 +256,16 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
 }
 
 // ---------------------------------------------------------- blocked form
-// For 4-byte keys and values: the tile is loaded wave-striped (coalesced),
-// then transposed through LDS so that lane l of wave w holds E = Q*V
-// consecutive elements (wave positions l*E .. l*E+E-1).  Each lane folds its
-// E elements in order and the wave needs one scan of the folds, instead of
-// one per chunk: for the compensated pairs that is most of the arithmetic.
-// The transposition buffer has one row of E words per lane, padded by 4
-// words so that the 16-byte accesses of both directions spread over banks.
-template <int Q>
-struct SegBlkBuf {
-    static constexpr int E = Q * 4;
-    static constexpr int ROW = E + 4;
-    uint32_t w[4][64 * ROW];
-};
-
+// For 4-byte keys and values the tile is transposed through LDS (to_blocked
+// in device.hpp) so that each lane folds E = Q*4 consecutive elements and the
+// wave needs one scan of the folds instead of one per chunk: for the
+// compensated pairs that is most of the arithmetic.
 template <typename P, int Q>
 struct SegLdsBlk {
     Seg<typename P::A> wave[4];
     Seg<typename P::CA> prefix;
-    SegBlkBuf<Q> t;
+    BlkBuf<Q> t;
 };
-
-// striped chunks of one wave -> this lane's row (all four waves call it)
-template <int Q, typename T>
-BDEV void seg_to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t lane)
-{
-    constexpr int E = Q * 4, ROW = E + 4;
-    static_assert(sizeof(T) == 4, "4-byte elements");
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int p = q * 256 + (int)lane * 4;  // first of the chunk's 4 wave positions
-        uint4 c;
-        c.x = __builtin_bit_cast(uint32_t, s[q][0]);
-        c.y = __builtin_bit_cast(uint32_t, s[q][1]);
-        c.z = __builtin_bit_cast(uint32_t, s[q][2]);
-        c.w = __builtin_bit_cast(uint32_t, s[q][3]);
-        *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int e = 0; e < E; e += 4) {
-        const uint4 c = *(const uint4 *)(buf + lane * ROW + e);
-        b[e] = __builtin_bit_cast(T, c.x);
-        b[e + 1] = __builtin_bit_cast(T, c.y);
-        b[e + 2] = __builtin_bit_cast(T, c.z);
-        b[e + 3] = __builtin_bit_cast(T, c.w);
-    }
-    __syncthreads();
-}
-
-// this lane's row -> striped chunks of one wave (all four waves call it)
-template <int Q, typename T>
-BDEV void seg_to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t lane)
-{
-    constexpr int E = Q * 4, ROW = E + 4;
-#pragma unroll
-    for (int e = 0; e < E; e += 4) {
-        uint4 c;
-        c.x = __builtin_bit_cast(uint32_t, b[e]);
-        c.y = __builtin_bit_cast(uint32_t, b[e + 1]);
-        c.z = __builtin_bit_cast(uint32_t, b[e + 2]);
-        c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
-        *(uint4 *)(buf + lane * ROW + e) = c;
-    }
-    __syncthreads();
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const int p = q * 256 + (int)lane * 4;
-        const uint4 c = *(const uint4 *)(buf + (p / E) * ROW + p % E);
-        s[q][0] = __builtin_bit_cast(T, c.x);
-        s[q][1] = __builtin_bit_cast(T, c.y);
-        s[q][2] = __builtin_bit_cast(T, c.z);
-        s[q][3] = __builtin_bit_cast(T, c.w);
-    }
-    __syncthreads();
-}
 
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
 BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
@@ -381,8 +316,8 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
                 }
             }
         }
-        seg_to_blocked<Q>(kc, kb, buf, lane);
-        seg_to_blocked<Q>(x, xb, buf, lane);
+        to_blocked<Q>(kc, kb, buf, lane);
+        to_blocked<Q>(x, xb, buf, lane);
     }
     // head flags: element i starts a run iff i == 0 or !(key[i-1] == key[i])
     const KT kfirst = (wstart > 0 && wstart - 1 < n) ? keys[wstart - 1] : KT(0);
@@ -462,7 +397,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     }
     if constexpr (MODE != SEG_RBK) {
         VT ys[Q][V];
-        seg_to_striped<Q>(y, ys, buf, lane);
+        to_striped<Q>(y, ys, buf, lane);
 #pragma unroll
         for (int q = 0; q < Q; ++q) {
             const int o = q * 64 * V + lane * V;
