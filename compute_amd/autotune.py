@@ -16,6 +16,7 @@ threads by the kernels' LDS layout; what varies is the work per thread):
   __bcmi_scan|q                        4-byte scans, items per lane
   __bcmi_reduce_by_key|q               reduce_by_key, items per lane
   __bcmi_segscan|q                     segmented scans, items per lane
+  __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
 
     python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
 """
@@ -129,6 +130,37 @@ def tune_scan(ca, W, q, log2n, log):
     _set(ca, "__bcmi_scan", "q", best[1])
 
 
+REDUCE_CANDIDATES = [(2, 4), (4, 4), (8, 4), (2, 8), (4, 8), (8, 8)]  # (unroll, blocks per CU)
+
+
+def tune_reduce(ca, W, q, log2n, log):
+    n = 1 << log2n
+    x = W.lognormal_f32(n)
+    x64 = x.astype(np.float64)
+    ref = float(np.sum(x64))
+    mag = float(np.sum(np.abs(x64)))
+    dx = ca.DeviceArray.from_host(x, q)
+    out = ca.DeviceArray(1, np.float32, q)
+    f64 = ca.DeviceArray(1, np.float64, q)
+    L = ca.lib()
+    f64p = ca.C.cast(f64.ptr, ca.C.POINTER(ca.C.c_double))
+    best = None
+    for unroll, bpc in REDUCE_CANDIDATES:
+        _set(ca, "__bcmi_reduce", "unroll", unroll)
+        _set(ca, "__bcmi_reduce", "blocks_per_cu", bpc)
+        ms = _median_ms(ca, q, lambda: ca.check(L.bcmi_reduce(q.h, ca.F32, ca.PLUS, dx.ptr, n, None, out.ptr,
+                                                                f64p, ca.COMPENSATED)))
+        got = float(f64.to_host(q)[0])
+        ok = abs(got - ref) <= 2.0 ** -40 * mag
+        log({"tunable": "__bcmi_reduce", "unroll": unroll, "blocks_per_cu": bpc, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, unroll, bpc)
+    if best is None:
+        raise RuntimeError("autotune: no reduce candidate matched")
+    _set(ca, "__bcmi_reduce", "unroll", best[1])
+    _set(ca, "__bcmi_reduce", "blocks_per_cu", best[2])
+
+
 def tune_segscan(ca, W, q, log2n, log):
     n = 1 << log2n
     L = ca.lib()
@@ -193,7 +225,7 @@ def main(argv=None):
     ap.add_argument("--log2n-sort", type=int, default=27)
     ap.add_argument("--log2n-scan", type=int, default=28)
     ap.add_argument("--log2n-segscan", type=int, default=27)
-    ap.add_argument("--only", default="sort,scan,segscan")
+    ap.add_argument("--only", default="sort,scan,segscan,reduce")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(HERE))
     import compute_amd as ca
@@ -208,6 +240,8 @@ def main(argv=None):
         tune_scan(ca, W, q, args.log2n_scan, log)
     if "segscan" in only:
         tune_segscan(ca, W, q, args.log2n_segscan, log)
+    if "reduce" in only:
+        tune_reduce(ca, W, q, args.log2n_scan, log)
     q.finish()
     os.makedirs(os.path.dirname(os.path.abspath(args.out)), exist_ok=True)
     ca.check(ca.lib().bcmi_tuning_cache_save(args.out.encode()))

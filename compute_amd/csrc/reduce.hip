@@ -124,7 +124,8 @@ BDEV S block_combine(S v, S *s_buf)
     }
 }
 
-template <typename T, typename P>
+// U chunks of 16 bytes in flight per thread and iteration.
+template <typename T, typename P, int U>
 __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in, size_t n, size_t head,
                                                         size_t nchunks, typename P::Part *__restrict__ partials)
 {
@@ -141,16 +142,17 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in
     const chunk16<T> *body = (const chunk16<T> *)(in + head);
     const size_t stride = (size_t)gridDim.x * kRedThreads;
     size_t c = (size_t)blockIdx.x * kRedThreads + threadIdx.x;
-    // two chunks in flight per iteration
-    for (; c + stride < nchunks; c += 2 * stride) {
-        chunk16<T> a = body[c], b = body[c + stride];
+    for (; c + (U - 1) * stride < nchunks; c += U * stride) {
+        chunk16<T> a[U];
 #pragma unroll
-        for (int k = 0; k < chunk16<T>::N; ++k) acc = P::add(acc, a.v[k]);
+        for (int u = 0; u < U; ++u) a[u] = body[c + u * stride];
 #pragma unroll
-        for (int k = 0; k < chunk16<T>::N; ++k) acc = P::add(acc, b.v[k]);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < chunk16<T>::N; ++k) acc = P::add(acc, a[u].v[k]);
     }
-    if (c < nchunks) {
-        chunk16<T> a = body[c];
+    for (; c < nchunks; c += stride) {
+        const chunk16<T> a = body[c];
 #pragma unroll
         for (int k = 0; k < chunk16<T>::N; ++k) acc = P::add(acc, a.v[k]);
     }
@@ -194,16 +196,26 @@ static int run_reduce(bcmi_queue q, const void *in_, size_t n, const void *init,
     }
     if (head > n) head = n;
     size_t nchunks = (n - head) / chunk16<T>::N;
+    // chunks in flight per thread ("unroll": 2, 4 or 8) and workgroups per
+    // CU ("blocks_per_cu") come from the tuning cache (key "__bcmi_reduce")
+    const unsigned unroll = tune_get("__bcmi_reduce", "unroll", 2);
+    const unsigned bpc = tune_get("__bcmi_reduce", "blocks_per_cu", 4);
     size_t want = (nchunks + kRedThreads * 8 - 1) / (kRedThreads * 8);
-    size_t maxb = (size_t)q->cu_count * 4;
+    size_t maxb = (size_t)q->cu_count * (bpc ? bpc : 4);
     size_t blocks = want < 1 ? 1 : (want > maxb ? maxb : want);
     void *scr;
     BCMI_TRY(scratch(q, blocks * sizeof(typename P::Part), &scr));
     T ini{};
     if (init) memcpy(&ini, init, sizeof(T));
-    BCMI_LAUNCH(q, "reduce",
-                hipLaunchKernelGGL((k_reduce<T, P>), dim3((unsigned)blocks), dim3(kRedThreads), 0, q->stream, in, n,
-                                   head, nchunks, (typename P::Part *)scr));
+    auto go = [&](auto u) {
+        BCMI_LAUNCH(q, "reduce",
+                    hipLaunchKernelGGL((k_reduce<T, P, decltype(u)::value>), dim3((unsigned)blocks), dim3(kRedThreads),
+                                       0, q->stream, in, n, head, nchunks, (typename P::Part *)scr));
+        return 0;
+    };
+    if (unroll >= 8) BCMI_TRY(go(std::integral_constant<int, 8>()));
+    else if (unroll >= 4) BCMI_TRY(go(std::integral_constant<int, 4>()));
+    else BCMI_TRY(go(std::integral_constant<int, 2>()));
     BCMI_LAUNCH(q, "reduce_final",
                 hipLaunchKernelGGL((k_reduce_final<T, P>), dim3(1), dim3(kRedThreads), 0, q->stream,
                                    (const typename P::Part *)scr, (uint32_t)blocks, ini, init ? 1 : 0, (T *)out,
