@@ -32,7 +32,7 @@ DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
 SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 24)]  # (rank, ipt)
 SORT_KEYS_ONLY_EXTRA = [(0, 40), (0, 48), (0, 64)]
-SORT_PAIRS_EXTRA = [(0, 40), (0, 48)]
+SORT_PAIRS_EXTRA = [(0, 40), (0, 48), (0, 32, 1), (0, 40, 1), (0, 48, 1)]  # (rank, ipt, last_ballot)
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 
@@ -87,9 +87,12 @@ def tune_sort(ca, W, q, log2n, log):
 
     first = None
     best = None
-    for rank, ipt in SORT_CANDIDATES + SORT_PAIRS_EXTRA:
+    for cand in SORT_CANDIDATES + SORT_PAIRS_EXTRA:
+        rank, ipt = cand[:2]
+        lb = cand[2] if len(cand) > 2 else 0
         _set(ca, "__bcmi_radix_sort_by_key", "rank", rank)
         _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
+        _set(ca, "__bcmi_radix_sort_by_key", "last_ballot", lb)
         ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)
         got = (wk.to_host(q).view(np.uint32), wv.to_host(q))
         if first is None:
@@ -100,13 +103,15 @@ def tune_sort(ca, W, q, log2n, log):
             first = got
         else:
             ok = bool(np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]))
-        log({"tunable": "__bcmi_radix_sort_by_key", "rank": rank, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_radix_sort_by_key", "rank": rank, "ipt": ipt, "last_ballot": lb, "ms": round(ms, 4),
+             "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, rank, ipt)
+            best = (ms, rank, ipt, lb)
     if best is None:
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
     _set(ca, "__bcmi_radix_sort_by_key", "rank", best[1])
     _set(ca, "__bcmi_radix_sort_by_key", "ipt", best[2])
+    _set(ca, "__bcmi_radix_sort_by_key", "last_ballot", best[3])
 
 
 def tune_scan(ca, W, q, log2n, log):

@@ -135,7 +135,10 @@ __global__ void k_insertion(T *keys, typename ValT<VB>::type *vals, uint32_t n, 
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
-template <typename KB, int VB, int IPT, int RANK, typename XF>
+// LASTB: the last (most significant) pass ranks by ballots on 24-key tiles.
+// Float keys of a narrow range share few top-digit values, and many lanes
+// with one digit serialise the LDS-mask ranking's same-address atomics.
+template <typename KB, int VB, int IPT, int RANK, typename XF, bool LASTB = false>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf)
 {
     using V = typename ValT<VB>::type;
@@ -159,8 +162,10 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     KB *ktmp = (KB *)scr;
     V *vtmp = (V *)((char *)scr + kbytes);
     uint32_t *partial = (uint32_t *)((char *)scr + kbytes + vbytes);
+    constexpr uint32_t TILEB = kSortThreads * 24;
+    const uint32_t tilesb = LASTB ? (n + TILEB - 1) / TILEB : 0;
     uint64_t *status;
-    BCMI_TRY(status_array(q, (size_t)tiles * 256, &status));
+    BCMI_TRY(status_array(q, (size_t)(tiles > tilesb ? tiles : tilesb) * 256, &status));
 
     BCMI_LAUNCH(q, "radix_histogram",
                 hipLaunchKernelGGL((k_histogram<KB, XF>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
@@ -172,10 +177,16 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     KB *ks = keys, *kd = ktmp;
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
-        BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK, XF>), dim3(tiles), dim3(kSortThreads), 0, q->stream, ks,
-                                       kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
-                                       (uint32_t)(P - p)));
+        if (LASTB && p == P - 1)
+            BCMI_LAUNCH(q, "radix_onesweep",
+                        hipLaunchKernelGGL((k_onesweep<KB, VB, 24, kRankBallot, XF>), dim3(tilesb), dim3(kSortThreads),
+                                           0, q->stream, ks, kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf,
+                                           q->ctrl, status, (uint32_t)(P - p)));
+        else
+            BCMI_LAUNCH(q, "radix_onesweep",
+                        hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK, XF>), dim3(tiles), dim3(kSortThreads), 0,
+                                           q->stream, ks, kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl,
+                                           status, (uint32_t)(P - p)));
         std::swap(ks, kd);
         std::swap(vs, vd);
     }
@@ -188,10 +199,15 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
 }
 
 template <typename KB, int VB, int IPT, int RANK>
-static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending)
+static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
+                      bool last_ballot = false)
 {
-    return with_key_xf<KB>(kind, descending,
-                           [&](auto xf) { return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf); });
+    return with_key_xf<KB>(kind, descending, [&](auto xf) {
+        if constexpr (VB == 4 && RANK == kRankLdsMask) {
+            if (last_ballot) return radix_sort_xf<KB, VB, IPT, RANK, decltype(xf), true>(q, keys, vals, n, xf);
+        }
+        return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf);
+    });
 }
 
 // Tile size (items per thread) and ranking method come from the tuning cache
@@ -214,7 +230,8 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
         // keys + u32 values): 16 keys per thread is fastest up to 2^19 keys
         // (2^20 with values), 32 with LDS ranking up to 2^23 (2^25 with
         // values).  Param "fit_size" = 0 keeps the tuned tile at every size.
-        if (tune_get(key, "fit_size", 1)) {
+        const bool fit = tune_get(key, "fit_size", 1) != 0;
+        if (fit) {
             const uint32_t small = VB ? (1u << 20) : (1u << 19), mid = VB ? (1u << 25) : (1u << 23);
             if (n <= small) {
                 ipt = 16;
@@ -224,12 +241,15 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
                 rank = kRankLdsMask;
             }
         }
+        // pairs: param "last_ballot" = 1 ranks the top digit by ballots
+        // (chosen on 2^27 pairs; with fit_size, only above 2^25)
+        const bool lb = VB && rank == kRankLdsMask && tune_get(key, "last_ballot", 0) && (!fit || n > (1u << 25));
         if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
         if (ipt == 16) return radix_sort<KB, VB, 16, kRankLdsMask>(q, keys, vals, n, kind, desc);
-        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc);
-        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc);
-        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc);
-        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc);
+        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
+        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
+        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
+        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
         if constexpr (VB == 0) {
             if (ipt == 64) return radix_sort<KB, VB, 64, kRankLdsMask>(q, keys, vals, n, kind, desc);
         }

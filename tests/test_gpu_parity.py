@@ -515,11 +515,12 @@ def _tune_get(key, param):
     return v.value
 
 
-@pytest.mark.parametrize("rank,ipt", [(0, 16), (0, 24), (0, 32), (1, 24), (0, 40), (0, 48), (0, 64)])
-def test_sort_tunable_tile_variants(queue, rank, ipt):
+@pytest.mark.parametrize("rank,ipt,lb", [(0, 16, 0), (0, 24, 0), (0, 32, 0), (1, 24, 0), (0, 40, 0), (0, 48, 0),
+                                         (0, 64, 0), (0, 40, 1)])
+def test_sort_tunable_tile_variants(queue, rank, ipt, lb):
     """Every tile size / ranking the tuner may select sorts exactly, keys
     only and with values, including partial last tiles and descending."""
-    params = ("ipt", "rank", "fit_size")
+    params = ("ipt", "rank", "fit_size", "last_ballot")
     saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
     L = ca.lib()
     try:
@@ -527,6 +528,7 @@ def test_sort_tunable_tile_variants(queue, rank, ipt):
             ca.check(L.bcmi_tune_set(key.encode(), b"ipt", ipt))
             ca.check(L.bcmi_tune_set(key.encode(), b"rank", rank))
             ca.check(L.bcmi_tune_set(key.encode(), b"fit_size", 0))  # the variant at every size
+            ca.check(L.bcmi_tune_set(key.encode(), b"last_ballot", lb))
         for n in (70001, (1 << 20) + 333):
             k = random_keys("uint32", n)
             d = dev(queue, k)
@@ -545,7 +547,8 @@ def test_sort_tunable_tile_variants(queue, rank, ipt):
             for p, val in zip(params, vals):
                 # an absent entry reads back as the sentinel: restore the built-in default
                 ca.check(L.bcmi_tune_set(key.encode(), p.encode(),
-                                         val if val != 0xFFFFFFFF else {"ipt": 32, "rank": 0, "fit_size": 1}[p]))
+                                         val if val != 0xFFFFFFFF else
+                                         {"ipt": 32, "rank": 0, "fit_size": 1, "last_ballot": 0}[p]))
 
 
 # ========================================================= segmented sort
