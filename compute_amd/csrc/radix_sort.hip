@@ -34,18 +34,20 @@ struct alignas(16) kchunk {
     KB v[N];
 };
 
-// Each wave counts into its own LDS sub-histogram (fewer same-address LDS
-// atomics); keys are read as 16-byte chunks, two chunks in flight.
+// Each half-wave counts into its own LDS sub-histogram: lanes adding to one
+// bin in the same instruction serialise, which matters for skewed digits
+// (the few top-digit values of floats in a narrow range); keys are read as
+// 16-byte chunks, two chunks in flight.
 template <typename KB, typename XF>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
                                                             XF xf, uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
-    constexpr int NW = kHistThreads / 64;
+    constexpr int NW = kHistThreads / 32;  // sub-histograms: one per half-wave
     __shared__ uint32_t s_hist[NW][P][256];
     for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
     __syncthreads();
-    uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
+    uint32_t(*h)[256] = s_hist[threadIdx.x >> 5];
     auto count = [&](KB x) {
         KB k = xf.key(x);
 #pragma unroll

@@ -227,6 +227,8 @@ int bcmi_device_get_info(int device, bcmi_device_info *info)
     BCMI_HIP(hipGetDeviceProperties(&p, device));
     memset(info, 0, sizeof(*info));
     snprintf(info->name, sizeof(info->name), "%s", p.name);
+    if (!info->name[0])  // some drivers report no marketing name
+        snprintf(info->name, sizeof(info->name), "%.16s, %d CUs", p.gcnArchName, p.multiProcessorCount);
     snprintf(info->arch, sizeof(info->arch), "%s", p.gcnArchName);
     info->compute_units = p.multiProcessorCount;
     info->wavefront_size = p.warpSize;
