@@ -40,7 +40,30 @@ EXPORTS = [
     "bcmi_segmented_scan", "bcmi_graph_begin", "bcmi_graph_end", "bcmi_graph_launch",
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
+    "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition",
 ]
+
+SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
+FN = {"id": 0, "mul": 1, "mod": 2, "and": 3, "signbit": 4}
+CMP = {"<": 0, "<=": 1, ">": 2, ">=": 3, "==": 4, "!=": 5}
+
+
+class Predicate(C.Structure):
+    """bcmi_predicate: f(_1) cmp c, optionally negated (not1)."""
+    _fields_ = [("fn", C.c_int32), ("cmp", C.c_int32), ("negate", C.c_int32), ("reserved", C.c_int32),
+                ("operand", C.c_uint64), ("value", C.c_uint64)]
+
+
+def predicate(dtype, cmp, c, fn="id", a=0, negate=False):
+    """Build a predicate over elements of `dtype`.  a and c are converted to
+    the comparison type: int32 for 8/16-bit integers and for signbit (C's
+    integer promotion / OpenCL's int result), else the element type."""
+    dt = np.dtype(dtype)
+    ct = np.dtype(np.int32) if (fn == "signbit" or (dt.kind in "iu" and dt.itemsize < 4)) else dt
+    def bits(v):
+        raw = np.array([v]).astype(ct).tobytes()
+        return int.from_bytes(raw.ljust(8, b"\0"), "little")
+    return Predicate(FN[fn], CMP[cmp], int(bool(negate)), 0, bits(a), bits(c))
 
 
 class BcmiError(RuntimeError):
@@ -112,6 +135,11 @@ def lib():
         L.bcmi_profile_enable.argtypes = [vp, i]
         L.bcmi_profile_query.argtypes = [vp, C.c_char_p, C.POINTER(C.c_int), C.POINTER(C.c_double)]
         L.bcmi_profile_reset.argtypes = [vp]
+        pp = C.POINTER(Predicate)
+        L.bcmi_copy_if.argtypes = [vp, i, vp, vp, sz, pp, i, vp]
+        L.bcmi_count_if.argtypes = [vp, i, vp, sz, pp, vp]
+        L.bcmi_unique.argtypes = [vp, i, vp, vp, sz, vp]
+        L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
         _lib = L
     return _lib
 
@@ -307,6 +335,44 @@ def transform(q, a, b, out, op):
 
 def saxpy(q, a, x, y, out):
     check(lib().bcmi_saxpy(q.h, float(a), x.ptr, y.ptr, out.ptr, x.size))
+
+
+# ------------------------------------------------------------- selection
+def _count_buf(q, count):
+    return count if count is not None else DeviceArray(1, np.uint64, q)
+
+
+def copy_if(q, inp, out, pred, out_kind=SELECT_VALUES, n=None, count=None, in_ptr=None, out_ptr=None):
+    """Selected elements (or their indices) of inp into out; returns the
+    count (blocking read), or the device count buffer when count is given."""
+    n = inp.size if n is None else n
+    cnt = _count_buf(q, count)
+    check(lib().bcmi_copy_if(q.h, DTYPE[inp.dtype], in_ptr or _p(inp), out_ptr or _p(out), n,
+                             C.byref(pred), out_kind, cnt.ptr))
+    return cnt if count is not None else int(cnt.to_host(q)[0])
+
+
+def count_if(q, inp, pred, n=None, in_ptr=None):
+    n = inp.size if n is None else n
+    cnt = DeviceArray(1, np.uint64, q)
+    check(lib().bcmi_count_if(q.h, DTYPE[inp.dtype], in_ptr or _p(inp), n, C.byref(pred), cnt.ptr))
+    return int(cnt.to_host(q)[0])
+
+
+def unique(q, inp, out, n=None, count=None, in_ptr=None, out_ptr=None):
+    n = inp.size if n is None else n
+    cnt = _count_buf(q, count)
+    check(lib().bcmi_unique(q.h, DTYPE[inp.dtype], in_ptr or _p(inp), out_ptr or _p(out), n, cnt.ptr))
+    return cnt if count is not None else int(cnt.to_host(q)[0])
+
+
+def partition(q, inp, out_true, out_false, pred, n=None, count=None):
+    """partition_copy (out_false given) or stable_partition into out_true."""
+    n = inp.size if n is None else n
+    cnt = _count_buf(q, count)
+    check(lib().bcmi_partition(q.h, DTYPE[inp.dtype], _p(inp), _p(out_true), _p(out_false), n,
+                               C.byref(pred), cnt.ptr))
+    return cnt if count is not None else int(cnt.to_host(q)[0])
 
 
 def segmented_sort(q, keys, values, offsets_host, descending=False):

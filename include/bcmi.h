@@ -210,6 +210,74 @@ int bcmi_segmented_scan(bcmi_queue queue, int key_dtype, int value_dtype,
                         size_t n, const void *init, int exclusive,
                         int precision);
 
+/* ----------------------------------------------- selection (compaction) */
+/* The unary predicates the dispatch recognises.  The reference prints a
+ * lambda expression as OpenCL C (lambda/context.hpp:145-156); this build
+ * has no run-time compiler and recognises the forms
+ *     f(_1) cmp c        with f = _1, _1 * a, _1 % a, _1 & a or signbit(_1)
+ * (signbit_: functional/relational.hpp:32), cmp one of < <= > >= == !=,
+ * optionally negated (not1: functional/logical.hpp:113-175).  a and c hold
+ * one value of the comparison type in their low bytes: int32 for 8- and
+ * 16-bit integers (C's integer promotion) and for signbit, else the element
+ * type.  % and & need an integer type, signbit a float type; % by 0 is
+ * rejected. */
+enum bcmi_pred_fn {
+    BCMI_FN_ID = 0, BCMI_FN_MUL = 1, BCMI_FN_MOD = 2, BCMI_FN_AND = 3,
+    BCMI_FN_SIGNBIT = 4
+};
+enum bcmi_cmp {
+    BCMI_CMP_LT = 0, BCMI_CMP_LE = 1, BCMI_CMP_GT = 2, BCMI_CMP_GE = 3,
+    BCMI_CMP_EQ = 4, BCMI_CMP_NE = 5
+};
+typedef struct bcmi_predicate {
+    int32_t fn;       /* bcmi_pred_fn */
+    int32_t cmp;      /* bcmi_cmp */
+    int32_t negate;   /* != 0: not1(predicate) */
+    int32_t reserved; /* 0 */
+    uint64_t operand; /* a */
+    uint64_t value;   /* c */
+} bcmi_predicate;
+
+/* what bcmi_copy_if writes for each selected element */
+enum bcmi_select_out {
+    BCMI_SELECT_VALUES = 0,    /* copy_if (copy_if.hpp:46-62) */
+    BCMI_SELECT_INDEX_U32 = 1, /* detail::copy_index_if (copy_if.hpp:25-41) */
+    BCMI_SELECT_INDEX_U64 = 2
+};
+
+/* copy_if / remove_if / remove (copy_if.hpp:46-62 via transform_if.hpp:
+ * 33-82; remove_if.hpp, remove.hpp are copy_if of not1(pred) back into the
+ * range): the elements (or their indices) for which pred holds, in input
+ * order.  out == in is allowed (the in-place remove_if).  *count_out
+ * (device pointer, uint64) receives the number written; the shim reads it
+ * back (blocking, as transform_if.hpp:57-62 does).  n < 2^32. */
+int bcmi_copy_if(bcmi_queue queue, int dtype, const void *in, void *out,
+                 size_t n, const bcmi_predicate *pred, int out_kind,
+                 uint64_t *count_out);
+
+/* count_if / count (count_if.hpp, count.hpp: count(v) is count_if(_1 == v)):
+ * *count_out (device, uint64) = number of elements for which pred holds. */
+int bcmi_count_if(bcmi_queue queue, int dtype, const void *in, size_t n,
+                  const bcmi_predicate *pred, uint64_t *count_out);
+
+/* unique_copy / unique with equal_to (unique_copy.hpp:99-175, unique.hpp):
+ * the first element of each run of adjacent equal elements (float ==, so
+ * -0 and +0 are one run and every NaN is kept).  out == in is allowed (the
+ * in-place unique); elements past the count are left as they were.
+ * *count_out (device, uint64) = number written.  n < 2^32. */
+int bcmi_unique(bcmi_queue queue, int dtype, const void *in, void *out,
+                size_t n, uint64_t *count_out);
+
+/* partition_copy (partition_copy.hpp): out_false != NULL -- the elements
+ * for which pred holds to out_true, the others to out_false, both in input
+ * order.  stable_partition / partition (stable_partition.hpp,
+ * partition.hpp): out_false == NULL -- both groups into out_true, the
+ * selected first (out_true == in is allowed).  *count_out (device, uint64)
+ * = number of elements for which pred holds.  n < 2^32. */
+int bcmi_partition(bcmi_queue queue, int dtype, const void *in,
+                   void *out_true, void *out_false, size_t n,
+                   const bcmi_predicate *pred, uint64_t *count_out);
+
 /* --------------------------------------------------------- hipGraphs */
 /* Capture everything enqueued on the queue between begin and end into a
  * replayable graph (queue scratch must be reserved beforehand). */

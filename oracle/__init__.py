@@ -37,6 +37,32 @@ def kind_of(dt):
     return 0
 
 
+class OrcPredicate(C.Structure):
+    """f(_1) cmp c, optionally negated (orc_predicate in bcmi_oracle.h)."""
+    _fields_ = [("fn", C.c_int32), ("cmp", C.c_int32), ("negate", C.c_int32), ("reserved", C.c_int32),
+                ("operand", C.c_uint64), ("value", C.c_uint64)]
+
+
+FN = {"id": 0, "mul": 1, "mod": 2, "and": 3, "signbit": 4}
+CMP = {"<": 0, "<=": 1, ">": 2, ">=": 3, "==": 4, "!=": 5}
+
+
+def _cmp_dtype(dt, fn):
+    dt = np.dtype(dt)
+    if fn == "signbit" or (dt.kind in "iu" and dt.itemsize < 4):
+        return np.dtype(np.int32)
+    return dt
+
+
+def pred(dtype, cmp, c, fn="id", a=0, negate=False):
+    """Predicate over elements of dtype; a and c are stored as one value of
+    the comparison type (C's integer promotion; int for signbit)."""
+    ct = _cmp_dtype(dtype, fn)
+    def bits(v):
+        return int(np.frombuffer(np.array([v]).astype(ct).tobytes().ljust(8, b"\0"), np.uint64)[0])
+    return OrcPredicate(FN[fn], CMP[cmp], int(bool(negate)), 0, bits(a), bits(c))
+
+
 def build():
     import subprocess
     subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
@@ -89,6 +115,19 @@ def lib():
         L.orc_reduce_by_key_f32_dd.restype = sz
         L.orc_saxpy_f32.argtypes = [C.c_float, vp, vp, vp, sz]
         L.orc_transform.argtypes = [i, i, vp, vp, vp, sz]
+        pp = C.POINTER(OrcPredicate)
+        L.orc_pred_eval.argtypes = [i, vp, sz, pp]
+        L.orc_pred_eval.restype = i
+        L.orc_copy_if.argtypes = [i, vp, sz, pp, i, vp]
+        L.orc_copy_if.restype = sz
+        L.orc_count_if.argtypes = [i, vp, sz, pp]
+        L.orc_count_if.restype = sz
+        L.orc_unique_copy.argtypes = [i, vp, sz, vp]
+        L.orc_unique_copy.restype = sz
+        L.orc_partition_copy.argtypes = [i, vp, sz, pp, vp, vp]
+        L.orc_partition_copy.restype = sz
+        L.orc_stable_partition.argtypes = [i, vp, sz, pp]
+        L.orc_stable_partition.restype = sz
         _lib = L
     return _lib
 
@@ -309,3 +348,43 @@ def transform(a, b, op):
     out = np.empty_like(a)
     lib().orc_transform(NP2CODE[a.dtype], op, _p(a), _p(b), _p(out), a.size)
     return out
+
+
+# ------------------------------------------------------------- selection
+def pred_eval(x, p):
+    x = np.ascontiguousarray(x)
+    return np.array([lib().orc_pred_eval(NP2CODE[x.dtype], _p(x), i, C.byref(p)) for i in range(x.size)], bool)
+
+
+def copy_if(x, p, out_kind=0):
+    x = np.ascontiguousarray(x)
+    odt = {0: x.dtype, 1: np.uint32, 2: np.uint64}[out_kind]
+    out = np.zeros(x.size, odt)
+    k = lib().orc_copy_if(NP2CODE[x.dtype], _p(x), x.size, C.byref(p), out_kind, _p(out))
+    return out[:k]
+
+
+def count_if(x, p):
+    x = np.ascontiguousarray(x)
+    return int(lib().orc_count_if(NP2CODE[x.dtype], _p(x), x.size, C.byref(p)))
+
+
+def unique_copy(x):
+    x = np.ascontiguousarray(x)
+    out = np.zeros(x.size, x.dtype)
+    k = lib().orc_unique_copy(NP2CODE[x.dtype], _p(x), x.size, _p(out))
+    return out[:k]
+
+
+def partition_copy(x, p):
+    x = np.ascontiguousarray(x)
+    t = np.zeros(x.size, x.dtype)
+    f = np.zeros(x.size, x.dtype)
+    k = lib().orc_partition_copy(NP2CODE[x.dtype], _p(x), x.size, C.byref(p), _p(t), _p(f))
+    return t[:k], f[:x.size - k]
+
+
+def stable_partition(x, p):
+    d = np.ascontiguousarray(x).copy()
+    k = lib().orc_stable_partition(NP2CODE[d.dtype], _p(d), d.size, C.byref(p))
+    return d, int(k)

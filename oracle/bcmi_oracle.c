@@ -898,3 +898,168 @@ void orc_transform(int dtype, int op, const void *a, const void *b, void *out, s
 {
     for (size_t i = 0; i < n; ++i) st(dtype, out, i, apply(dtype, op, ld(dtype, a, i), ld(dtype, b, i)));
 }
+
+/* ======================================================================
+ * Selection.  The reference prints a lambda predicate as an OpenCL C
+ * expression (lambda/context.hpp:145-156) that the device compiler
+ * evaluates with C's usual arithmetic conversions; this restates that
+ * evaluation for the recognised forms f(_1) cmp c.
+ * ==================================================================== */
+static int promoted_i32(int dt) { return dt == ORC_I8 || dt == ORC_U8 || dt == ORC_I16 || dt == ORC_U16; }
+
+int orc_pred_eval(int dtype, const void *in, size_t i, const orc_predicate *p)
+{
+    num_t x = ld(dtype, in, i);
+    int lt, eq, gt, r;
+    if (p->fn == ORC_FN_SIGNBIT) {
+        /* signbit (functional/relational.hpp:32): OpenCL's int result */
+        int32_t s = dtype == ORC_F32 ? (int32_t)signbit(x.f) != 0 : (int32_t)signbit(x.d) != 0;
+        int32_t c; memcpy(&c, &p->value, 4);
+        lt = s < c; eq = s == c; gt = s > c;
+    } else if (promoted_i32(dtype) || dtype == ORC_I32) {
+        int32_t y = promoted_i32(dtype) ? (int32_t)(is_signed_dt(dtype) ? x.i : (int64_t)x.u) : (int32_t)x.i;
+        int32_t a, c;
+        memcpy(&a, &p->operand, 4);
+        memcpy(&c, &p->value, 4);
+        if (p->fn == ORC_FN_MUL) y = (int32_t)((uint32_t)y * (uint32_t)a);
+        else if (p->fn == ORC_FN_MOD) y = (a == -1) ? 0 : y % a;
+        else if (p->fn == ORC_FN_AND) y = y & a;
+        lt = y < c; eq = y == c; gt = y > c;
+    } else if (dtype == ORC_U32) {
+        uint32_t y = (uint32_t)x.u, a, c;
+        memcpy(&a, &p->operand, 4);
+        memcpy(&c, &p->value, 4);
+        if (p->fn == ORC_FN_MUL) y = y * a;
+        else if (p->fn == ORC_FN_MOD) y = y % a;
+        else if (p->fn == ORC_FN_AND) y = y & a;
+        lt = y < c; eq = y == c; gt = y > c;
+    } else if (dtype == ORC_I64) {
+        int64_t y = x.i, a, c;
+        memcpy(&a, &p->operand, 8);
+        memcpy(&c, &p->value, 8);
+        if (p->fn == ORC_FN_MUL) y = (int64_t)((uint64_t)y * (uint64_t)a);
+        else if (p->fn == ORC_FN_MOD) y = (a == -1) ? 0 : y % a;
+        else if (p->fn == ORC_FN_AND) y = y & a;
+        lt = y < c; eq = y == c; gt = y > c;
+    } else if (dtype == ORC_U64) {
+        uint64_t y = x.u, a = p->operand, c = p->value;
+        if (p->fn == ORC_FN_MUL) y = y * a;
+        else if (p->fn == ORC_FN_MOD) y = y % a;
+        else if (p->fn == ORC_FN_AND) y = y & a;
+        lt = y < c; eq = y == c; gt = y > c;
+    } else if (dtype == ORC_F32) {
+        float y = x.f, a, c;
+        memcpy(&a, &p->operand, 4);
+        memcpy(&c, &p->value, 4);
+        if (p->fn == ORC_FN_MUL) y = y * a;
+        lt = y < c; eq = y == c; gt = y > c;
+    } else {
+        double y = x.d, a, c;
+        memcpy(&a, &p->operand, 8);
+        memcpy(&c, &p->value, 8);
+        if (p->fn == ORC_FN_MUL) y = y * a;
+        lt = y < c; eq = y == c; gt = y > c;
+    }
+    switch (p->cmp) {
+    case ORC_CMP_LT: r = lt; break;
+    case ORC_CMP_LE: r = lt || eq; break;
+    case ORC_CMP_GT: r = gt; break;
+    case ORC_CMP_GE: r = gt || eq; break;
+    case ORC_CMP_EQ: r = eq; break;
+    default: r = !eq; break;
+    }
+    return p->negate ? !r : r;
+}
+
+/* transform_if_impl (transform_if.hpp:33-82) with the identity function:
+ * k1 writes indices[i] = pred(first[i]) ? 1 : 0 (:54-57); the count is the
+ * last flag plus the last element of the exclusive scan of the flags
+ * (:60-64); k2 writes result[indices[i]] = first[i], or i for
+ * copy_index_if (copy_if.hpp:25-41), where pred holds (:67-78). */
+size_t orc_copy_if(int dtype, const void *in, size_t n, const orc_predicate *p, int out_kind, void *out)
+{
+    if (n == 0) return 0;
+    uint64_t *idx = (uint64_t *)malloc(n * sizeof(uint64_t));
+    for (size_t i = 0; i < n; ++i) idx[i] = orc_pred_eval(dtype, in, i, p) ? 1 : 0;
+    uint64_t copied = idx[n - 1], acc = 0;
+    for (size_t i = 0; i < n; ++i) {
+        uint64_t f = idx[i];
+        idx[i] = acc;
+        acc += f;
+    }
+    copied += idx[n - 1];
+    for (size_t i = 0; i < n; ++i) {
+        if (!orc_pred_eval(dtype, in, i, p)) continue;
+        if (out_kind == 1) ((uint32_t *)out)[idx[i]] = (uint32_t)i;
+        else if (out_kind == 2) ((uint64_t *)out)[idx[i]] = (uint64_t)i;
+        else st(dtype, out, idx[i], ld(dtype, in, i));
+    }
+    free(idx);
+    return (size_t)copied;
+}
+
+/* count_if (count_if.hpp; count_if_with_reduce.hpp sums pred(x) as 0/1) */
+size_t orc_count_if(int dtype, const void *in, size_t n, const orc_predicate *p)
+{
+    size_t c = 0;
+    for (size_t i = 0; i < n; ++i) c += orc_pred_eval(dtype, in, i, p) ? 1 : 0;
+    return c;
+}
+
+/* unique_copy with equal_to (unique_copy.hpp:139-164): below 32 elements
+ * serial_unique_copy (:26-73) keeps next when !(current == next) and then
+ * makes it current; otherwise flags[0] = 1, flags[i] = !(x[i-1] == x[i])
+ * (:114-120) and the flagged elements are gathered in order (:122-131). */
+size_t orc_unique_copy(int dtype, const void *in, size_t n, void *out)
+{
+    if (n == 0) return 0;
+    if (n < 32) {
+        size_t index = 0;
+        num_t current = ld(dtype, in, 0);
+        st(dtype, out, 0, current);
+        for (size_t i = 1; i < n; ++i) {
+            num_t next = ld(dtype, in, i);
+            if (!key_equal(dtype, current, next)) {
+                st(dtype, out, ++index, next);
+                current = next;
+            }
+        }
+        return index + 1;
+    }
+    size_t r = 0;
+    for (size_t i = 0; i < n; ++i) {
+        int flag = i == 0 || !key_equal(dtype, ld(dtype, in, i - 1), ld(dtype, in, i));
+        if (flag) st(dtype, out, r++, ld(dtype, in, i));
+    }
+    return r;
+}
+
+/* partition_copy (partition_copy.hpp): copy_if(pred) to first_true, then
+ * copy_if(not1(pred)) to first_false. */
+size_t orc_partition_copy(int dtype, const void *in, size_t n, const orc_predicate *p, void *out_true,
+                          void *out_false)
+{
+    orc_predicate q = *p;
+    q.negate = !p->negate;
+    size_t t = orc_copy_if(dtype, in, n, p, 0, out_true);
+    orc_copy_if(dtype, in, n, &q, 0, out_false);
+    return t;
+}
+
+/* stable_partition (stable_partition.hpp): copy the range to a temporary,
+ * copy_if(pred) from it to first, then copy_if(not1(pred)) after the last
+ * true; returns the number of true elements.  partition (partition.hpp)
+ * is the same call. */
+size_t orc_stable_partition(int dtype, void *data, size_t n, const orc_predicate *p)
+{
+    if (n == 0) return 0;
+    size_t bytes = n * (size_t)dt_size(dtype);
+    void *tmp = malloc(bytes);
+    memcpy(tmp, data, bytes);
+    orc_predicate q = *p;
+    q.negate = !p->negate;
+    size_t t = orc_copy_if(dtype, tmp, n, p, 0, data);
+    orc_copy_if(dtype, tmp, n, &q, 0, (char *)data + t * (size_t)dt_size(dtype));
+    free(tmp);
+    return t;
+}

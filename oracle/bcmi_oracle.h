@@ -106,6 +106,27 @@ void orc_saxpy_f32(float a, const float *x, const float *y, float *out,
 void orc_transform(int dtype, int op, const void *a, const void *b, void *out,
                    size_t n);
 
+/* ------------------------------------------- selection (stream compaction) --
+ * A unary predicate "f(_1) cmp c" (f = _1, _1*a, _1%a, _1&a, signbit(_1);
+ * cmp < <= > >= == !=; negate = not1), evaluated with C's usual arithmetic
+ * conversions: 8/16-bit integers promote to int, signbit yields int. */
+enum { ORC_FN_ID = 0, ORC_FN_MUL = 1, ORC_FN_MOD = 2, ORC_FN_AND = 3, ORC_FN_SIGNBIT = 4 };
+enum { ORC_CMP_LT = 0, ORC_CMP_LE = 1, ORC_CMP_GT = 2, ORC_CMP_GE = 3, ORC_CMP_EQ = 4, ORC_CMP_NE = 5 };
+typedef struct {
+    int32_t fn, cmp, negate, reserved;
+    uint64_t operand, value;   /* low bytes: one value of the comparison type */
+} orc_predicate;
+
+int    orc_pred_eval(int dtype, const void *in, size_t i, const orc_predicate *p);
+/* out_kind 0: values, 1: uint32 indices, 2: uint64 indices */
+size_t orc_copy_if(int dtype, const void *in, size_t n, const orc_predicate *p,
+                   int out_kind, void *out);
+size_t orc_count_if(int dtype, const void *in, size_t n, const orc_predicate *p);
+size_t orc_unique_copy(int dtype, const void *in, size_t n, void *out);
+size_t orc_partition_copy(int dtype, const void *in, size_t n,
+                          const orc_predicate *p, void *out_true, void *out_false);
+size_t orc_stable_partition(int dtype, void *data, size_t n, const orc_predicate *p);
+
 #ifdef __cplusplus
 }
 #endif

@@ -254,6 +254,83 @@ case("transform_add_int", "test/test_transform.cpp:75", "transform", "int32", a=
 case("transform_mul_int", "test/test_transform.cpp:75", "transform", "int32", a=[1, 2, 3, 4],
      b=[10, 20, 30, 40], op="multiplies", expected=[10, 40, 90, 160])
 
+# -------------------------------------------------------------- selection
+# Predicates: {"fn": id|mul|mod|and|signbit, "a": operand, "cmp": op, "c": rhs,
+# "negate": not1}.  The vector-type cases (float4 dot product,
+# test/test_copy_if.cpp:70; int4/int2 counts, test/test_count.cpp:74,159) and
+# the constant_iterator case (test/test_count.cpp:44) are outside the
+# scalar-element path and are not transcribed.
+def pred(cmp, c, fn="id", a=0, negate=False):
+    return {"fn": fn, "a": a, "cmp": cmp, "c": c, "negate": negate}
+
+
+CI = [1, 6, 3, 5, 8, 2, 4]
+case("copy_if_int_lt5", "test/test_copy_if.cpp:37", "copy_if", "int32", input=CI,
+     pred=pred("<", 5), expected=[1, 3, 2, 4])
+case("copy_if_int_x2_ge10", "test/test_copy_if.cpp:44", "copy_if", "int32", input=CI,
+     pred=pred(">=", 10, "mul", 2), expected=[6, 5, 8])
+ODD = [1, 2, 3, 4, 5, 1, 2, 3, 4, 5]
+case("copy_if_odd", "test/test_copy_if.cpp:59", "copy_if", "int32", input=ODD,
+     pred=pred("==", 1, "mod", 2), expected=[1, 3, 5, 1, 3, 5])
+case("copy_if_even", "test/test_copy_if.cpp:66", "copy_if", "int32", input=ODD,
+     pred=pred("==", 0, "mod", 2), expected=[2, 4, 2, 4])
+case("copy_index_if_int", "test/test_copy_if.cpp:115", "copy_index_if", "int32", input=CI,
+     pred=pred("<", 5), expected=[0, 2, 5, 6])
+
+case("unique_int", "test/test_unique.cpp:35", "unique", "int32", input=[1, 6, 6, 4, 2, 2, 4],
+     expected=[1, 6, 4, 2, 4, 2, 4], expected_count=5)
+case("unique_all_same_float", "test/test_unique.cpp:47", "unique", "float32", input=[3.14] * 1024,
+     expected_count=1, expected_prefix=[3.14])
+IOTA = list(range(1024))
+case("unique_iota_uint", "test/test_unique.cpp:66", "unique", "uint32", input=IOTA, expected_count=1024,
+     expected=IOTA)
+EVENS = [x - (x & 1) for x in IOTA]
+case("unique_even_uints", "test/test_unique.cpp:86", "unique", "uint32", input=EVENS, expected_count=512,
+     expected_prefix=list(range(0, 1024, 2)))
+case("unique_copy_int", "test/test_unique_copy.cpp:33", "unique_copy", "int32", input=[1, 6, 6, 4, 2, 2, 4],
+     expected=[1, 6, 4, 2, 4])
+
+# partition() is stable_partition() (partition.hpp); the reference test checks
+# the count and the signs of the two groups
+case("partition_float_signbit", "test/test_partition.cpp:51", "partition", "float32",
+     input=[1.0, 2.0, -1.0, -2.0, 3.0, 4.0, -3.0, -4.0], pred=pred("!=", 0, "signbit"),
+     expected=[-1.0, -2.0, -3.0, -4.0, 1.0, 2.0, 3.0, 4.0], expected_count=4)
+case("partition_small_one", "test/test_partition.cpp:75", "partition", "float32", input=[1.0],
+     pred=pred("!=", 0, "signbit"), expected=[1.0], expected_count=0)
+case("partition_small_two", "test/test_partition.cpp:79", "partition", "float32", input=[1.0, -1.0],
+     pred=pred("!=", 0, "signbit"), expected=[-1.0, 1.0], expected_count=1)
+case("stable_partition_int", "test/test_stable_partition.cpp:34", "partition", "int32",
+     input=[1, 1, -2, 0, 5, -1, 2, 4, 0, -1], pred=pred(">", 0),
+     expected=[1, 1, 5, 2, 4, -2, 0, -1, 0, -1], expected_count=5)
+
+# remove(v) = remove_if(_1 == v) = copy_if(not1(_1 == v)) back into the range
+# (remove.hpp, remove_if.hpp); each step works on the prefix the last left
+case("remove_int", "test/test_remove.cpp:32", "remove", "int32", input=[1, 2, 1, 3, 2, 4, 3, 4, 5],
+     steps=[{"len": 9, "value": 2, "count": 7, "expected": [1, 1, 3, 4, 3, 4, 5]},
+            {"len": 7, "value": 4, "count": 5, "expected": [1, 1, 3, 3, 5]},
+            {"len": 5, "value": 1, "count": 3, "expected": [3, 3, 5]},
+            {"len": 3, "value": 5, "count": 2, "expected": [3, 3]},
+            {"len": 2, "value": 3, "count": 0, "expected": []}])
+
+CNT = [1, 2, 1, 2, 3]
+for line, lo, hi, v, want in [(36, 0, 5, 1, 2), (37, 0, 5, 2, 2), (38, 0, 5, 3, 1), (39, 1, 5, 1, 1),
+                              (40, 1, 4, 3, 0), (41, 1, 4, 2, 2)]:
+    case("count_int_%d" % line, "test/test_count.cpp:%d" % line, "count_if", "int32", input=CNT[lo:hi],
+         pred=pred("==", v), expected=want)
+case("count_if_greater_than_two", "test/test_count.cpp:69", "count_if", "float32",
+     input=[1.0, 2.5, -1.0, 3.0, 5.0], pred=pred(">", 2.0), expected=3)
+NL = [ord(ch) for ch in "abcdefg\nhijklmn\nopqrs\ntuv\nwxyz\n"]
+case("count_newlines", "test/test_count.cpp:124", "count_if", "int8", input=NL, pred=pred("==", 10), expected=5)
+UC = [0x00, 0x10, 0x2F, 0x10, 0x01, 0x00, 0x01, 0x00]
+for line, v, want in [(138, 0x00, 3), (142, 0x10, 2), (146, 0x2F, 1), (150, 0x01, 2), (154, 0xFF, 0)]:
+    case("count_uchar_%d" % line, "test/test_count.cpp:%d" % line, "count_if", "uint8", input=UC,
+         pred=pred("==", v), expected=want)
+IOTA2K = list(range(2048))
+case("count_if_odd", "test/test_count.cpp:202", "count_if", "int32", input=IOTA2K,
+     pred=pred("!=", 0, "and", 1), expected=1024)
+case("count_if_with_reduce", "test/test_count.cpp:217", "count_if", "int32", input=IOTA2K,
+     pred=pred(">", 1024), expected=1023)
+
 if __name__ == "__main__":
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.json")
     with open(out, "w") as f:

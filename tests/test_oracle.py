@@ -442,3 +442,147 @@ def test_golden_transform(golden):
         op = {"plus": O.PLUS, "multiplies": O.MULTIPLIES}[c["op"]]
         got = O.transform(_np(c["a"], c["dtype"]), _np(c["b"], c["dtype"]), op)
         assert list(got) == c["expected"], c["id"]
+
+
+# ------------------------------------------------------------- selection
+def _pred_of(case):
+    p = case["pred"]
+    return O.pred(case["dtype"], p["cmp"], p["c"], p["fn"], p["a"], p["negate"])
+
+
+def test_golden_selection(golden):
+    seen = set()
+    for c in golden:
+        algo = c["algo"]
+        if algo not in ("copy_if", "copy_index_if", "unique", "unique_copy", "partition", "remove", "count_if"):
+            continue
+        seen.add(algo)
+        x = np.array(c["input"], dtype=c["dtype"])
+        if algo == "copy_if":
+            assert O.copy_if(x, _pred_of(c)).tolist() == c["expected"], c["id"]
+        elif algo == "copy_index_if":
+            assert O.copy_if(x, _pred_of(c), 1).tolist() == c["expected"], c["id"]
+        elif algo == "count_if":
+            assert O.count_if(x, _pred_of(c)) == c["expected"], c["id"]
+        elif algo == "unique_copy":
+            assert O.unique_copy(x).tolist() == c["expected"], c["id"]
+        elif algo == "unique":
+            u = O.unique_copy(x)
+            d = x.copy()
+            d[:u.size] = u  # unique.hpp: unique_copy from a temporary back into the range
+            assert u.size == c["expected_count"], c["id"]
+            if "expected" in c:
+                assert np.array_equal(d, np.array(c["expected"], c["dtype"])), c["id"]
+            if "expected_prefix" in c:
+                assert np.array_equal(u, np.array(c["expected_prefix"], c["dtype"])), c["id"]
+        elif algo == "partition":
+            d, k = O.stable_partition(x, _pred_of(c))
+            assert k == c["expected_count"], c["id"]
+            assert np.array_equal(d, np.array(c["expected"], c["dtype"])), c["id"]
+        elif algo == "remove":
+            d = x.copy()
+            for s in c["steps"]:
+                p = O.pred(c["dtype"], "==", s["value"], negate=True)
+                kept = O.copy_if(d[:s["len"]], p)
+                d[:kept.size] = kept
+                assert kept.size == s["count"] and kept.tolist() == s["expected"], (c["id"], s)
+    assert seen == {"copy_if", "copy_index_if", "unique", "unique_copy", "partition", "remove", "count_if"}
+
+
+def _c_rem(x, a):
+    """C's truncating remainder on Python ints."""
+    r = abs(x) % abs(a)
+    return -r if x < 0 else r
+
+
+def _wrap(v, bits, signed):
+    v &= (1 << bits) - 1
+    return v - (1 << bits) if signed and v >> (bits - 1) else v
+
+
+PRED_CASES = [
+    # (dtype, fn, a, cmp, c): chosen so that floor-mod, unsigned-vs-signed,
+    # missing integer promotion or a swapped comparison give another answer
+    ("int32", "mod", 2, "==", 1), ("int32", "mod", 2, "==", -1), ("int32", "mod", 3, "!=", 0),
+    ("int32", "mod", -4, "<", 0), ("int64", "mod", 7, ">=", 3), ("int8", "mod", 4, "==", -1),
+    ("uint32", "mod", 6, "==", 5), ("uint64", "mod", 1 << 40, ">", 5),
+    ("int8", "mul", 3, ">=", 200), ("uint8", "mul", 2, ">", 255), ("int16", "mul", 3, "<", -40000),
+    ("int32", "mul", 3, ">", 0), ("uint32", "id", 0, ">", 2 ** 31), ("uint32", "id", 0, "<=", 7),
+    ("int32", "and", 1, "!=", 0), ("uint16", "and", 0x8001, "==", 0x8000), ("int64", "id", 0, "<", -5),
+    ("uint64", "id", 0, ">=", 2 ** 63),
+]
+
+
+@pytest.mark.parametrize("case", PRED_CASES, ids=lambda c: "-".join(map(str, c)))
+@pytest.mark.parametrize("negate", [False, True])
+def test_predicate_integers_follow_c_arithmetic(case, negate):
+    dt, fn, a, cmp, c = case
+    info = np.iinfo(dt)
+    x = RNG.integers(info.min, info.max, size=600, dtype=dt, endpoint=True)
+    x[:8] = [info.min, info.max, 0, 1, info.min + 1, info.max - 1, 2, 3] if info.min < 0 else \
+        [0, info.max, 1, 2, info.max - 1, 3, 4, 5]
+    signed = info.min < 0 or np.dtype(dt).itemsize < 4
+    bits = max(32, np.dtype(dt).itemsize * 8)
+    want = []
+    for v in x.tolist():
+        y = v
+        if fn == "mul":
+            y = _wrap(v * a, bits, signed)
+        elif fn == "mod":
+            y = _c_rem(v, a)
+        elif fn == "and":
+            y = _wrap(v & _wrap(a, bits, signed), bits, signed)
+        cc = _wrap(c, bits, signed)
+        r = {"<": y < cc, "<=": y <= cc, ">": y > cc, ">=": y >= cc, "==": y == cc, "!=": y != cc}[cmp]
+        want.append(r != negate)
+    got = O.pred_eval(x, O.pred(dt, cmp, c, fn, a, negate))
+    assert got.tolist() == want
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_predicate_floats_follow_ieee(dt):
+    x = RNG.normal(0, 10, 500).astype(dt)
+    x[:6] = [np.nan, -np.nan, 0.0, -0.0, np.inf, -np.inf]
+    x[6:10] = np.copysign(np.nan, -1.0), 2.0, 2.5, -2.5
+    for cmp in ("<", "<=", ">", ">=", "==", "!="):
+        for fn, a in (("id", 0), ("mul", 0.5)):
+            for neg in (False, True):
+                y = x * np.array(a, dt) if fn == "mul" else x
+                with np.errstate(invalid="ignore"):
+                    r = {"<": y < 2.5, "<=": y <= 2.5, ">": y > 2.5, ">=": y >= 2.5, "==": y == 2.5,
+                         "!=": y != 2.5}[cmp]
+                got = O.pred_eval(x, O.pred(dt, cmp, 2.5, fn, a, neg))
+                assert np.array_equal(got, r != neg), (cmp, fn, neg)
+    sb = O.pred_eval(x, O.pred(dt, "!=", 0, "signbit"))
+    assert np.array_equal(sb, np.signbit(x))  # -0.0 and -NaN included
+
+
+def _bits_equal(a, b):
+    """Equal dtype, shape and bits (NaN payloads and signed zeros included)."""
+    return a.dtype == b.dtype and a.shape == b.shape and a.tobytes() == b.tobytes()
+
+
+@pytest.mark.parametrize("dt", ["int8", "uint16", "int32", "uint64", "float32", "float64"])
+def test_selection_matches_numpy(dt):
+    for n in (0, 1, 31, 32, 1000):
+        if np.dtype(dt).kind == "f":
+            x = RNG.integers(-3, 4, n).astype(dt)
+            if n > 10:
+                x[3], x[4], x[7], x[8] = np.nan, np.nan, 0.0, -0.0
+            p = O.pred(dt, ">", 0.5)
+            mask = x > 0.5
+        else:
+            x = RNG.integers(0, 5, n).astype(dt)
+            p = O.pred(dt, "==", 1, "mod", 2)
+            mask = x % 2 == 1
+        assert _bits_equal(O.copy_if(x, p), x[mask])
+        assert np.array_equal(O.copy_if(x, p, 1), np.flatnonzero(mask).astype(np.uint32))
+        assert np.array_equal(O.copy_if(x, p, 2), np.flatnonzero(mask).astype(np.uint64))
+        assert O.count_if(x, p) == int(mask.sum())
+        t, f = O.partition_copy(x, p)
+        assert _bits_equal(t, x[mask]) and _bits_equal(f, x[~mask])
+        d, k = O.stable_partition(x, p)
+        assert k == int(mask.sum()) and _bits_equal(d, np.concatenate([x[mask], x[~mask]]))
+        head = np.ones(n, bool)
+        head[1:] = x[1:] != x[:-1]  # != on floats: NaN heads its own run, -0 == +0
+        assert _bits_equal(O.unique_copy(x), x[head]), n
