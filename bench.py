@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0  # MI355X HBM3E
-SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "segsort"]
+SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "select", "segsort"]
 
 
 def timed_steps(ca, q, fn, steps, warmup, restore=None):
@@ -268,6 +268,59 @@ def main():
         sec["segmented_exclusive_scan_2p27"] = {"GBps": round(gbs(12 * n, t), 1), "ms": round(t, 4),
                                                 "frac": round(gbs(12 * n, t) / PEAK_GBS, 4)}
         del dk, dv, ko, vo, k, v
+
+    # -------------------------------------------------- stream compaction
+    if "select" in only:
+        import ctypes
+        n = 1 << 28
+        x = W.scan_i32(n)  # the exclusive-scan input: int32 uniform in [-100, 100]
+        dx = ca.DeviceArray.from_host(x, q)
+        do, df = ca.DeviceArray(n, np.int32, q), ca.DeviceArray(n, np.int32, q)
+        cnt = ca.DeviceArray(1, np.uint64, q)
+        pr = ca.predicate(np.int32, ">=", 0)
+        kept = int(np.count_nonzero(x >= 0))
+        fn = lambda: ca.copy_if(q, dx, do, pr, count=cnt)
+        ms, _ = timed_steps(ca, q, fn, K, Wm)
+        t = float(np.mean(ms))
+        assert int(cnt.to_host(q)[0]) == kept, "copy_if count"
+        kern = kernel_breakdown(q, fn, ["select"])
+        alg = 4 * n + 4 * kept
+        sec["copy_if_i32_2p28"] = {"GBps": round(gbs(alg, t), 1), "ms": round(t, 4),
+                                   "frac": round(gbs(alg, t) / PEAK_GBS, 4), "selected": kept,
+                                   "alg_bytes": "4 B read per element + 4 B per selected element",
+                                   "kernels_ms": kern}
+        fc = lambda: ca.check(ca.lib().bcmi_count_if(q.h, ca.I32, dx.ptr, n, ctypes.byref(pr), cnt.ptr))
+        ms, _ = timed_steps(ca, q, fc, K, Wm)
+        t = float(np.mean(ms))
+        assert int(cnt.to_host(q)[0]) == kept, "count_if count"
+        sec["count_if_i32_2p28"] = {"GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4),
+                                    "frac": round(gbs(4 * n, t) / PEAK_GBS, 4)}
+        fp = lambda: ca.partition(q, dx, do, df, pr, count=cnt)
+        ms, _ = timed_steps(ca, q, fp, K, Wm)
+        t = float(np.mean(ms))
+        sec["partition_copy_i32_2p28"] = {"GBps": round(gbs(8 * n, t), 1), "ms": round(t, 4),
+                                          "frac": round(gbs(8 * n, t) / PEAK_GBS, 4)}
+        fs = lambda: ca.partition(q, dx, do, None, pr, count=cnt)
+        ms, _ = timed_steps(ca, q, fs, K, Wm)
+        t = float(np.mean(ms))
+        got = do.to_host(q)
+        assert np.array_equal(got[:kept], x[x >= 0]) and np.array_equal(got[kept:], x[x < 0]), "stable_partition"
+        sec["stable_partition_i32_2p28"] = {"GBps_vs_8B": round(gbs(8 * n, t), 1), "ms": round(t, 4),
+                                            "frac_vs_8B": round(gbs(8 * n, t) / PEAK_GBS, 4),
+                                            "passes": 2, "note": "two passes (16 B/element moved); "
+                                                                 "frac against the 8 B/element minimum"}
+        del dx, do, df, x, got
+        nk = 1 << 27
+        k, _ = W.runs_i32_f32(nk)
+        dk, dko = ca.DeviceArray.from_host(k, q), ca.DeviceArray(nk, np.int32, q)
+        fu = lambda: ca.unique(q, dk, dko, count=cnt)
+        ms, _ = timed_steps(ca, q, fu, K, Wm)
+        t = float(np.mean(ms))
+        runs = int(cnt.to_host(q)[0])
+        alg = 4 * nk + 4 * runs
+        sec["unique_i32_runs_2p27"] = {"GBps": round(gbs(alg, t), 1), "ms": round(t, 4),
+                                       "frac": round(gbs(alg, t) / PEAK_GBS, 4), "runs": runs}
+        del dk, dko, k
 
     # ------------------------------------------------------ segmented sort
     if "segsort" in only:

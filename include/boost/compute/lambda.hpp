@@ -1,12 +1,23 @@
 //---------------------------------------------------------------------------//
 // The lambda placeholders _1, _2 (reference: lambda/placeholders.hpp).  The
-// reference prints arbitrary lambda expressions as OpenCL C; this build has
-// no run-time compiler, so it recognises the expression its benchmark uses,
-// a * _1 + _2 (saxpy, perf/perf_saxpy.cpp:50-55), and maps it to a
-// precompiled kernel.
+// reference prints arbitrary lambda expressions as OpenCL C
+// (lambda/context.hpp:145-156); this build has no run-time compiler, so it
+// recognises the expression forms its kernels implement and maps each to a
+// precompiled kernel:
+//   a * _1 + _2                       saxpy (perf/perf_saxpy.cpp:50-55)
+//   f(_1) cmp c                       unary predicates of the selection
+//                                     algorithms (copy_if, count_if, remove_if,
+//                                     partition...), with f one of _1, _1 * a,
+//                                     a * _1, _1 % a, _1 & a and cmp one of
+//                                     < <= > >= == !=
+// Anything else does not compile.
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_LAMBDA_HPP
 #define BOOST_COMPUTE_LAMBDA_HPP
+
+#include <type_traits>
+
+#include <bcmi.h>
 
 namespace boost {
 namespace compute {
@@ -36,12 +47,74 @@ inline scaled_arg1<T> operator*(T a, placeholder<1>)
 }
 
 template <class T>
+inline scaled_arg1<T> operator*(placeholder<1>, T a)
+{
+    return scaled_arg1<T>{a};
+}
+
+template <class T>
 inline saxpy_expr<T> operator+(scaled_arg1<T> s, placeholder<2>)
 {
     return saxpy_expr<T>{s.a};
 }
 
+// f(_1) for f = _1 % a, _1 & a
+template <int FN, class A>
+struct arg1_expr {
+    A a;
+};
+
+template <class A, class = typename std::enable_if<std::is_integral<A>::value>::type>
+inline arg1_expr<BCMI_FN_MOD, A> operator%(placeholder<1>, A a)
+{
+    return arg1_expr<BCMI_FN_MOD, A>{a};
+}
+
+template <class A, class = typename std::enable_if<std::is_integral<A>::value>::type>
+inline arg1_expr<BCMI_FN_AND, A> operator&(placeholder<1>, A a)
+{
+    return arg1_expr<BCMI_FN_AND, A>{a};
+}
+
+// f(_1) cmp c
+template <int FN, class A, class L>
+struct pred_expr {
+    A a;      // f's operand (unused for _1)
+    L c;      // right-hand side
+    int cmp;  // bcmi_cmp
+};
+
+#define BOOST_COMPUTE_LAMBDA_CMP(OP, CODE)                                                              \
+    template <class L, class = typename std::enable_if<std::is_arithmetic<L>::value>::type>             \
+    inline pred_expr<BCMI_FN_ID, int, L> operator OP(placeholder<1>, L c)                               \
+    {                                                                                                   \
+        return pred_expr<BCMI_FN_ID, int, L>{0, c, CODE};                                               \
+    }                                                                                                   \
+    template <class A, class L, class = typename std::enable_if<std::is_arithmetic<L>::value>::type>    \
+    inline pred_expr<BCMI_FN_MUL, A, L> operator OP(scaled_arg1<A> s, L c)                              \
+    {                                                                                                   \
+        return pred_expr<BCMI_FN_MUL, A, L>{s.a, c, CODE};                                              \
+    }                                                                                                   \
+    template <int FN, class A, class L,                                                                 \
+              class = typename std::enable_if<std::is_arithmetic<L>::value>::type>                      \
+    inline pred_expr<FN, A, L> operator OP(arg1_expr<FN, A> f, L c)                                     \
+    {                                                                                                   \
+        return pred_expr<FN, A, L>{f.a, c, CODE};                                                       \
+    }
+
+BOOST_COMPUTE_LAMBDA_CMP(<, BCMI_CMP_LT)
+BOOST_COMPUTE_LAMBDA_CMP(<=, BCMI_CMP_LE)
+BOOST_COMPUTE_LAMBDA_CMP(>, BCMI_CMP_GT)
+BOOST_COMPUTE_LAMBDA_CMP(>=, BCMI_CMP_GE)
+BOOST_COMPUTE_LAMBDA_CMP(==, BCMI_CMP_EQ)
+BOOST_COMPUTE_LAMBDA_CMP(!=, BCMI_CMP_NE)
+#undef BOOST_COMPUTE_LAMBDA_CMP
+
 }  // namespace lambda
+
+using lambda::_1;
+using lambda::_2;
+
 }  // namespace compute
 }  // namespace boost
 

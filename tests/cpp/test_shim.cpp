@@ -200,6 +200,105 @@ int main()
         for (int i = 0; i < 1000; ++i) ok = ok && r[i] == std::fma(2.5f, x[i], y[i]);
         CHECK(ok);
     }
+    {  // copy_if / copy_index_if: test/test_copy_if.cpp:24-67,101-117
+        using bc::_1;
+        int data[] = {1, 6, 3, 5, 8, 2, 4};
+        bc::vector<int> input(data, data + 7, queue);
+        bc::vector<int> output(7, queue.get_context());
+        bc::fill(output.begin(), output.end(), -1, queue);
+        bc::vector<int>::iterator it = bc::copy_if(input.begin(), input.end(), output.begin(), _1 < 5, queue);
+        CHECK(it == output.begin() + 4);
+        CHECK(host(output, queue) == (std::vector<int>{1, 3, 2, 4, -1, -1, -1}));
+        bc::fill(output.begin(), output.end(), 42, queue);
+        it = bc::copy_if(input.begin(), input.end(), output.begin(), _1 * 2 >= 10, queue);
+        CHECK(it == output.begin() + 3);
+        CHECK(host(output, queue) == (std::vector<int>{6, 5, 8, 42, 42, 42, 42}));
+        bc::fill(output.begin(), output.end(), -1, queue);
+        it = bc::detail::copy_index_if(input.begin(), input.end(), output.begin(), _1 < 5, queue);
+        CHECK(it == output.begin() + 4);
+        CHECK(host(output, queue) == (std::vector<int>{0, 2, 5, 6, -1, -1, -1}));
+        int odd[] = {1, 2, 3, 4, 5, 1, 2, 3, 4, 5};
+        bc::vector<int> in2(odd, odd + 10, queue), odds(10, queue.get_context()), evens(10, queue.get_context());
+        CHECK(bc::copy_if(in2.begin(), in2.end(), odds.begin(), _1 % 2 == 1, queue) == odds.begin() + 6);
+        CHECK(bc::copy_if(in2.begin(), in2.end(), evens.begin(), _1 % 2 == 0, queue) == evens.begin() + 4);
+        std::vector<int> ho = host(odds, queue), he = host(evens, queue);
+        CHECK(std::vector<int>(ho.begin(), ho.begin() + 6) == (std::vector<int>{1, 3, 5, 1, 3, 5}));
+        CHECK(std::vector<int>(he.begin(), he.begin() + 4) == (std::vector<int>{2, 4, 2, 4}));
+    }
+    {  // unique / unique_copy: test/test_unique.cpp:26-51, test/test_unique_copy.cpp:23-34
+        int data[] = {1, 6, 6, 4, 2, 2, 4};
+        bc::vector<int> input(data, data + 7, queue);
+        bc::vector<int> result(5, queue.get_context());
+        CHECK(bc::unique_copy(input.begin(), input.end(), result.begin(), queue) == result.begin() + 5);
+        CHECK(host(result, queue) == (std::vector<int>{1, 6, 4, 2, 4}));
+        CHECK(bc::unique(input.begin(), input.end(), queue) == input.begin() + 5);
+        CHECK(host(input, queue) == (std::vector<int>{1, 6, 4, 2, 4, 2, 4}));
+        bc::vector<float> same(1024, queue.get_context());
+        bc::fill(same.begin(), same.end(), 3.14f, queue);
+        CHECK(bc::unique(same.begin(), same.end(), bc::equal_to<float>(), queue) == same.begin() + 1);
+        CHECK(same.begin().read(queue) == 3.14f);
+    }
+    {  // partition / stable_partition / partition_copy: test/test_partition.cpp:27-80,
+       // test/test_stable_partition.cpp:25-36
+        float fdata[] = {1.0f, 2.0f, -1.0f, -2.0f, 3.0f, 4.0f, -3.0f, -4.0f};
+        bc::vector<float> v(fdata, fdata + 8, queue);
+        bc::vector<float>::iterator it = bc::partition(v.begin(), v.end(), bc::signbit_<float>(), queue);
+        CHECK(it == v.begin() + 4);
+        std::vector<float> h = host(v, queue);
+        CHECK(h[0] < 0 && h[1] < 0 && h[2] < 0 && h[3] < 0 && h[4] > 0 && h[5] > 0 && h[6] > 0 && h[7] > 0);
+        int idata[] = {1, 1, -2, 0, 5, -1, 2, 4, 0, -1};
+        bc::vector<int> vi(idata, idata + 10, queue);
+        bc::vector<int>::iterator iit = bc::stable_partition(vi.begin(), vi.begin() + 10, bc::_1 > 0, queue);
+        CHECK(iit == vi.begin() + 5);
+        CHECK(host(vi, queue) == (std::vector<int>{1, 1, 5, 2, 4, -2, 0, -1, 0, -1}));
+        bc::vector<int> src(idata, idata + 10, queue), t(10, queue.get_context()), f(10, queue.get_context());
+        std::pair<bc::vector<int>::iterator, bc::vector<int>::iterator> ends =
+            bc::partition_copy(src.begin(), src.end(), t.begin(), f.begin(), bc::_1 > 0, queue);
+        CHECK(ends.first == t.begin() + 5 && ends.second == f.begin() + 5);
+        std::vector<int> ht = host(t, queue), hf = host(f, queue);
+        CHECK(std::vector<int>(ht.begin(), ht.begin() + 5) == (std::vector<int>{1, 1, 5, 2, 4}));
+        CHECK(std::vector<int>(hf.begin(), hf.begin() + 5) == (std::vector<int>{-2, 0, -1, 0, -1}));
+    }
+    {  // remove / remove_if: test/test_remove.cpp:23-53
+        int data[] = {1, 2, 1, 3, 2, 4, 3, 4, 5};
+        bc::vector<int> v(data, data + 9, queue);
+        bc::vector<int>::iterator it = bc::remove(v.begin(), v.end(), 2, queue);
+        CHECK(it == v.begin() + 7);
+        std::vector<int> h = host(v, queue);
+        CHECK(std::vector<int>(h.begin(), h.begin() + 7) == (std::vector<int>{1, 1, 3, 4, 3, 4, 5}));
+        it = bc::remove(v.begin(), v.begin() + 7, 4, queue);
+        CHECK(it == v.begin() + 5);
+        it = bc::remove_if(v.begin(), v.begin() + 5, bc::_1 == 1, queue);
+        CHECK(it == v.begin() + 3);
+        h = host(v, queue);
+        CHECK(std::vector<int>(h.begin(), h.begin() + 3) == (std::vector<int>{3, 3, 5}));
+    }
+    {  // count / count_if: test/test_count.cpp:32-42,63-72,117-157,206-218
+        int data[] = {1, 2, 1, 2, 3};
+        bc::vector<int> v(data, data + 5, queue);
+        CHECK(bc::count(v.begin(), v.end(), 1, queue) == 2);
+        CHECK(bc::count(v.begin(), v.end(), 3, queue) == 1);
+        CHECK(bc::count(v.begin() + 1, v.end(), 1, queue) == 1);
+        CHECK(bc::count(v.begin() + 1, v.end() - 1, 2, queue) == 2);
+        float fdata[] = {1.0f, 2.5f, -1.0f, 3.0f, 5.0f, -8.0f};
+        bc::vector<float> fv(fdata, fdata + 5, queue);
+        CHECK(bc::count_if(fv.begin(), fv.end(), bc::_1 > 2.0f, queue) == 3);
+        std::string s = "abcdefg\nhijklmn\nopqrs\ntuv\nwxyz\n";
+        bc::vector<char> cv(s.begin(), s.end(), queue);
+        CHECK(bc::count(cv.begin(), cv.end(), '\n', queue) == 5);
+        unsigned char ud[] = {0x00, 0x10, 0x2F, 0x10, 0x01, 0x00, 0x01, 0x00};
+        bc::vector<unsigned char> uv(ud, ud + 8, queue);
+        CHECK(bc::count(uv.begin(), uv.end(), 0x00, queue) == 3);
+        CHECK(bc::count(uv.begin(), uv.end(), 0x2F, queue) == 1);
+        CHECK(bc::count(uv.begin(), uv.end(), 0xFF, queue) == 0);
+        std::vector<int> iota(2048);
+        std::iota(iota.begin(), iota.end(), 0);
+        bc::vector<int> iv(iota.begin(), iota.end(), queue);
+        CHECK(bc::count_if(iv.begin(), iv.end(), bc::_1 > 1024, queue) == 1023);
+        CHECK(bc::count_if(iv.begin(), iv.end(), bc::_1 & 1, queue) == 1024);  // nonzero is true, as in C
+        CHECK(bc::count_if(iv.begin(), iv.end(), (bc::_1 & 1) != 0, queue) == 1024);
+        CHECK(bc::count_if(iv.begin(), iv.end(), bc::not1(bc::_1 > 1024), queue) == 1025);
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {

@@ -102,3 +102,23 @@ def test_cpp_headers_compile_standalone():
     r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I", inc, "-x", "c++", "-"],
                        input=src + "int main(){return 0;}\n", text=True, capture_output=True, timeout=120)
     assert r.returncode == 0, r.stderr[:2000]
+
+
+@pytest.mark.parametrize("expr,ok", [
+    ("bc::_1 < 5", True), ("bc::_1 * 2 >= 10", True), ("bc::_1 % 2 == 1", True), ("(bc::_1 & 1) != 0", True),
+    ("bc::_1 & 1", True), ("bc::not1(bc::_1 > 0)", True),
+    # C would compare an int element with an unsigned or double literal in
+    # that wider type; the kernels compare in int, so these must not compile
+    ("bc::_1 < 5u", False), ("bc::_1 > 2.5", False), ("bc::_1 * 2.0 > 1", False),
+    ("bc::signbit_<float>()", False), ("bc::_1 + 1", False),
+])
+def test_selection_predicates_follow_c_conversions_at_compile_time(expr, ok):
+    """A predicate literal whose type would make C compare in another type
+    than the kernel does is rejected when the program is compiled."""
+    inc = os.path.normpath(os.path.join(os.path.dirname(CPP_BUILD), "..", "..", "include"))
+    src = ("#include <boost/compute/algorithm/count_if.hpp>\nnamespace bc = boost::compute;\n"
+           "size_t f(bc::vector<int> &v, bc::command_queue &q) { return bc::count_if(v.begin(), v.end(), %s, q); }\n"
+           "int main(){return 0;}\n" % expr)
+    r = subprocess.run(["g++", "-std=c++14", "-fsyntax-only", "-I", inc, "-x", "c++", "-"], input=src, text=True,
+                       capture_output=True, timeout=120)
+    assert (r.returncode == 0) == ok, r.stderr[:1500]
