@@ -126,6 +126,21 @@ BDEV uint32_t launch_epoch(const Ctrl *ctrl, uint32_t back)
     return ((e - back) % kEpochMask) + 1u;
 }
 
+// The same epoch, read where it is first needed.  The coherent load costs a
+// round trip to the device-wide cache; issued at kernel start it holds the
+// tile's data loads behind it.  `after` (a value computed from the tile's
+// data) makes the load's address depend on the data, so it is issued after
+// the data loads.
+template <typename D>
+BDEV uint32_t launch_epoch_after(const Ctrl *ctrl, uint32_t back, D after)
+{
+    uint32_t z;
+    asm volatile("v_mov_b32 %0, 0 ; %1" : "=v"(z) : "v"(after));
+    const uint32_t *p = (const uint32_t *)((const char *)&ctrl->epoch + z);
+    const uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return ((__builtin_amdgcn_readfirstlane(e) - back) % kEpochMask) + 1u;
+}
+
 // Tiles are numbered by blockIdx.x.  Workgroups are dispatched in id order,
 // so when a block runs, every block with a smaller id has been dispatched
 // and is resident or finished: the look-back wait always makes progress.
@@ -280,27 +295,22 @@ BDEV uint32_t lb_read(const uint64_t *status, size_t tile, uint32_t epoch, Cr &v
     return lb_flag(w[0]);
 }
 
-// Called by all 64 lanes of one wave.  Publishes the tile's aggregate,
-// folds the predecessors' aggregates / inclusive prefixes 64 tiles at a
-// time until an inclusive prefix is found, publishes this tile's inclusive
-// prefix and returns its exclusive prefix (init folded in at tile 0).
-// `cop(a, b)` must be associative; a is the earlier operand.  Lane l reads
-// tile (base - l), so the lanes are folded with the operands swapped to keep
-// tile order.  When a word before the first inclusive prefix is not yet
-// published, only that one word is polled (by lane 0) until it appears, and
-// then the 64-wide read is repeated: a thousand waiting tiles re-reading 64
-// words each would load the memory system with coherent reads.
+// Called by all 64 lanes of one wave of a tile > 0: folds the predecessors'
+// aggregates / inclusive prefixes 64 tiles at a time until an inclusive
+// prefix is found and returns the tile's exclusive prefix (without init;
+// tile 0's inclusive prefix carries it).  Publishes nothing, so it can run
+// before the tile's own aggregate is known.  `cop(a, b)` must be
+// associative; a is the earlier operand.  Lane l reads tile (base - l), so
+// the lanes are folded with the operands swapped to keep tile order.  When
+// a word before the first inclusive prefix is not yet published, only that
+// one word is polled (by lane 0) until it appears, and then the 64-wide
+// read is repeated: a thousand waiting tiles re-reading 64 words each would
+// load the memory system with coherent reads.
 template <typename Cr, typename Op>
-BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, Cr init, Cr cid, Op cop,
-                      Ctrl *ctrl)
+BDEV Cr lookback_prefix(const uint64_t *status, uint32_t tile, uint32_t epoch, Cr cid, Op cop, Ctrl *ctrl)
 {
     uint32_t spins = 0;
     const uint32_t lane = lane_id();
-    if (tile == 0) {
-        if (lane == 0) lb_publish(status, 0, epoch, LB_P, cop(init, agg));
-        return init;
-    }
-    if (lane == 0) lb_publish(status, tile, epoch, LB_A, agg);
     Cr acc = cid;
     int64_t base = (int64_t)tile - 1;
     bool gave_up = false;
@@ -335,6 +345,23 @@ BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, C
         if (first < 64) break;
         base -= 64;
     }
+    return acc;
+}
+
+// Called by all 64 lanes of one wave.  Publishes the tile's aggregate,
+// obtains the exclusive prefix by look-back, publishes the tile's inclusive
+// prefix and returns its exclusive prefix (init folded in at tile 0).
+template <typename Cr, typename Op>
+BDEV Cr lookback_scan(uint64_t *status, uint32_t tile, uint32_t epoch, Cr agg, Cr init, Cr cid, Op cop,
+                      Ctrl *ctrl)
+{
+    const uint32_t lane = lane_id();
+    if (tile == 0) {
+        if (lane == 0) lb_publish(status, 0, epoch, LB_P, cop(init, agg));
+        return init;
+    }
+    if (lane == 0) lb_publish(status, tile, epoch, LB_A, agg);
+    const Cr acc = lookback_prefix<Cr>(status, tile, epoch, cid, cop, ctrl);
     if (lane == 0) lb_publish(status, tile, epoch, LB_P, cop(acc, agg));
     return acc;
 }

@@ -76,9 +76,11 @@ BDEV bool sel_cmp(C y, C c, const SelPred &p)
     return (((lt & p.mlt) | (eq & p.meq) | (gt & p.mgt)) ^ p.inv) != 0u;
 }
 
-// C's % (truncating; x % -1 == 0 without the INT_MIN overflow)
+// C's % (truncating; x % -1 == 0 without the INT_MIN overflow).  Not
+// inlined: the division sequences of a whole tile, interleaved by the
+// scheduler, would set the register budget of every predicate's kernel.
 template <typename C>
-BDEV C sel_mod(C y, C a, uint32_t pow2)
+__device__ __attribute__((noinline)) C sel_mod(C y, C a, uint32_t pow2)
 {
     if (pow2) {
         C r = y & (a - 1);
@@ -132,21 +134,36 @@ enum : int {
     kSelUnique = 5    // first element of each run of equal elements (unique_copy)
 };
 
+#ifndef BCMI_SEL_EPT4
+#define BCMI_SEL_EPT4 64  // elements per thread, types of 1, 2 and 4 bytes
+#endif
+#ifndef BCMI_SEL_EPT8
+#define BCMI_SEL_EPT8 32  // elements per thread, 8-byte types
+#endif
+#ifndef BCMI_SEL_STAGE
+#define BCMI_SEL_STAGE 32768  // LDS bytes of output staging per block
+#endif
+
 template <typename T>
 struct SelShape {
-    static constexpr int V = 16 / (int)sizeof(T);         // elements per chunk
-    static constexpr int EPT = sizeof(T) == 8 ? 16 : 32;  // elements per thread (one flag word)
-    static constexpr int Q = EPT / V;                     // chunks per thread
-    static constexpr int WSEG = Q * 64 * V;               // elements per wave
+    static constexpr int V = 16 / (int)sizeof(T);                               // elements per chunk
+    static constexpr int EPT = sizeof(T) == 8 ? BCMI_SEL_EPT8 : BCMI_SEL_EPT4;  // one flag word per thread
+    static constexpr int Q = EPT / V;                                           // chunks per thread
+    static constexpr int WSEG = Q * 64 * V;                                     // elements per wave
     static constexpr int TILE = 4 * WSEG;
+    using F = std::conditional_t<(EPT > 32), uint64_t, uint32_t>;  // flag word
 };
 
-// Index modes stage the position inside the tile (the index is formed when
-// it is written), so a tile's staging never exceeds 32 KB of LDS.
+// The tile's output is staged in LDS in ROUNDS rounds of 4 / ROUNDS waves
+// each, so the staging buffer stays within BCMI_SEL_STAGE bytes.  Index
+// modes stage the position inside the tile (the index is formed when it is
+// written).
 template <typename T, int MODE>
 struct SelLds {
     using S = std::conditional_t<MODE == kSelIdx32 || MODE == kSelIdx64, uint32_t, T>;
-    S stage[SelShape<T>::TILE];
+    static constexpr size_t BYTES = (size_t)SelShape<T>::TILE * sizeof(S);
+    static constexpr int ROUNDS = BYTES > 2 * BCMI_SEL_STAGE ? 4 : BYTES > BCMI_SEL_STAGE ? 2 : 1;
+    S stage[SelShape<T>::TILE / ROUNDS];
     uint32_t wcnt[4];
     uint32_t prefix;
     uint32_t bad;
@@ -160,25 +177,28 @@ struct alignas(16) sel_chunk {
 // Flags of the elements (bit q*V + k for chunk q, element k) under the
 // predicate FN; invalid elements get 0.
 template <typename T, int FN, typename OK>
-BDEV uint32_t sel_flags(const T (&x)[SelShape<T>::Q][SelShape<T>::V], const SelPred &p, OK ok)
+BDEV typename SelShape<T>::F sel_flags(const T (&x)[SelShape<T>::Q][SelShape<T>::V], const SelPred &p, OK ok)
 {
+    using F = typename SelShape<T>::F;
     constexpr int V = SelShape<T>::V, Q = SelShape<T>::Q;
-    uint32_t f = 0;
+    F f = 0;
 #pragma unroll
     for (int q = 0; q < Q; ++q)
 #pragma unroll
         for (int k = 0; k < V; ++k)
-            if (ok(q, k) && sel_eval<T, FN>(x[q][k], p)) f |= 1u << (q * V + k);
+            if (ok(q, k) && sel_eval<T, FN>(x[q][k], p)) f |= F(1) << (q * V + k);
     return f;
 }
 
 template <typename T, int MODE, bool FULL>
 BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n, void *out_a, void *out_b,
-                      const SelPred &p, Ctrl *ctrl, uint64_t *status, uint64_t *count_out, uint32_t epoch)
+                      const SelPred &p, Ctrl *ctrl, uint64_t *status, uint64_t *count_out, uint32_t back)
 {
     using Sh = SelShape<T>;
     using C = typename SelCmp<T>::type;
+    using F = typename Sh::F;
     constexpr int V = Sh::V, Q = Sh::Q, WSEG = Sh::WSEG, TILE = Sh::TILE;
+    constexpr int ROUNDS = SelLds<T, MODE>::ROUNDS, WPR = 4 / ROUNDS, RSEG = WPR * WSEG;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t tile = blockIdx.x;
@@ -186,19 +206,22 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
     const uint32_t wu = __builtin_amdgcn_readfirstlane(w);
     const size_t wstart = tstart + (size_t)wu * WSEG;
     const size_t end = head + n;  // aligned positions [head, end) are the range
-    // chunk q of lane l covers aligned positions wstart + q*64*V + l*V + k
-    auto pos = [&](int q, int k) { return wstart + (size_t)(q * 64 * V) + lane * V + k; };
+    // chunk q of lane l covers wave positions loc = q*64*V + l*V + k, aligned
+    // positions wstart + loc; the range's positions in the wave: [wlo, whi)
+    const T *win = abase + wstart;
+    const int32_t wlo = (int32_t)std::min<int64_t>(std::max<int64_t>((int64_t)head - (int64_t)wstart, 0), WSEG);
+    const int32_t whi = (int32_t)std::min<int64_t>(std::max<int64_t>((int64_t)end - (int64_t)wstart, 0), WSEG);
+    auto loc = [&](int q, int k) { return (int32_t)(q * 64 * V + lane * V + k); };
     auto ok = [&](int q, int k) {
         if constexpr (FULL) return true;
-        const size_t a = pos(q, k);
-        return a >= head && a < end;
+        return loc(q, k) >= wlo && loc(q, k) < whi;
     };
 
     T x[Q][V];
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
-        const T *src = abase + pos(q, 0);
-        if (FULL || (pos(q, 0) >= head && pos(q, V - 1) < end)) {
+        const T *src = win + loc(q, 0);
+        if (FULL || (loc(q, 0) >= wlo && loc(q, V - 1) < whi)) {
             const sel_chunk<T> c = *(const sel_chunk<T> *)src;
 #pragma unroll
             for (int k = 0; k < V; ++k) x[q][k] = c.v[k];
@@ -208,7 +231,7 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
         }
     }
 
-    uint32_t f;
+    F f;
     if constexpr (MODE == kSelUnique) {
         // The element before the wave's first position.  In place (out ==
         // in) it is still the input value: a tile before this one writes an
@@ -228,8 +251,8 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
 #pragma unroll
             for (int k = 0; k < V; ++k) {
                 const T pv = k == 0 ? prev0 : x[q][k - 1];
-                const bool first = pos(q, k) == head;
-                if (ok(q, k) && (first || !(x[q][k] == pv))) f |= 1u << (q * V + k);
+                const bool first = loc(q, k) == wlo && wstart + wlo == head;
+                if (ok(q, k) && (first || !(x[q][k] == pv))) f |= F(1) << (q * V + k);
             }
         }
     } else if constexpr (std::is_floating_point<T>::value) {
@@ -254,8 +277,8 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
     uint32_t running = 0;
 #pragma unroll
     for (int q = 0; q < Q; q += 2) {
-        const uint32_t c0 = __builtin_popcount((f >> (q * V)) & CM);
-        const uint32_t c1 = q + 1 < Q ? __builtin_popcount((f >> ((q + 1) * V)) & CM) : 0u;
+        const uint32_t c0 = __builtin_popcount((uint32_t)(f >> (q * V)) & CM);
+        const uint32_t c1 = q + 1 < Q ? __builtin_popcount((uint32_t)(f >> ((q + 1) * V)) & CM) : 0u;
         const uint32_t packed = c0 | (c1 << 16);
         const uint32_t inc = wave_inclusive_scan(packed, 0u, [](uint32_t a, uint32_t b) { return a + b; });
         const uint32_t exc = inc - packed;
@@ -278,6 +301,7 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
     const uint32_t vcount = (uint32_t)(vhi_abs - vlo_abs);
     const uint32_t vlo = (uint32_t)(vlo_abs - tstart);
     if (w == 0) {
+        const uint32_t epoch = launch_epoch_after(ctrl, back, f);
         const uint32_t prefix = lookback_scan<uint32_t>(status, tile, epoch, cnt, 0u, 0u,
                                                         [](uint32_t a, uint32_t b) { return a + b; }, ctrl);
         if (lane == 0) {
@@ -290,60 +314,81 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
             if (tile == gridDim.x - 1) *count_out = (uint64_t)prefix + cnt;
         }
     }
-    // stage the output in LDS in input order
+    // stage the output in LDS in input order, one round of WPR waves at a
+    // time, and write each round as contiguous runs
+    constexpr bool PART = MODE == kSelPart2 || MODE == kSelPartRev;
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        uint32_t r = wpre + lrank[q];
+    for (int h = 0; h < ROUNDS; ++h) {
+        uint32_t rh = 0, ch = 0;  // selected before the round / in the round
+        for (int i = 0; i < 4; ++i) {
+            if (i < h * WPR) rh += L.wcnt[i];
+            else if (i < (h + 1) * WPR) ch += L.wcnt[i];
+        }
+        // valid positions before the round and in it, relative to vlo
+        const int64_t lo = (int64_t)h * RSEG - vlo, hi = (int64_t)(h + 1) * RSEG - vlo;
+        const uint32_t vb = (uint32_t)std::min<int64_t>(std::max<int64_t>(lo, 0), vcount);
+        const uint32_t vh = (uint32_t)std::min<int64_t>(std::max<int64_t>(hi, 0), vcount) - vb;
+        const uint32_t uh = vb - rh;  // unselected before the round
+        if ((int)(w / WPR) == h) {
 #pragma unroll
-        for (int k = 0; k < V; ++k) {
-            const bool sel = (f >> (q * V + k)) & 1u;
-            const uint32_t e = (uint32_t)(pos(q, k) - tstart);  // position inside the tile
-            if constexpr (MODE == kSelPart2 || MODE == kSelPartRev) {
-                if (ok(q, k)) L.stage[sel ? r : cnt + (e - vlo - r)] = x[q][k];
-            } else if (sel) {
-                if constexpr (MODE == kSelIdx32 || MODE == kSelIdx64) L.stage[r] = e;
-                else L.stage[r] = x[q][k];
+            for (int q = 0; q < Q; ++q) {
+                uint32_t r = wpre + lrank[q];
+#pragma unroll
+                for (int k = 0; k < V; ++k) {
+                    const bool sel = (f >> (q * V + k)) & 1u;
+                    const uint32_t e = wu * WSEG + (uint32_t)loc(q, k);  // position inside the tile
+                    if constexpr (PART) {
+                        if (ok(q, k)) L.stage[sel ? r - rh : ch + (e - vlo - r) - uh] = x[q][k];
+                    } else if (sel) {
+                        if constexpr (MODE == kSelIdx32 || MODE == kSelIdx64) L.stage[r - rh] = e;
+                        else L.stage[r - rh] = x[q][k];
+                    }
+                    r += sel ? 1u : 0u;
+                }
             }
-            r += sel ? 1u : 0u;
         }
-    }
-    __syncthreads();
-    if (L.bad) return;
-    const uint32_t prefix = L.prefix;
-    if constexpr (MODE == kSelPart2 || MODE == kSelPartRev) {
-        const size_t fbefore = (vlo_abs - head) - prefix;  // unselected elements before the tile
-        T *ot = (T *)out_a;
-        T *of = (T *)out_b;
-        for (uint32_t i = tid; i < vcount; i += kSelThreads) {
-            const T v = L.stage[i];
-            if (i < cnt) ot[(size_t)prefix + i] = v;
-            else if constexpr (MODE == kSelPart2) of[fbefore + (i - cnt)] = v;
-            else ot[n - 1 - (fbefore + (i - cnt))] = v;
+        __syncthreads();
+        if (L.bad) return;
+        const uint32_t prefix = L.prefix + rh;  // output index of the round's first selected element
+        if constexpr (PART) {
+            const size_t fbefore = (vlo_abs - head) - L.prefix + uh;  // unselected elements before the round
+            T *ot = (T *)out_a;
+            T *of = (T *)out_b;
+            for (uint32_t i = tid; i < vh; i += kSelThreads) {
+                const T v = L.stage[i];
+                if (i < ch) ot[(size_t)prefix + i] = v;
+                else if constexpr (MODE == kSelPart2) of[fbefore + (i - ch)] = v;
+                else ot[n - 1 - (fbefore + (i - ch))] = v;
+            }
+        } else if constexpr (MODE == kSelIdx32 || MODE == kSelIdx64) {
+            using I = std::conditional_t<MODE == kSelIdx32, uint32_t, uint64_t>;
+            I *o = (I *)out_a;
+            const size_t ibase = tstart - head;  // index of tile position 0
+            for (uint32_t i = tid; i < ch; i += kSelThreads) o[(size_t)prefix + i] = (I)(ibase + L.stage[i]);
+        } else {
+            T *o = (T *)out_a;
+            for (uint32_t i = tid; i < ch; i += kSelThreads) o[(size_t)prefix + i] = L.stage[i];
         }
-    } else if constexpr (MODE == kSelIdx32 || MODE == kSelIdx64) {
-        using I = std::conditional_t<MODE == kSelIdx32, uint32_t, uint64_t>;
-        I *o = (I *)out_a;
-        const size_t ibase = tstart - head;  // index of tile position 0
-        for (uint32_t i = tid; i < cnt; i += kSelThreads) o[(size_t)prefix + i] = (I)(ibase + L.stage[i]);
-    } else {
-        T *o = (T *)out_a;
-        for (uint32_t i = tid; i < cnt; i += kSelThreads) o[(size_t)prefix + i] = L.stage[i];
+        if (h + 1 < ROUNDS) __syncthreads();
     }
 }
 
+#ifndef BCMI_SEL_WPE
+#define BCMI_SEL_WPE 4  // minimum waves per SIMD the register allocation must allow
+#endif
+
 template <typename T, int MODE>
-__global__ __launch_bounds__(kSelThreads) void k_select(const T *abase, size_t head, size_t n, void *out_a,
+__global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(BCMI_SEL_WPE))) void k_select(const T *abase, size_t head, size_t n, void *out_a,
                                                         void *out_b, SelPred p, Ctrl *ctrl, uint64_t *status,
                                                         uint64_t *count_out, uint32_t back)
 {
     constexpr size_t TILE = SelShape<T>::TILE;
     __shared__ SelLds<T, MODE> lds;
-    const uint32_t epoch = launch_epoch(ctrl, back);
     const size_t tstart = (size_t)blockIdx.x * TILE;
     if (tstart >= head && tstart + TILE <= head + n)
-        select_body<T, MODE, true>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, epoch);
+        select_body<T, MODE, true>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, back);
     else
-        select_body<T, MODE, false>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, epoch);
+        select_body<T, MODE, false>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, back);
 }
 
 // stable_partition, pass 2: out[i] = tmp[i] below the selected count, and

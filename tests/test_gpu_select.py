@@ -12,8 +12,9 @@ pytestmark = pytest.mark.gpu
 
 RNG = np.random.default_rng(4242)
 ALL = ["int8", "uint8", "int16", "uint16", "int32", "uint32", "int64", "uint64", "float32", "float64"]
-# sizes around the tile boundaries: 8192 elements per tile (4096 for 8-byte types)
-SIZES = [1, 2, 31, 32, 33, 4095, 4096, 4097, 8191, 8192, 8193, 3 * 8192 + 5, 100003]
+# sizes around the tile boundaries (16384 elements per tile, 8192 for 8-byte
+# types) and the staging rounds inside a tile (every 8192 positions)
+SIZES = [1, 2, 31, 33, 4095, 4097, 8191, 8192, 8193, 16383, 16384, 16385, 3 * 16384 + 5, 100003]
 
 
 def dev(q, a):
@@ -166,7 +167,7 @@ def test_unique_in_place_runs_across_tiles(queue):
     n = 6 * 8192 + 11
     x = np.arange(n, dtype=np.int32)
     x[1] = x[0]
-    for b in range(8192, n, 8192):
+    for b in range(8192, n, 8192):  # tile and staging-round boundaries
         x[b] = x[b - 1]
         x[b + 1] = x[b - 1]
     want = O.unique_copy(x)
