@@ -40,12 +40,36 @@ EXPORTS = [
     "bcmi_segmented_scan", "bcmi_graph_begin", "bcmi_graph_end", "bcmi_graph_launch",
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
-    "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition",
+    "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
 FN = {"id": 0, "mul": 1, "mod": 2, "and": 3, "signbit": 4}
 CMP = {"<": 0, "<=": 1, ">": 2, ">=": 3, "==": 4, "!=": 5}
+
+
+MAP_FN = {"id": 0, "abs": 1, "square": 2}
+MAP_OP = {"plus": PLUS, "minus": MINUS, "multiplies": MULTIPLIES}
+
+
+class Seq(C.Structure):
+    """bcmi_seq: a device array, or a counting sequence start + i (ptr NULL)."""
+    _fields_ = [("ptr", C.c_void_p), ("start", C.c_int64)]
+
+
+class Map(C.Structure):
+    """bcmi_map: y = g(a op b) (binary) or g(a - c) (unary)."""
+    _fields_ = [("op", C.c_int32), ("fn", C.c_int32), ("offset", C.c_uint64)]
+
+
+def mapping(dtype, fn="id", op="multiplies", c=0):
+    raw = np.array([c]).astype(np.dtype(dtype)).tobytes()
+    return Map(MAP_OP[op], MAP_FN[fn], int.from_bytes(raw.ljust(8, b"\0"), "little"))
+
+
+def counting(start):
+    """A counting sequence (make_counting_iterator(start))."""
+    return Seq(None, int(start))
 
 
 class Predicate(C.Structure):
@@ -140,6 +164,8 @@ def lib():
         L.bcmi_count_if.argtypes = [vp, i, vp, sz, pp, vp]
         L.bcmi_unique.argtypes = [vp, i, vp, vp, sz, vp]
         L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
+        L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,
+                                            C.POINTER(C.c_double), i]
         _lib = L
     return _lib
 
@@ -373,6 +399,31 @@ def partition(q, inp, out_true, out_false, pred, n=None, count=None):
     check(lib().bcmi_partition(q.h, DTYPE[inp.dtype], _p(inp), _p(out_true), _p(out_false), n,
                                C.byref(pred), cnt.ptr))
     return cnt if count is not None else int(cnt.to_host(q)[0])
+
+
+# ---------------------------------------------- transform_reduce / inner_product
+def _as_seq(x, at=0):
+    if isinstance(x, Seq):
+        return x
+    return Seq(x.ptr.value + at * x.dtype.itemsize if at else x.ptr.value, 0)
+
+
+def transform_reduce(q, dtype, a, b, n, m, out, reduce_op=PLUS, init=None, precision=COMPENSATED):
+    """out[0] = init op y_0 op ... op y_{n-1}, y = m(a[i], b[i]) (b None: unary).
+    a and b are DeviceArrays or counting(start).  Returns the FP64 value
+    (a DeviceArray of one double) for float sums."""
+    dt = np.dtype(dtype)
+    sa = _as_seq(a)
+    sb = _as_seq(b) if b is not None else None
+    ini = None
+    if init is not None:
+        ini_arr = np.array([init], dt)
+        ini = ini_arr.ctypes.data_as(C.c_void_p)
+    f64 = DeviceArray(1, np.float64, q)
+    check(lib().bcmi_transform_reduce(q.h, DTYPE[dt], C.byref(sa), C.byref(sb) if sb is not None else None, n,
+                                      C.byref(m), reduce_op, ini, _p(out),
+                                      C.cast(f64.ptr, C.POINTER(C.c_double)), precision))
+    return f64
 
 
 def segmented_sort(q, keys, values, offsets_host, descending=False):

@@ -12,117 +12,13 @@
 // partial, and a one-block kernel folds the partials in FP64 in a fixed
 // order (so the result is deterministic).  Integers wrap exactly;
 // min/max/multiplies fold in the element type.
-#include "device.hpp"
-#include "ops.hpp"
+#include "reduce.hpp"
 
 #include <cstring>
 #include <limits>
 #include <type_traits>
 
 namespace bcmi {
-
-constexpr int kRedThreads = 256;
-
-// Exact policy: fold in the element type (integers; min/max; and the naive
-// float tree used only for error comparison).
-template <typename T, int OP>
-struct PlainPolicy {
-    using State = T;
-    using Part = T;
-    static BDEV State id() { return op_identity<T, OP>(); }
-    static BDEV State add(State s, T x) { return apply_op<T, OP>(s, x); }
-    static BDEV State merge(State a, State b) { return apply_op<T, OP>(a, b); }
-    static BDEV Part part(State s) { return s; }
-    static BDEV Part part_id() { return id(); }
-    static BDEV Part part_merge(Part a, Part b) { return merge(a, b); }
-    static BDEV T finish(Part p, bool has_init, T init) { return has_init ? apply_op<T, OP>(init, p) : p; }
-    static BDEV double f64(Part p) { return (double)p; }
-};
-
-// Compensated float plus: TwoSum pairs in FP32, FP64 partials.
-struct CompF32Policy {
-    using State = cf32;
-    using Part = double;
-    static BDEV State id() { return cf32{0.f, 0.f}; }
-    static BDEV State add(State s, float x) { return cf_add(s, x); }
-    static BDEV State merge(State a, State b) { return cf_merge(a, b); }
-    static BDEV Part part(State s) { return cf_value(s); }
-    static BDEV Part part_id() { return 0.0; }
-    static BDEV Part part_merge(Part a, Part b) { return a + b; }
-    static BDEV float finish(Part p, bool has_init, float init) { return (float)(has_init ? (double)init + p : p); }
-    static BDEV double f64(Part p) { return p; }
-};
-
-// Compensated double plus: TwoSum in FP64.
-struct cf64 {
-    double s, c;
-};
-struct CompF64Policy {
-    using State = cf64;
-    using Part = cf64;
-    static BDEV void ts(double a, double b, double &s, double &e)
-    {
-        s = a + b;
-        double bp = s - a;
-        e = (a - (s - bp)) + (b - bp);
-    }
-    static BDEV State id() { return cf64{0.0, 0.0}; }
-    static BDEV State add(State a, double x)
-    {
-        double s, e;
-        ts(a.s, x, s, e);
-        return cf64{s, a.c + e};
-    }
-    static BDEV State merge(State a, State b)
-    {
-        double s, e;
-        ts(a.s, b.s, s, e);
-        return cf64{s, (a.c + b.c) + e};
-    }
-    static BDEV Part part(State s) { return s; }
-    static BDEV Part part_id() { return id(); }
-    static BDEV Part part_merge(Part a, Part b) { return merge(a, b); }
-    static BDEV double finish(Part p, bool has_init, double init)
-    {
-        if (has_init) p = add(p, init);
-        return p.s + p.c;
-    }
-    static BDEV double f64(Part p) { return p.s + p.c; }
-};
-
-// 16-byte-chunk view of the input
-template <typename T>
-struct alignas(16) chunk16 {
-    static constexpr int N = 16 / sizeof(T);
-    T v[N];
-};
-
-// DPP moves 4- or 8-byte values; 16-byte states (cf64) go through LDS.
-template <typename S, typename P>
-BDEV S block_combine(S v, S *s_buf)
-{
-    const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
-    if constexpr (sizeof(S) == 4 || sizeof(S) == 8) {
-        v = wave_reduce(v, P::id(), [](S a, S b) { return P::merge(a, b); });
-        if (lane == 0) s_buf[w] = v;
-        __syncthreads();
-        if (threadIdx.x == 0) {
-            S acc = s_buf[0];
-            for (int i = 1; i < kRedThreads / 64; ++i) acc = P::merge(acc, s_buf[i]);
-            s_buf[0] = acc;
-        }
-        __syncthreads();
-        return s_buf[0];
-    } else {
-        s_buf[threadIdx.x] = v;
-        __syncthreads();
-        for (int off = kRedThreads / 2; off > 0; off >>= 1) {
-            if ((int)threadIdx.x < off) s_buf[threadIdx.x] = P::merge(s_buf[threadIdx.x], s_buf[threadIdx.x + off]);
-            __syncthreads();
-        }
-        return s_buf[0];
-    }
-}
 
 // U chunks of 16 bytes in flight per thread and iteration.
 template <typename T, typename P, int U>
@@ -158,30 +54,6 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce(const T *__restrict__ in
     }
     S blk = block_combine<S, P>(acc, s_buf);
     if (threadIdx.x == 0) partials[blockIdx.x] = P::part(blk);
-}
-
-// One block folds the per-block partials in FP64 / the element type in a
-// fixed order, so the result is deterministic.
-template <typename T, typename P>
-__global__ __launch_bounds__(kRedThreads) void k_reduce_final(const typename P::Part *__restrict__ partials,
-                                                              uint32_t nparts, T init, int has_init, T *out,
-                                                              double *out_f64)
-{
-    using Pt = typename P::Part;
-    __shared__ Pt s_part[kRedThreads];
-    Pt p = P::part_id();
-    for (uint32_t i = threadIdx.x; i < nparts; i += kRedThreads) p = P::part_merge(p, partials[i]);
-    s_part[threadIdx.x] = p;
-    __syncthreads();
-    for (int off = kRedThreads / 2; off > 0; off >>= 1) {
-        if ((int)threadIdx.x < off) s_part[threadIdx.x] = P::part_merge(s_part[threadIdx.x], s_part[threadIdx.x + off]);
-        __syncthreads();
-    }
-    if (threadIdx.x == 0) {
-        Pt total = s_part[0];
-        *out = P::finish(total, has_init != 0, init);
-        if (out_f64) *out_f64 = P::f64(total) + (has_init ? (double)init : 0.0);
-    }
 }
 
 template <typename T, typename P>

@@ -278,6 +278,44 @@ int bcmi_partition(bcmi_queue queue, int dtype, const void *in,
                    void *out_true, void *out_false, size_t n,
                    const bcmi_predicate *pred, uint64_t *count_out);
 
+/* ------------------------------- transform_reduce / inner_product */
+/* An input sequence: a device array, or a counting sequence
+ * (make_counting_iterator, iterator/counting_iterator.hpp: element i is
+ * start + i, computed in the element type). */
+typedef struct bcmi_seq {
+    const void *ptr; /* device pointer of element 0; NULL: a counting sequence */
+    int64_t start;   /* first value of a counting sequence */
+} bcmi_seq;
+
+/* The element maps the dispatch recognises.  The reference feeds reduce
+ * through transform_iterator / zip_iterator with any function object
+ * (transform_reduce.hpp, inner_product.hpp); here
+ *     binary (b given):  y = g(a[i] op b[i]),  op = BCMI_PLUS, BCMI_MINUS or
+ *                        BCMI_MULTIPLIES  (inner_product: op = multiplies)
+ *     unary  (b NULL):   y = g(a[i] - c)      (c = 0: y = g(a[i]))
+ * with g = identity, abs or square, evaluated in the element type (integers
+ * wrap).  c holds one element in its low bytes. */
+enum bcmi_map_fn { BCMI_MAP_ID = 0, BCMI_MAP_ABS = 1, BCMI_MAP_SQUARE = 2 };
+typedef struct bcmi_map {
+    int32_t op;      /* binary combine (ignored when unary) */
+    int32_t fn;      /* g: bcmi_map_fn */
+    uint64_t offset; /* c */
+} bcmi_map;
+
+/* transform_reduce (transform_reduce.hpp) and inner_product
+ * (inner_product.hpp): *out (device, one element) = init op_r y[0] op_r
+ * ... op_r y[n-1] (init NULL: no init, i.e. transform_reduce), with op_r =
+ * reduce_op (plus, multiplies, min, max).  Float plus with
+ * BCMI_PRECISION_COMPENSATED sums in error-free pairs and keeps the
+ * rounding error of a final product or square exactly (fma), with an FP64
+ * combine across blocks; out_f64 (may be NULL) receives that FP64 value.
+ * The reference accumulates a float inner product in one work-item
+ * (accumulate.hpp:62-69); this is parallel (divergence A5). */
+int bcmi_transform_reduce(bcmi_queue queue, int dtype, const bcmi_seq *a,
+                          const bcmi_seq *b, size_t n, const bcmi_map *map,
+                          int reduce_op, const void *init, void *out,
+                          double *out_f64, int precision);
+
 /* --------------------------------------------------------- hipGraphs */
 /* Capture everything enqueued on the queue between begin and end into a
  * replayable graph (queue scratch must be reserved beforehand). */

@@ -63,6 +63,20 @@ def pred(dtype, cmp, c, fn="id", a=0, negate=False):
     return OrcPredicate(FN[fn], CMP[cmp], int(bool(negate)), 0, bits(a), bits(c))
 
 
+class OrcMap(C.Structure):
+    """y = g(a op b) or g(a - c) (orc_map in bcmi_oracle.h)."""
+    _fields_ = [("op", C.c_int32), ("fn", C.c_int32), ("offset", C.c_uint64)]
+
+
+MAP_FN = {"id": 0, "abs": 1, "square": 2}
+MAP_OP = {"plus": 0, "multiplies": 1, "minus": 4}
+
+
+def mapping(dtype, fn="id", op="multiplies", c=0):
+    raw = np.array([c]).astype(dtype).tobytes().ljust(8, b"\0")
+    return OrcMap(MAP_OP[op], MAP_FN[fn], int.from_bytes(raw, "little"))
+
+
 def build():
     import subprocess
     subprocess.check_call(["make", "-s", "-B", "-C", _HERE])
@@ -128,6 +142,10 @@ def lib():
         L.orc_partition_copy.restype = sz
         L.orc_stable_partition.argtypes = [i, vp, sz, pp]
         L.orc_stable_partition.restype = sz
+        pm = C.POINTER(OrcMap)
+        L.orc_map_reduce.argtypes = [i, vp, C.c_int64, vp, C.c_int64, i, sz, pm, i, vp, vp]
+        L.orc_map_sum_dd.argtypes = [i, vp, C.c_int64, vp, C.c_int64, i, sz, pm, C.POINTER(C.c_double)]
+        L.orc_map_sum_dd.restype = C.c_double
         _lib = L
     return _lib
 
@@ -388,3 +406,31 @@ def stable_partition(x, p):
     d = np.ascontiguousarray(x).copy()
     k = lib().orc_stable_partition(NP2CODE[d.dtype], _p(d), d.size, C.byref(p))
     return d, int(k)
+
+
+# ------------------------------------------------- transform_reduce
+def _seq(x):
+    """(pointer, start) of an array or of a counting sequence ('count', start)."""
+    if isinstance(x, tuple):
+        return None, int(x[1])
+    return _p(x), 0
+
+
+def map_reduce(dtype, a, b, n, m, rop, init=None):
+    dt = np.dtype(dtype)
+    (pa, a0), (pb, b0) = _seq(a), (_seq(b) if b is not None else (None, 0))
+    out = np.zeros(1, dt)
+    ini = None
+    if init is not None:
+        ini_arr = np.array([init], dt)
+        ini = _p(ini_arr)
+    lib().orc_map_reduce(NP2CODE[dt], pa, a0, pb, b0, int(b is not None), n, C.byref(m), rop, ini, _p(out))
+    return out[0]
+
+
+def map_sum_dd(dtype, a, b, n, m):
+    dt = np.dtype(dtype)
+    (pa, a0), (pb, b0) = _seq(a), (_seq(b) if b is not None else (None, 0))
+    lo = C.c_double()
+    hi = lib().orc_map_sum_dd(NP2CODE[dt], pa, a0, pb, b0, int(b is not None), n, C.byref(m), C.byref(lo))
+    return hi, lo.value

@@ -1063,3 +1063,116 @@ size_t orc_stable_partition(int dtype, void *data, size_t n, const orc_predicate
     free(tmp);
     return t;
 }
+
+/* ======================================================================
+ * transform_reduce (transform_reduce.hpp: reduce over
+ * make_transform_iterator(first, f), or over a zip_iterator of two ranges)
+ * and inner_product (inner_product.hpp: accumulate over the products of a
+ * zipped pair; its general form transforms into a temporary and
+ * accumulates).  The element map is evaluated in the element type, one
+ * rounding (or wrap) per operation, as the printed OpenCL C expression is.
+ * counting_iterator (iterator/counting_iterator.hpp) yields init + i.
+ * ==================================================================== */
+static num_t map_elem(int dt, const void *p, int64_t start, size_t i)
+{
+    if (p) return ld(dt, p, i);
+    num_t v; v.u = 0;
+    if (dt == ORC_F32) v.f = (float)start + (float)i;
+    else if (dt == ORC_F64) v.d = (double)start + (double)i;
+    else {
+        int W = dt_size(dt) * 8;
+        uint64_t mask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+        uint64_t u = ((uint64_t)start + (uint64_t)i) & mask;
+        if (is_signed_dt(dt) && W < 64 && (u >> (W - 1))) u |= ~mask;  /* sign-extend */
+        v.u = u;
+    }
+    return v;
+}
+
+static num_t map_sub_(int dt, num_t a, num_t b)
+{
+    num_t r; r.u = 0;
+    if (dt == ORC_F32) { r.f = a.f - b.f; return r; }
+    if (dt == ORC_F64) { r.d = a.d - b.d; return r; }
+    int W = dt_size(dt) * 8;
+    uint64_t mask = (W == 64) ? ~0ull : ((1ull << W) - 1);
+    uint64_t u = (a.u - b.u) & mask;
+    if (is_signed_dt(dt) && W < 64 && (u >> (W - 1))) u |= ~mask;
+    r.u = u;
+    return r;
+}
+
+static num_t map_abs_(int dt, num_t a)
+{
+    num_t r = a;
+    if (dt == ORC_F32) { r.f = fabsf(a.f); return r; }
+    if (dt == ORC_F64) { r.d = fabs(a.d); return r; }
+    if (is_signed_dt(dt) && a.i < 0) {
+        num_t z; z.u = 0;
+        return map_sub_(dt, z, a);  /* wraps: abs(INT_MIN) keeps INT_MIN's bits */
+    }
+    return r;
+}
+
+/* the value g is applied to: a op b, or a - c */
+static num_t map_inner(int dt, num_t a, num_t b, int binary, const orc_map *m)
+{
+    if (!binary) return map_sub_(dt, a, ld(dt, &m->offset, 0));  /* c: one element in the low bytes */
+    if (m->op == 4) return map_sub_(dt, a, b);
+    return apply(dt, m->op == ORC_MULTIPLIES ? ORC_MULTIPLIES : ORC_PLUS, a, b);
+}
+
+void orc_map_reduce(int dtype, const void *a, int64_t a_start, const void *b, int64_t b_start, int binary, size_t n,
+                    const orc_map *m, int rop, const void *init, void *out)
+{
+    int have = 0;
+    num_t acc; acc.u = 0;
+    if (init) { acc = ld(dtype, init, 0); have = 1; }
+    for (size_t i = 0; i < n; ++i) {
+        num_t x = map_elem(dtype, a, a_start, i);
+        num_t y = binary ? map_elem(dtype, b, b_start, i) : x;
+        num_t t;
+        if (binary && m->op == ORC_MULTIPLIES && m->fn == ORC_MAP_ABS)
+            t = apply(dtype, ORC_MULTIPLIES, map_abs_(dtype, x), map_abs_(dtype, y));
+        else {
+            t = map_inner(dtype, x, y, binary, m);
+            if (m->fn == ORC_MAP_ABS) t = map_abs_(dtype, t);
+            else if (m->fn == ORC_MAP_SQUARE) t = apply(dtype, ORC_MULTIPLIES, t, t);
+        }
+        acc = have ? apply(dtype, rop, acc, t) : t;
+        have = 1;
+    }
+    if (have) st(dtype, out, 0, acc);
+}
+
+/* exact product of two floats of the element type as a double-double */
+static void prod_dd(int dt, num_t x, num_t y, double *hi, double *lo)
+{
+    if (dt == ORC_F32) { *hi = (double)x.f * (double)y.f; *lo = 0.0; return; }  /* exact in double */
+    *hi = x.d * y.d;
+    *lo = fma(x.d, y.d, -*hi);
+}
+
+double orc_map_sum_dd(int dtype, const void *a, int64_t a_start, const void *b, int64_t b_start, int binary,
+                      size_t n, const orc_map *m, double *lo_out)
+{
+    double hi = 0.0, lo = 0.0;
+    for (size_t i = 0; i < n; ++i) {
+        num_t x = map_elem(dtype, a, a_start, i);
+        num_t y = binary ? map_elem(dtype, b, b_start, i) : x;
+        double ph, pl;
+        if (binary && m->op == ORC_MULTIPLIES && m->fn != ORC_MAP_SQUARE) {
+            if (m->fn == ORC_MAP_ABS) { x = map_abs_(dtype, x); y = map_abs_(dtype, y); }
+            prod_dd(dtype, x, y, &ph, &pl);
+        } else {
+            num_t t = map_inner(dtype, x, y, binary, m);
+            if (m->fn == ORC_MAP_ABS) t = map_abs_(dtype, t);
+            if (m->fn == ORC_MAP_SQUARE) prod_dd(dtype, t, t, &ph, &pl);
+            else { ph = dtype == ORC_F32 ? (double)t.f : t.d; pl = 0.0; }
+        }
+        dd_add(&hi, &lo, ph);
+        dd_add(&hi, &lo, pl);
+    }
+    *lo_out = lo;
+    return hi;
+}

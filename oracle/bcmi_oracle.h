@@ -127,6 +127,24 @@ size_t orc_partition_copy(int dtype, const void *in, size_t n,
                           const orc_predicate *p, void *out_true, void *out_false);
 size_t orc_stable_partition(int dtype, void *data, size_t n, const orc_predicate *p);
 
+/* --------------------------------------- transform_reduce / inner_product --
+ * y_i = g(a_i op b_i) (binary) or g(a_i - c) (unary), g = id | abs | square,
+ * op = plus | minus | multiplies (the ORC_ op codes, minus = 4), each step in
+ * the element type.  A NULL array is a counting sequence start + i. */
+enum { ORC_MAP_ID = 0, ORC_MAP_ABS = 1, ORC_MAP_SQUARE = 2 };
+typedef struct {
+    int32_t op, fn;
+    uint64_t offset;
+} orc_map;
+
+/* init op_r y_0 op_r ... (left fold in the element type; init NULL: none) */
+void   orc_map_reduce(int dtype, const void *a, int64_t a_start, const void *b, int64_t b_start, int binary,
+                      size_t n, const orc_map *m, int rop, const void *init, void *out);
+/* floats: sum of the terms with each final product or square exact, in
+ * double-double; returns hi, *lo the low part */
+double orc_map_sum_dd(int dtype, const void *a, int64_t a_start, const void *b, int64_t b_start, int binary,
+                      size_t n, const orc_map *m, double *lo);
+
 #ifdef __cplusplus
 }
 #endif

@@ -331,6 +331,19 @@ case("count_if_odd", "test/test_count.cpp:202", "count_if", "int32", input=IOTA2
 case("count_if_with_reduce", "test/test_count.cpp:217", "count_if", "int32", input=IOTA2K,
      pred=pred(">", 1024), expected=1023)
 
+# ----------------------------------------------- transform_reduce / inner_product
+# {"op": binary combine, "fn": g, "c": unary offset}; a "count" input is
+# make_counting_iterator(start).  The float4 length case
+# (test/test_transform_reduce.cpp:42) is on a vector type and is not transcribed.
+case("transform_reduce_sum_abs_int", "test/test_transform_reduce.cpp:39", "transform_reduce", "int32",
+     a=[1, -2, -3, -4, 5], map={"fn": "abs", "c": 0}, reduce="plus", expected=15)
+case("transform_reduce_variance", "test/test_transform_reduce.cpp:88", "transform_reduce", "float32",
+     a=[1, 2, 3, 4, 5, 6, 7, 8, 9, 10], map={"fn": "square", "c": 5.5}, reduce="plus", expected=82.5, rtol=1e-4)
+case("inner_product_int", "test/test_inner_product.cpp:36", "inner_product", "int32", a=[1, 2, 3, 4],
+     b=[10, 20, 30, 40], init=0, expected=300)
+case("inner_product_counting_iterator", "test/test_inner_product.cpp:41", "inner_product", "int32",
+     a={"count": 0, "n": 100}, b={"count": 0}, init=0, expected=328350)
+
 if __name__ == "__main__":
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.json")
     with open(out, "w") as f:

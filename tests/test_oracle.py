@@ -586,3 +586,92 @@ def test_selection_matches_numpy(dt):
         head = np.ones(n, bool)
         head[1:] = x[1:] != x[:-1]  # != on floats: NaN heads its own run, -0 == +0
         assert _bits_equal(O.unique_copy(x), x[head]), n
+
+
+# ------------------------------------------------- transform_reduce
+def _seq_of(spec, dtype):
+    if isinstance(spec, dict):
+        return ("count", spec["count"])
+    return np.array(spec, dtype)
+
+
+def test_golden_transform_reduce(golden):
+    seen = set()
+    for c in golden:
+        if c["algo"] not in ("transform_reduce", "inner_product"):
+            continue
+        seen.add(c["algo"])
+        dt = np.dtype(c["dtype"])
+        a = _seq_of(c["a"], dt)
+        n = c["a"]["n"] if isinstance(c["a"], dict) else len(c["a"])
+        if c["algo"] == "inner_product":
+            got = O.map_reduce(dt, a, _seq_of(c["b"], dt), n, O.mapping(dt), O.PLUS, c["init"])
+            assert got == c["expected"], c["id"]
+        else:
+            m = O.mapping(dt, c["map"]["fn"], c=c["map"]["c"])
+            if dt.kind == "f":
+                hi, lo = O.map_sum_dd(dt, a, None, n, m)
+                assert abs(hi + lo - c["expected"]) <= c["rtol"] * abs(c["expected"]), c["id"]
+            else:
+                assert O.map_reduce(dt, a, None, n, m, O.PLUS) == c["expected"], c["id"]
+    assert seen == {"transform_reduce", "inner_product"}
+
+
+def _wrapv(v, dt):
+    bits = np.dtype(dt).itemsize * 8
+    v &= (1 << bits) - 1
+    return v - (1 << bits) if np.dtype(dt).kind == "i" and v >> (bits - 1) else v
+
+
+@pytest.mark.parametrize("dt", ["int8", "uint16", "int32", "uint32", "int64", "uint64"])
+def test_map_reduce_integers_wrap_like_python(dt):
+    info = np.iinfo(dt)
+    a = RNG.integers(info.min, info.max, 300, endpoint=True, dtype=dt)
+    b = RNG.integers(info.min, info.max, 300, endpoint=True, dtype=dt)
+    a[:3] = [info.min, info.max, 0]
+    cases = [("multiplies", "id", 0), ("minus", "abs", 0), ("plus", "square", 0), (None, "abs", 0),
+             (None, "square", 7), (None, "id", -3 if info.min < 0 else 3)]
+    for op, fn, cc in cases:
+        m = O.mapping(dt, fn, op or "plus", cc)
+        for rop, fold in ((O.PLUS, lambda s, t: s + t), (O.MULTIPLIES, lambda s, t: s * t),
+                          (O.MIN, min), (O.MAX, max)):
+            ys = []
+            for x, y in zip(a.tolist(), b.tolist()):
+                if op is None:
+                    t = _wrapv(x - _wrapv(cc, dt), dt)
+                elif op == "multiplies":
+                    t = _wrapv(x * y, dt)
+                elif op == "minus":
+                    t = _wrapv(x - y, dt)
+                else:
+                    t = _wrapv(x + y, dt)
+                if fn == "abs":
+                    t = _wrapv(-t, dt) if t < 0 else t  # abs(INT_MIN) keeps INT_MIN
+                elif fn == "square":
+                    t = _wrapv(t * t, dt)
+                ys.append(t)
+            want = ys[0]
+            for t in ys[1:]:
+                want = _wrapv(fold(want, t), dt)
+            got = O.map_reduce(dt, a, b if op else None, a.size, m, rop)
+            assert int(got) == want, (op, fn, cc, rop)
+
+
+@pytest.mark.parametrize("dt", ["float32", "float64"])
+def test_map_sum_dd_is_the_exact_sum_of_exact_products(dt):
+    a = (RNG.normal(size=200) * 10.0 ** RNG.integers(-3, 4, 200)).astype(dt)
+    b = (RNG.normal(size=200) * 10.0 ** RNG.integers(-3, 4, 200)).astype(dt)
+    f = Fraction
+    exact = sum(f(float(x)) * f(float(y)) for x, y in zip(a, b))
+    hi, lo = O.map_sum_dd(dt, a, b, a.size, O.mapping(dt))
+    assert f(hi) + f(lo) - exact <= abs(exact) * f(2) ** -100 + f(2) ** -1070
+    assert abs(float(f(hi) + f(lo)) - float(exact)) <= abs(float(exact)) * 2 ** -52
+    # squares of (x - c), the difference rounded in the element type first
+    c = np.array(0.1, dt)
+    d = (a - c).astype(dt)
+    exact = sum(f(float(t)) * f(float(t)) for t in d)
+    hi, lo = O.map_sum_dd(dt, a, None, a.size, O.mapping(dt, "square", c=c))
+    assert abs(float(f(hi) + f(lo)) - float(exact)) <= abs(float(exact)) * 2 ** -52
+    # counting sequences: sum of i*i for i < 1000
+    hi, lo = O.map_sum_dd(dt, ("count", 0), ("count", 0), 1000, O.mapping(dt))
+    assert hi + lo == sum(i * i for i in range(1000))
