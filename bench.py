@@ -31,7 +31,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0  # MI355X HBM3E
-SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "select", "segsort"]
+SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "select", "mapreduce", "segsort"]
 
 
 def timed_steps(ca, q, fn, steps, warmup, restore=None):
@@ -321,6 +321,32 @@ def main():
         sec["unique_i32_runs_2p27"] = {"GBps": round(gbs(alg, t), 1), "ms": round(t, 4),
                                        "frac": round(gbs(alg, t) / PEAK_GBS, 4), "runs": runs}
         del dk, dko, k
+
+    # ------------------------------------------- transform_reduce / inner_product
+    if "mapreduce" in only:
+        n = 1 << 28
+        x, y = W.saxpy_f32(n)  # the saxpy inputs: float32 ~ N(0, 1)
+        dx, dy = ca.DeviceArray.from_host(x, q), ca.DeviceArray.from_host(y, q)
+        res1 = ca.DeviceArray(1, np.float32, q)
+        mp = ca.mapping(np.float32, "id", "multiplies")
+        fi = lambda: ca.transform_reduce(q, np.float32, dx, dy, n, mp, res1)
+        ms, _ = timed_steps(ca, q, fi, K, Wm)
+        t = float(np.mean(ms))
+        got = float(res1.to_host(q)[0])
+        ref = float(np.dot(x.astype(np.float64), y.astype(np.float64)))
+        ulp = float(np.spacing(np.float32(abs(ref))))
+        kern = kernel_breakdown(q, fi, ["mapreduce", "mapreduce_final"])
+        sec["inner_product_f32_2p28"] = {"GBps": round(gbs(8 * n, t), 1), "ms": round(t, 4),
+                                         "frac": round(gbs(8 * n, t) / PEAK_GBS, 4),
+                                         "err_ulp_vs_fp64": round(abs(got - ref) / ulp, 3), "kernels_ms": kern}
+        mean = float(np.mean(x, dtype=np.float64))
+        mv = ca.mapping(np.float32, "square", c=np.float32(mean))
+        fv = lambda: ca.transform_reduce(q, np.float32, dx, None, n, mv, res1)
+        ms, _ = timed_steps(ca, q, fv, K, Wm)
+        t = float(np.mean(ms))
+        sec["sum_sq_dev_f32_2p28"] = {"GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4),
+                                      "frac": round(gbs(4 * n, t) / PEAK_GBS, 4)}
+        del dx, dy, x, y, res1
 
     # ------------------------------------------------------ segmented sort
     if "segsort" in only:

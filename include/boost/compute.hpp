@@ -10,6 +10,7 @@
 #include <boost/compute/container.hpp>
 #include <boost/compute/core.hpp>
 #include <boost/compute/functional.hpp>
+#include <boost/compute/iterator.hpp>
 #include <boost/compute/lambda.hpp>
 #include <boost/compute/type_traits/is_device_iterator.hpp>
 #endif

@@ -1,0 +1,9 @@
+//---------------------------------------------------------------------------//
+// The iterators this build provides (reference: iterator.hpp): buffer
+// iterators and counting_iterator.
+//---------------------------------------------------------------------------//
+#ifndef BOOST_COMPUTE_ITERATOR_HPP
+#define BOOST_COMPUTE_ITERATOR_HPP
+#include <boost/compute/iterator/buffer_iterator.hpp>
+#include <boost/compute/iterator/counting_iterator.hpp>
+#endif

@@ -10,6 +10,8 @@
 //                                     partition...), with f one of _1, _1 * a,
 //                                     a * _1, _1 % a, _1 & a and cmp one of
 //                                     < <= > >= == !=
+//   pow(_1, 2), pow(_1 - c, 2)        element maps of transform_reduce
+//                                     (lambda/functional.hpp:521)
 // Anything else does not compile.
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_LAMBDA_HPP
@@ -18,6 +20,8 @@
 #include <type_traits>
 
 #include <bcmi.h>
+
+#include <boost/compute/exception/opencl_error.hpp>
 
 namespace boost {
 namespace compute {
@@ -74,6 +78,36 @@ template <class A, class = typename std::enable_if<std::is_integral<A>::value>::
 inline arg1_expr<BCMI_FN_AND, A> operator&(placeholder<1>, A a)
 {
     return arg1_expr<BCMI_FN_AND, A>{a};
+}
+
+// _1 - c, and pow(_1 - c, 2) / pow(_1, 2): the squared-deviation element map
+template <class T>
+struct minus_arg1 {
+    T c;
+};
+
+template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+inline minus_arg1<T> operator-(placeholder<1>, T c)
+{
+    return minus_arg1<T>{c};
+}
+
+template <class T>
+struct square_expr {
+    T c;  // offset subtracted before squaring
+};
+
+template <class T>
+inline square_expr<T> pow(minus_arg1<T> m, int e)
+{
+    if (e != 2) throw opencl_error(BCMI_INVALID_VALUE, "pow: only the exponent 2 is supported");
+    return square_expr<T>{m.c};
+}
+
+inline square_expr<int> pow(placeholder<1>, int e)
+{
+    if (e != 2) throw opencl_error(BCMI_INVALID_VALUE, "pow: only the exponent 2 is supported");
+    return square_expr<int>{0};
 }
 
 // f(_1) cmp c

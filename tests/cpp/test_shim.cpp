@@ -299,6 +299,36 @@ int main()
         CHECK(bc::count_if(iv.begin(), iv.end(), (bc::_1 & 1) != 0, queue) == 1024);
         CHECK(bc::count_if(iv.begin(), iv.end(), bc::not1(bc::_1 > 1024), queue) == 1025);
     }
+    {  // transform_reduce / inner_product: test/test_transform_reduce.cpp:23-99,
+       // test/test_inner_product.cpp:23-52
+        int data[] = {1, -2, -3, -4, 5};
+        bc::vector<int> vec(data, data + 5, queue);
+        int sum = 0;
+        bc::transform_reduce(vec.begin(), vec.end(), &sum, bc::abs<int>(), bc::plus<int>(), queue);
+        CHECK(sum == 15);
+        float fdata[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
+        bc::vector<float> fv(fdata, fdata + 10, queue);
+        float fsum = 0;
+        bc::reduce(fv.begin(), fv.end(), &fsum, bc::plus<float>(), queue);
+        const float mean = fsum / fv.size();
+        CHECK(std::fabs(mean - 5.5f) < 1e-5f);
+        using bc::lambda::_1;
+        using bc::lambda::pow;
+        bc::transform_reduce(fv.begin(), fv.end(), &fsum, pow(_1 - mean, 2), bc::plus<float>(), queue);
+        CHECK(std::fabs(fsum / fv.size() - 8.25f) < 1e-4f);
+        int d1[] = {1, 2, 3, 4}, d2[] = {10, 20, 30, 40};
+        bc::vector<int> in1(d1, d1 + 4, queue), in2(d2, d2 + 4, queue);
+        CHECK(bc::inner_product(in1.begin(), in1.end(), in2.begin(), 0, queue) == 300);
+        CHECK(bc::inner_product(bc::make_counting_iterator<int>(0), bc::make_counting_iterator<int>(100),
+                                bc::make_counting_iterator<int>(0), 0, queue) == 328350);
+        // the general form: init accumulated with transform(a, b)
+        CHECK(bc::inner_product(in1.begin(), in1.end(), in2.begin(), 5, bc::plus<int>(), bc::minus<int>(), queue) ==
+              5 + (1 - 10) + (2 - 20) + (3 - 30) + (4 - 40));
+        bc::vector<int> res(1, queue.get_context());
+        bc::transform_reduce(in1.begin(), in1.end(), in2.begin(), res.begin(), bc::multiplies<int>(), bc::max<int>(),
+                             queue);
+        CHECK(res.begin().read(queue) == 160);
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {
