@@ -8,6 +8,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstdint>
+#include <cstring>
 
 #include "runtime.hpp"
 
@@ -131,6 +132,17 @@ BDEV uint32_t launch_epoch(const Ctrl *ctrl, uint32_t back)
 // tile's data loads behind it.  `after` (a value computed from the tile's
 // data) makes the load's address depend on the data, so it is issued after
 // the data loads.
+// The low 32 bits of a value's representation (a data dependence to hand
+// to launch_epoch_after).
+template <typename D>
+BDEV uint32_t low_word(const D &v)
+{
+    static_assert(sizeof(D) >= 4, "at least 4 bytes");
+    uint32_t w;
+    memcpy(&w, &v, 4);
+    return w;
+}
+
 template <typename D>
 BDEV uint32_t launch_epoch_after(const Ctrl *ctrl, uint32_t back, D after)
 {

@@ -181,7 +181,7 @@ template <typename KB, int VB, int IPT, int RANK, typename XF, bool FULL>
 BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
                         const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
                         uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf,
-                        Ctrl *ctrl, uint64_t *__restrict__ status, uint32_t epoch)
+                        Ctrl *ctrl, uint64_t *__restrict__ status, uint32_t back)
 {
     using V = typename ValT<VB>::type;
     constexpr int TILE = kSortThreads * IPT;
@@ -226,6 +226,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     // per digit (thread tid == digit): wave prefixes and the tile count
     const uint32_t c0 = L.cnt[0][tid], c1 = L.cnt[1][tid], c2 = L.cnt[2][tid], c3 = L.cnt[3][tid];
     const uint32_t total = c0 + c1 + c2 + c3;
+    const uint32_t epoch = launch_epoch_after(ctrl, back, total);  // read after the tile's loads
     if (tile == 0) lb_store(&status[tid], lb_word(epoch, LB_P, total));
     else lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_A, total));
     uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
@@ -322,15 +323,15 @@ template <typename KB, int VB, int IPT, int RANK, typename XF>
 BDEV void onesweep_tile(const KB *__restrict__ kin, KB *__restrict__ kout,
                         const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
                         uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf,
-                        Ctrl *ctrl, uint64_t *__restrict__ status, uint32_t epoch)
+                        Ctrl *ctrl, uint64_t *__restrict__ status, uint32_t back)
 {
     __shared__ OnesweepLds<KB, VB, IPT> lds;
     if ((uint64_t)(tile + 1) * (kSortThreads * IPT) <= n)
         onesweep_body<KB, VB, IPT, RANK, XF, true>(lds, kin, kout, vin, vout, n, tile, shift, goff0, xf, ctrl, status,
-                                                   epoch);
+                                                   back);
     else
         onesweep_body<KB, VB, IPT, RANK, XF, false>(lds, kin, kout, vin, vout, n, tile, shift, goff0, xf, ctrl,
-                                                    status, epoch);
+                                                    status, back);
 }
 
 // Four workgroups per CU need <= 128 VGPRs per lane (4 waves per SIMD): the
@@ -349,7 +350,7 @@ void k_onesweep(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                            uint64_t *__restrict__ status, uint32_t back)
 {
     onesweep_tile<KB, VB, IPT, RANK, XF>(kin, kout, vin, vout, n, blockIdx.x, shift, ctrl->offsets[pass], xf, ctrl,
-                                     status, launch_epoch(ctrl, back));
+                                     status, back);
 }
 
 }  // namespace bcmi

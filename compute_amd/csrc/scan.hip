@@ -82,7 +82,7 @@ struct ScanLds {
 // aligned, so full chunks move as one 16-byte load / store.
 template <typename T, typename P, bool VEC, int Q, bool FULL>
 BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
-                    typename P::Cr init, int exclusive, uint32_t epoch)
+                    typename P::Cr init, int exclusive, uint32_t back)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
@@ -132,6 +132,7 @@ BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, ui
     for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, L.wave[i]);
     if (w == 0) {
         const S total = P::merge(P::merge(L.wave[0], L.wave[1]), P::merge(L.wave[2], L.wave[3]));
+        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
                                             [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
         if (lane == 0) L.prefix = prefix;
@@ -169,11 +170,10 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
     __shared__ ScanLds<P> lds;
-    const uint32_t epoch = launch_epoch(ctrl, back);
     if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        scan_body<T, P, VEC, Q, true>(lds, in, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, true>(lds, in, out, n, ctrl, status, init, exclusive, back);
     else
-        scan_body<T, P, VEC, Q, false>(lds, in, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, false>(lds, in, out, n, ctrl, status, init, exclusive, back);
 }
 
 template <typename T, typename P, int Q>

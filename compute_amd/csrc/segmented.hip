@@ -371,8 +371,7 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
     const size_t b = t.base;
     onesweep_tile<KB, VB, IPT, kRankLdsMask, XF>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
                                              t.tile, shift, seg_off + ((size_t)t.seg * passes + pass) * 256, xf,
-                                             ctrl, status + (size_t)(blockIdx.x - t.tile) * 256,
-                                             launch_epoch(ctrl, back));
+                                             ctrl, status + (size_t)(blockIdx.x - t.tile) * 256, back);
 }
 
 // One block (256 threads, thread = digit) per large segment.

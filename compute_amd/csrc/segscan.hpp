@@ -108,7 +108,7 @@ struct alignas(V * sizeof(KT) >= 16 ? 16 : V * sizeof(KT)) key_chunk {
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
 BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                        KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
-                       uint64_t *count_out, uint32_t epoch)
+                       uint64_t *count_out, uint32_t back)
 {
     using A = typename P::A;
     using CA = typename P::CA;
@@ -192,6 +192,7 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
         const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
         const C agg{P::carry(total.v), total.n};
         const C cid{P::cid(), 0u};
+        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
             return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
         }, ctrl);
@@ -270,7 +271,7 @@ struct SegLdsBlk {
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
 BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                            KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
-                           uint64_t *count_out, uint32_t epoch)
+                           uint64_t *count_out, uint32_t back)
 {
     static_assert(sizeof(KT) == 4 && sizeof(VT) == 4, "blocked form: 4-byte keys and values");
     using A = typename P::A;
@@ -351,6 +352,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
         const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
         const C agg{P::carry(total.v), total.n};
         const C cid{P::cid(), 0u};
+        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
             return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
         }, ctrl);
@@ -423,24 +425,23 @@ __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ 
                                                          uint32_t back)
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
-    const uint32_t epoch = launch_epoch(ctrl, back);
     const bool full = (size_t)(blockIdx.x + 1) * TILE <= n;
     if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4 && Q <= 8) {
         __shared__ SegLdsBlk<P, Q> lds;
         if (full)
             segscan_body_blk<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                            count_out, epoch);
+                                                            count_out, back);
         else
             segscan_body_blk<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                             count_out, epoch);
+                                                             count_out, back);
     } else {
         __shared__ SegLds<P> lds;
         if (full)
             segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                        count_out, epoch);
+                                                        count_out, back);
         else
             segscan_body<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                         count_out, epoch);
+                                                         count_out, back);
     }
 }
 
