@@ -41,6 +41,7 @@ EXPORTS = [
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
+    "bcmi_extract_keys",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -164,6 +165,7 @@ def lib():
         L.bcmi_count_if.argtypes = [vp, i, vp, sz, pp, vp]
         L.bcmi_unique.argtypes = [vp, i, vp, vp, sz, vp]
         L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
+        L.bcmi_extract_keys.argtypes = [vp, i, i, vp, sz, sz, sz, vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,
                                             C.POINTER(C.c_double), i]
         _lib = L
@@ -424,6 +426,19 @@ def transform_reduce(q, dtype, a, b, n, m, out, reduce_op=PLUS, init=None, preci
                                       C.byref(m), reduce_op, ini, _p(out),
                                       C.cast(f64.ptr, C.POINTER(C.c_double)), precision))
     return f64
+
+
+def sort_by_transform(q, records, key_dtype, key_offset=0, fn="id", descending=False, n=None):
+    """experimental::sort_by_transform: keys = f(field at key_offset of each
+    record) into a temporary, then sort_by_key(keys, records) (records of
+    1, 2, 4, 8 or 16 bytes, in place)."""
+    n = records.size if n is None else n
+    kd = np.dtype(key_dtype)
+    keys = DeviceArray(max(n, 1), kd, q)
+    rb = records.dtype.itemsize
+    check(lib().bcmi_extract_keys(q.h, DTYPE[kd], MAP_FN[fn], records.ptr, rb, key_offset, n, keys.ptr))
+    check(lib().bcmi_sort(q.h, SORT_BY_KEY, DTYPE[kd], keys.ptr, records.ptr, rb, n, int(descending)))
+    return keys
 
 
 def segmented_sort(q, keys, values, offsets_host, descending=False):

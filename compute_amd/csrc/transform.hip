@@ -6,6 +6,7 @@
 // of every operand, a grid that covers the array once (HBM-bound:
 // saxpy moves 12 B per element).
 #include "device.hpp"
+#include "ops.hpp"
 
 #include <type_traits>
 
@@ -142,6 +143,62 @@ static int transform_typed(bcmi_queue q, int op, const void *a, const void *b, v
 }  // namespace bcmi
 
 using namespace bcmi;
+
+// experimental::sort_by_transform's key extraction: one key_dtype field of
+// each record, optionally |x| (integers wrap).
+template <typename K>
+__global__ __launch_bounds__(kTfThreads) void k_extract_keys(const unsigned char *__restrict__ rec, size_t rb,
+                                                             size_t off, size_t n, int fn, K *__restrict__ keys)
+{
+    const size_t stride = (size_t)gridDim.x * kTfThreads;
+    for (size_t i = (size_t)blockIdx.x * kTfThreads + threadIdx.x; i < n; i += stride) {
+        K k = *(const K *)(rec + i * rb + off);
+        if (fn == BCMI_MAP_ABS) {
+            if constexpr (std::is_floating_point<K>::value) {
+                k = __builtin_fabs(k);
+            } else if constexpr (std::is_signed<K>::value) {
+                using U = typename wrap_t<K>::type;
+                if (k < 0) k = (K)(U)(0u - (U)k);
+            }
+        }
+        keys[i] = k;
+    }
+}
+
+template <typename K>
+static int extract_keys(bcmi_queue q, int fn, const void *rec, size_t rb, size_t off, size_t n, void *keys)
+{
+    BCMI_CHECK(off + sizeof(K) <= rb, BCMI_INVALID_VALUE, "extract_keys: field past the record");
+    BCMI_CHECK(((uintptr_t)rec + off) % sizeof(K) == 0 && rb % sizeof(K) == 0, BCMI_INVALID_VALUE,
+               "extract_keys: key field not aligned to its type");
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kTfThreads - 1) / kTfThreads, (size_t)cu_blocks(q, 8));
+    BCMI_LAUNCH(q, "extract_keys",
+                hipLaunchKernelGGL((k_extract_keys<K>), dim3(blocks), dim3(kTfThreads), 0, q->stream,
+                                   (const unsigned char *)rec, rb, off, n, fn, (K *)keys));
+    return 0;
+}
+
+extern "C" int bcmi_extract_keys(bcmi_queue q, int key_dtype, int fn, const void *records, size_t record_bytes,
+                                 size_t key_offset, size_t n, void *keys)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(fn == BCMI_MAP_ID || fn == BCMI_MAP_ABS, BCMI_INVALID_VALUE, "extract_keys: bad fn %d", fn);
+    if (n == 0) return 0;
+    BCMI_CHECK(records && keys, BCMI_INVALID_VALUE, "extract_keys: null pointer");
+    switch (key_dtype) {
+    case BCMI_I8: return extract_keys<int8_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_U8: return extract_keys<uint8_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_I16: return extract_keys<int16_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_U16: return extract_keys<uint16_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_I32: return extract_keys<int32_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_U32: return extract_keys<uint32_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_I64: return extract_keys<int64_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_U64: return extract_keys<uint64_t>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_F32: return extract_keys<float>(q, fn, records, record_bytes, key_offset, n, keys);
+    case BCMI_F64: return extract_keys<double>(q, fn, records, record_bytes, key_offset, n, keys);
+    default: set_error("extract_keys: bad dtype %d", key_dtype); return BCMI_INVALID_VALUE;
+    }
+}
 
 extern "C" int bcmi_saxpy(bcmi_queue q, float a, const float *x, const float *y, float *out, size_t n)
 {

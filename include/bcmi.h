@@ -316,6 +316,19 @@ int bcmi_transform_reduce(bcmi_queue queue, int dtype, const bcmi_seq *a,
                           int reduce_op, const void *init, void *out,
                           double *out_f64, int precision);
 
+/* The key extraction of experimental::sort_by_transform
+ * (experimental/sort_by_transform.hpp): keys[i] = f(field of element i),
+ * where element i is record_bytes bytes at records + i * record_bytes, the
+ * field is one key_dtype value at byte key_offset (get<N> of an OpenCL
+ * vector type: offset N * sizeof(component); 0 for a scalar), and f is
+ * BCMI_MAP_ID or BCMI_MAP_ABS (abs<T>; integer abs wraps, so INT_MIN stays
+ * INT_MIN).  The reference then sorts the elements by these keys with
+ * sort_by_key; bcmi_sort(api = BCMI_SORT_BY_KEY) with the records as
+ * values does that. */
+int bcmi_extract_keys(bcmi_queue queue, int key_dtype, int fn,
+                      const void *records, size_t record_bytes,
+                      size_t key_offset, size_t n, void *keys);
+
 /* --------------------------------------------------------- hipGraphs */
 /* Capture everything enqueued on the queue between begin and end into a
  * replayable graph (queue scratch must be reserved beforehand). */

@@ -9,6 +9,7 @@
 #include <boost/compute/algorithm/detail/radix_sort.hpp>
 #include <boost/compute/container.hpp>
 #include <boost/compute/core.hpp>
+#include <boost/compute/experimental/sort_by_transform.hpp>
 #include <boost/compute/functional.hpp>
 #include <boost/compute/iterator.hpp>
 #include <boost/compute/lambda.hpp>

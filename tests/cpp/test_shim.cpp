@@ -329,6 +329,28 @@ int main()
                              queue);
         CHECK(res.begin().read(queue) == 160);
     }
+    {  // experimental::sort_by_transform: test/test_sort_by_transform.cpp:25-37, :75-108
+        int data[] = {1, -2, 4, -3, 0, 5, -8, -9};
+        bc::vector<int> v(data, data + 8, queue);
+        bc::experimental::sort_by_transform(v.begin(), v.end(), bc::abs<int>(), bc::less<int>(), queue);
+        CHECK(host(v, queue) == (std::vector<int>{0, 1, -2, -3, 4, 5, -8, -9}));
+        float f[] = {1, 2, 3, 0, 9, 8, 7, 0, 4, 5, 6, 0, 0, 0, 0, 0};
+        bc::vector<bc::float4_> fv(reinterpret_cast<bc::float4_ *>(f), reinterpret_cast<bc::float4_ *>(f) + 4,
+                                   queue);
+        bc::experimental::sort_by_transform(fv.begin(), fv.end(), bc::get<1>(), bc::less<float>(), queue);
+        std::vector<bc::float4_> h = host(fv, queue);
+        CHECK(h[0] == bc::float4_(0.f, 0.f, 0.f, 0.f) && h[1] == bc::float4_(1.f, 2.f, 3.f, 0.f) &&
+              h[2] == bc::float4_(4.f, 5.f, 6.f, 0.f) && h[3] == bc::float4_(9.f, 8.f, 7.f, 0.f));
+        // stable by the first component: the order test/test_merge_sort_gpu.cpp:362-369 expects
+        bc::int2_ d2[] = {bc::int2_(8, 3), bc::int2_(5, 1), bc::int2_(2, 1), bc::int2_(6, 1),
+                          bc::int2_(8, 1), bc::int2_(7, 1), bc::int2_(4, 1), bc::int2_(8, 2)};
+        bc::vector<bc::int2_> iv(d2, d2 + 8, queue);
+        bc::experimental::sort_by_transform(iv.begin(), iv.end(), bc::get<0>(), bc::less<int>(), queue);
+        std::vector<bc::int2_> hi = host(iv, queue);
+        CHECK(hi[0] == bc::int2_(2, 1) && hi[1] == bc::int2_(4, 1) && hi[2] == bc::int2_(5, 1) &&
+              hi[3] == bc::int2_(6, 1) && hi[4] == bc::int2_(7, 1) && hi[5] == bc::int2_(8, 3) &&
+              hi[6] == bc::int2_(8, 1) && hi[7] == bc::int2_(8, 2));
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {

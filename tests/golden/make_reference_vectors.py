@@ -344,6 +344,18 @@ case("inner_product_int", "test/test_inner_product.cpp:36", "inner_product", "in
 case("inner_product_counting_iterator", "test/test_inner_product.cpp:41", "inner_product", "int32",
      a={"count": 0, "n": 100}, b={"count": 0}, init=0, expected=328350)
 
+# ------------------------------------------------ experimental::sort_by_transform
+# records as lists of components; "get" = component index (None: the scalar
+# itself), "fn" = id | abs.  The length<float4_> case
+# (test/test_sort_by_transform.cpp:41) has implementation-defined rounding and
+# is not transcribed.
+case("sort_int_by_abs", "test/test_sort_by_transform.cpp:25", "sort_by_transform", "int32",
+     records=[1, -2, 4, -3, 0, 5, -8, -9], get=None, fn="abs", descending=False,
+     expected=[0, 1, -2, -3, 4, 5, -8, -9])
+case("sort_float4_by_component", "test/test_sort_by_transform.cpp:75", "sort_by_transform", "float32",
+     records=[[1, 2, 3, 0], [9, 8, 7, 0], [4, 5, 6, 0], [0, 0, 0, 0]], get=1, fn="id", descending=False,
+     expected=[[0, 0, 0, 0], [1, 2, 3, 0], [4, 5, 6, 0], [9, 8, 7, 0]])
+
 if __name__ == "__main__":
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.json")
     with open(out, "w") as f:

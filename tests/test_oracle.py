@@ -675,3 +675,38 @@ def test_map_sum_dd_is_the_exact_sum_of_exact_products(dt):
     # counting sequences: sum of i*i for i < 1000
     hi, lo = O.map_sum_dd(dt, ("count", 0), ("count", 0), 1000, O.mapping(dt))
     assert hi + lo == sum(i * i for i in range(1000))
+
+
+# ------------------------------------------- experimental::sort_by_transform
+def sort_by_transform_oracle(records, get, fn, descending):
+    """keys = f(component) then the sort_by_key dispatch (insertion sort below
+    32, else stable radix order) with the records as values
+    (experimental/sort_by_transform.hpp)."""
+    rec = np.ascontiguousarray(records)
+    keys = (rec if get is None else rec[:, get]).copy()
+    if fn == "abs":
+        keys = np.abs(keys)  # integer abs wraps (INT_MIN stays); float abs clears the sign bit
+    vals = rec.view(np.dtype("V%d" % (rec.itemsize * (1 if rec.ndim == 1 else rec.shape[1])))).reshape(-1).copy()
+    _, v = O.sort_dispatch(O.SORT_BY_KEY, keys, vals, descending)
+    return np.frombuffer(v.tobytes(), rec.dtype).reshape(rec.shape)
+
+
+def test_golden_sort_by_transform(golden):
+    seen = 0
+    for c in golden:
+        if c["algo"] != "sort_by_transform":
+            continue
+        seen += 1
+        got = sort_by_transform_oracle(np.array(c["records"], c["dtype"]), c["get"], c["fn"], c["descending"])
+        assert got.tolist() == c["expected"], c["id"]
+    assert seen == 2
+
+
+def test_sort_by_transform_oracle_is_a_stable_sort_by_the_key():
+    rec = RNG.integers(-50, 50, (500, 2)).astype(np.int32)
+    got = sort_by_transform_oracle(rec, 0, "id", False)
+    want = rec[np.argsort(rec[:, 0], kind="stable")]
+    assert np.array_equal(got, want)
+    got = sort_by_transform_oracle(rec[:, 1].copy(), None, "abs", True)
+    k = np.abs(rec[:, 1])
+    assert np.array_equal(got, rec[:, 1][np.argsort(-k, kind="stable")])
