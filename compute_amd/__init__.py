@@ -41,7 +41,7 @@ EXPORTS = [
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
-    "bcmi_extract_keys",
+    "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -166,6 +166,8 @@ def lib():
         L.bcmi_unique.argtypes = [vp, i, vp, vp, sz, vp]
         L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
         L.bcmi_extract_keys.argtypes = [vp, i, i, vp, sz, sz, sz, vp]
+        L.bcmi_host_malloc.argtypes = [sz, C.POINTER(vp)]
+        L.bcmi_host_free.argtypes = [vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,
                                             C.POINTER(C.c_double), i]
         _lib = L

@@ -344,6 +344,21 @@ int bcmi_free(void *ptr)
     return 0;
 }
 
+int bcmi_host_malloc(size_t bytes, void **ptr)
+{
+    BCMI_CHECK(ptr, BCMI_INVALID_VALUE, "null pointer");
+    *ptr = nullptr;
+    if (bytes == 0) return 0;
+    BCMI_HIP(hipHostMalloc(ptr, bytes, hipHostMallocDefault));
+    return 0;
+}
+
+int bcmi_host_free(void *ptr)
+{
+    if (ptr) BCMI_HIP(hipHostFree(ptr));
+    return 0;
+}
+
 int bcmi_memcpy(bcmi_queue q, void *dst, const void *src, size_t bytes, int kind, int blocking)
 {
     BCMI_TRY(use_queue(q));

@@ -126,6 +126,14 @@ int bcmi_memcpy(bcmi_queue queue, void *dst, const void *src, size_t bytes,
 int bcmi_fill(bcmi_queue queue, void *dst, const void *pattern,
               size_t pattern_bytes, size_t count);
 
+/* Pinned (page-locked) host memory, the analogue of the reference's
+ * host-pointer buffers (buffer.hpp CL_MEM_ALLOC_HOST_PTR,
+ * allocator/pinned_allocator.hpp): bcmi_memcpy with blocking == 0 between it
+ * and device memory runs asynchronously on the queue's stream (copy_async,
+ * algorithm/copy.hpp:102-150), at full link rate. */
+int bcmi_host_malloc(size_t bytes, void **ptr);
+int bcmi_host_free(void *ptr);
+
 /* --------------------------------------------------------------- events */
 /* event.hpp:86-89,204,264 (profiling duration) */
 typedef struct bcmi_event_s *bcmi_event;

@@ -7,6 +7,7 @@
 #define BOOST_COMPUTE_HPP
 #include <boost/compute/algorithm.hpp>
 #include <boost/compute/algorithm/detail/radix_sort.hpp>
+#include <boost/compute/async/future.hpp>
 #include <boost/compute/container.hpp>
 #include <boost/compute/core.hpp>
 #include <boost/compute/experimental/sort_by_transform.hpp>

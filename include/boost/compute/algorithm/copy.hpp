@@ -10,6 +10,8 @@
 #include <iterator>
 #include <vector>
 
+#include <boost/compute/async/future.hpp>
+#include <boost/compute/event.hpp>
 #include <boost/compute/system.hpp>
 #include <boost/compute/type_traits/is_device_iterator.hpp>
 
@@ -59,6 +61,41 @@ inline OutputIterator copy(InputIterator first, InputIterator last, OutputIterat
     return detail::copy_dispatch(first, last, result, queue,
                                  std::integral_constant<bool, is_device_iterator<InputIterator>::value>(),
                                  std::integral_constant<bool, is_device_iterator<OutputIterator>::value>());
+}
+
+// copy_async (reference: algorithm/copy.hpp:102-150 dispatch_copy_async):
+// an asynchronous copy between a contiguous host range (a pointer) and a
+// device range, or between device ranges, on the queue's stream.  The host
+// memory must stay valid until the future is waited on; page-locked host
+// memory (bcmi_host_malloc) lets the transfer overlap host work.
+template <class T>
+inline future<buffer_iterator<T> > copy_async(const T *first, const T *last, buffer_iterator<T> result,
+                                              command_queue &queue = system::default_queue())
+{
+    const size_t n = (size_t)(last - first);
+    if (n) detail::bcmi_check(bcmi_memcpy(queue.get(), result.device_ptr(), first, n * sizeof(T), BCMI_H2D, 0));
+    return future<buffer_iterator<T> >(result + (std::ptrdiff_t)n, event::marker(queue));
+}
+
+template <class T>
+inline future<T *> copy_async(buffer_iterator<T> first, buffer_iterator<T> last, T *result,
+                              command_queue &queue = system::default_queue())
+{
+    const size_t n = (size_t)(last - first);
+    if (n) detail::bcmi_check(bcmi_memcpy(queue.get(), result, first.device_ptr(), n * sizeof(T), BCMI_D2H, 0));
+    return future<T *>(result + n, event::marker(queue));
+}
+
+template <class T>
+inline future<buffer_iterator<T> > copy_async(buffer_iterator<T> first, buffer_iterator<T> last,
+                                              buffer_iterator<T> result,
+                                              command_queue &queue = system::default_queue())
+{
+    const size_t n = (size_t)(last - first);
+    if (n)
+        detail::bcmi_check(
+            bcmi_memcpy(queue.get(), result.device_ptr(), first.device_ptr(), n * sizeof(T), BCMI_D2D, 0));
+    return future<buffer_iterator<T> >(result + (std::ptrdiff_t)n, event::marker(queue));
 }
 
 template <class InputIterator, class Size, class OutputIterator>

@@ -4,6 +4,7 @@
 #include <boost/compute/command_queue.hpp>
 #include <boost/compute/context.hpp>
 #include <boost/compute/device.hpp>
+#include <boost/compute/event.hpp>
 #include <boost/compute/exception/opencl_error.hpp>
 #include <boost/compute/system.hpp>
 #include <boost/compute/types/fundamental.hpp>

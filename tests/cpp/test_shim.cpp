@@ -351,6 +351,24 @@ int main()
               hi[3] == bc::int2_(6, 1) && hi[4] == bc::int2_(7, 1) && hi[5] == bc::int2_(8, 3) &&
               hi[6] == bc::int2_(8, 1) && hi[7] == bc::int2_(8, 2));
     }
+    {  // copy_async through page-locked host memory, with the queue busy meanwhile
+        const size_t n = 1 << 22;
+        unsigned *pin = 0, *back = 0;
+        CHECK(bcmi_host_malloc(n * sizeof(unsigned), (void **)&pin) == 0);
+        CHECK(bcmi_host_malloc(n * sizeof(unsigned), (void **)&back) == 0);
+        for (size_t i = 0; i < n; ++i) pin[i] = (unsigned)(i * 2654435761u);
+        bc::vector<unsigned> dv(n, queue.get_context());
+        bc::future<bc::vector<unsigned>::iterator> f = bc::copy_async(pin, pin + n, dv.begin(), queue);
+        CHECK(f.valid());
+        CHECK(f.get() == dv.end());
+        bc::sort(dv.begin(), dv.end(), queue);  // ordered after the copy on the queue
+        bc::future<unsigned *> g = bc::copy_async(dv.begin(), dv.end(), back, queue);
+        CHECK(g.get() == back + n);
+        std::vector<unsigned> ref(pin, pin + n);
+        std::sort(ref.begin(), ref.end());
+        CHECK(std::equal(ref.begin(), ref.end(), back));
+        CHECK(bcmi_host_free(pin) == 0 && bcmi_host_free(back) == 0);
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {
