@@ -412,10 +412,11 @@ def _as_seq(x, at=0):
     return Seq(x.ptr.value + at * x.dtype.itemsize if at else x.ptr.value, 0)
 
 
-def transform_reduce(q, dtype, a, b, n, m, out, reduce_op=PLUS, init=None, precision=COMPENSATED):
+def transform_reduce(q, dtype, a, b, n, m, out, reduce_op=PLUS, init=None, precision=COMPENSATED, f64=None):
     """out[0] = init op y_0 op ... op y_{n-1}, y = m(a[i], b[i]) (b None: unary).
     a and b are DeviceArrays or counting(start).  Returns the FP64 value
-    (a DeviceArray of one double) for float sums."""
+    (a DeviceArray of one double, f64 when given: nothing is allocated then,
+    as graph capture needs) for float sums."""
     dt = np.dtype(dtype)
     sa = _as_seq(a)
     sb = _as_seq(b) if b is not None else None
@@ -423,7 +424,8 @@ def transform_reduce(q, dtype, a, b, n, m, out, reduce_op=PLUS, init=None, preci
     if init is not None:
         ini_arr = np.array([init], dt)
         ini = ini_arr.ctypes.data_as(C.c_void_p)
-    f64 = DeviceArray(1, np.float64, q)
+    if f64 is None:
+        f64 = DeviceArray(1, np.float64, q)
     check(lib().bcmi_transform_reduce(q.h, DTYPE[dt], C.byref(sa), C.byref(sb) if sb is not None else None, n,
                                       C.byref(m), reduce_op, ini, _p(out),
                                       C.cast(f64.ptr, C.POINTER(C.c_double)), precision))

@@ -274,3 +274,47 @@ def test_copy_if_full_size_properties(queue):
     assert ca.partition(queue, d, o2, None, pc) == k
     got = o2.to_host(queue)
     assert np.array_equal(got[:k], x[mask]) and np.array_equal(got[k:], x[~mask])
+
+
+def test_graph_replay_of_compaction_and_mapped_reduction(queue):
+    # captured once, replayed with new input contents and eager work between
+    # replays: every replay must advance the look-back epochs and match
+    n = 3 * 16384 + 77
+    pc, po = both(np.int32, ">=", 0)
+    x = RNG.integers(-100, 100, n).astype(np.int32)
+    d = dev(queue, x)
+    out = dev(queue, np.zeros(n, np.int32))
+    part = dev(queue, np.zeros(n, np.int32))
+    uq = dev(queue, np.zeros(n, np.int32))
+    red = dev(queue, np.zeros(1, np.int32))
+    f64 = dev(queue, np.zeros(1, np.float64))
+    c1, c2, c3 = (dev(queue, np.zeros(1, np.uint64)) for _ in range(3))
+    mp = ca.mapping(np.int32, "abs", "minus")
+
+    def work():
+        ca.copy_if(queue, d, out, pc, count=c1)
+        ca.partition(queue, d, part, None, pc, count=c2)
+        ca.unique(queue, d, uq, count=c3)
+        ca.transform_reduce(queue, np.int32, d, out, n, mp, red, ca.MAX, f64=f64)
+
+    work()  # sizes the queue's scratch and look-back state before capture
+    queue.finish()
+    g = ca.Graph(queue)
+    with g:
+        work()
+    for rep in range(3):
+        x = (RNG.integers(-100, 100, n) // (rep + 1)).astype(np.int32)  # runs for unique
+        ca.check(ca.lib().bcmi_memcpy(queue.h, d.ptr, x.ctypes.data_as(ca.C.c_void_p), x.nbytes, ca.H2D, 1))
+        ca.copy_if(queue, d, uq, pc)  # eager work on the same queue between replays
+        g.launch()
+        queue.finish()
+        want = O.copy_if(x, po)
+        k = int(c1.to_host(queue)[0])
+        got = out.to_host(queue)
+        assert k == want.size and same(got[:k], want), rep
+        ws, wk = O.stable_partition(x, po)
+        assert int(c2.to_host(queue)[0]) == wk and same(part.to_host(queue), ws), rep
+        wu = O.unique_copy(x)
+        assert int(c3.to_host(queue)[0]) == wu.size and same(uq.to_host(queue)[:wu.size], wu), rep
+        wr = O.map_reduce(np.int32, x, got, n, O.mapping(np.int32, "abs", "minus"), O.MAX)
+        assert red.to_host(queue)[0] == wr, rep
