@@ -31,7 +31,8 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_GBS = 8000.0  # MI355X HBM3E
-SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "select", "mapreduce", "segsort"]
+SECTIONS = ["sort", "sort_by_key", "scan", "reduce", "saxpy", "rbk", "segscan", "select", "mapreduce", "msort",
+            "segsort"]
 
 
 def timed_steps(ca, q, fn, steps, warmup, restore=None):
@@ -347,6 +348,30 @@ def main():
         sec["sum_sq_dev_f32_2p28"] = {"GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4),
                                       "frac": round(gbs(4 * n, t) / PEAK_GBS, 4)}
         del dx, dy, x, y, res1
+
+    # ------------------------------------------- comparison (merge) sort
+    if "msort" in only:
+        # 2^24 int2_ records sorted by get<0>(_1) < get<0>(_2), stable: x
+        # uniform in [0, 2^20) (ties), y = the input position
+        n = 1 << 24
+        rng = np.random.default_rng(0xB0057001)
+        rec = np.stack([rng.integers(0, 1 << 20, n), np.arange(n)], axis=1).astype(np.int32)
+        src, work = ca.DeviceArray.from_host(rec, q), ca.DeviceArray(2 * n, np.int32, q)
+        cmp = ca.comparator(np.int32, "field", 0)
+        fn = lambda: ca.merge_sort(q, cmp, work, 8, n)
+        ms, _ = timed_steps(ca, q, fn, K, Wm, lambda: d2d(ca, q, work, src))
+        t = float(np.mean(ms))
+        kern = kernel_breakdown(q, fn, ["merge_sort_block", "merge_sort_merge"], restore=lambda: d2d(ca, q, work, src))
+        passes = int(np.ceil(np.log2(n / 4096)))
+        merge_ms = kern.get("merge_sort_merge", float("nan")) / passes
+        sec["merge_sort_int2_by_x_2p24"] = {
+            "Mrecords_per_s": round(n / (t * 1e-3) / 1e6, 1), "ms": round(t, 4), "kernels_ms": kern,
+            "merge_passes": passes, "merge_pass_ms": round(merge_ms, 4),
+            "merge_pass_GBps": round(gbs(16 * n, merge_ms), 1),
+            "merge_pass_frac": round(gbs(16 * n, merge_ms) / PEAK_GBS, 4),
+            "note": "8-byte {code, index} items: a merge pass reads and writes 16 B per record; the last pass "
+                    "gathers the records instead of writing items"}
+        del src, work, rec
 
     # ------------------------------------------------------ segmented sort
     if "segsort" in only:

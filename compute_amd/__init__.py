@@ -41,7 +41,7 @@ EXPORTS = [
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
-    "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free",
+    "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free", "bcmi_merge_sort",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -66,6 +66,24 @@ class Map(C.Structure):
 def mapping(dtype, fn="id", op="multiplies", c=0):
     raw = np.array([c]).astype(np.dtype(dtype)).tobytes()
     return Map(MAP_OP[op], MAP_FN[fn], int.from_bytes(raw.ljust(8, b"\0"), "little"))
+
+
+KEY_FORM = {"field": 0, "abs": 1, "abs_sum2": 2, "length": 3}
+
+
+class Comparator(C.Structure):
+    """bcmi_comparator: key(a at lhs_offset) < (or >) key(b at rhs_offset)."""
+    _fields_ = [("dtype", C.c_int32), ("form", C.c_int32), ("lhs_offset", C.c_uint32),
+                ("rhs_offset", C.c_uint32), ("offset2", C.c_uint32), ("count", C.c_uint32),
+                ("greater", C.c_int32)]
+
+
+def comparator(dtype, form="field", lhs=0, rhs=None, second=0, count=0, greater=False):
+    """A comparator over fields of scalar type `dtype` at byte offsets of each
+    record: form field | abs | abs_sum2 (abs(field + field at `second`)) |
+    length (of `count` components)."""
+    return Comparator(DTYPE[np.dtype(dtype)], KEY_FORM[form], lhs, lhs if rhs is None else rhs, second, count,
+                      int(bool(greater)))
 
 
 def counting(start):
@@ -166,6 +184,7 @@ def lib():
         L.bcmi_unique.argtypes = [vp, i, vp, vp, sz, vp]
         L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
         L.bcmi_extract_keys.argtypes = [vp, i, i, vp, sz, sz, sz, vp]
+        L.bcmi_merge_sort.argtypes = [vp, C.POINTER(Comparator), vp, sz, vp, sz, sz]
         L.bcmi_host_malloc.argtypes = [sz, C.POINTER(vp)]
         L.bcmi_host_free.argtypes = [vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,
@@ -443,6 +462,17 @@ def sort_by_transform(q, records, key_dtype, key_offset=0, fn="id", descending=F
     check(lib().bcmi_extract_keys(q.h, DTYPE[kd], MAP_FN[fn], records.ptr, rb, key_offset, n, keys.ptr))
     check(lib().bcmi_sort(q.h, SORT_BY_KEY, DTYPE[kd], keys.ptr, records.ptr, rb, n, int(descending)))
     return keys
+
+
+def merge_sort(q, cmp, records, record_bytes, n=None, values=None, value_bytes=None):
+    """Comparison sort of n records of record_bytes bytes by a comparator
+    (sort / stable_sort / sort_by_key with a user comparator), in place and
+    stable; values (optional) move with their records."""
+    n = records.nbytes // record_bytes if n is None else n
+    if values is not None and value_bytes is None:
+        value_bytes = values.nbytes // max(n, 1)
+    check(lib().bcmi_merge_sort(q.h, C.byref(cmp), records.ptr, record_bytes,
+                                values.ptr if values is not None else None, value_bytes or 0, n))
 
 
 def segmented_sort(q, keys, values, offsets_host, descending=False):

@@ -337,6 +337,56 @@ int bcmi_extract_keys(bcmi_queue queue, int key_dtype, int fn,
                       const void *records, size_t record_bytes,
                       size_t key_offset, size_t n, void *keys);
 
+/* ------------------------------------------------ comparator merge sort */
+/* Comparison sort for comparators other than less<T> / greater<T> on a
+ * fundamental key: the reference's merge_sort_on_gpu
+ * (algorithm/detail/merge_sort_on_gpu.hpp:467-571), reached from sort
+ * (sort.hpp:84-104), sort_by_key (sort_by_key.hpp:88-106), stable_sort
+ * (stable_sort.hpp:36-50) and stable_sort_by_key with a user comparator.
+ *
+ * The comparator is  key(a, lhs_offset) CMP key(b, rhs_offset),  CMP being
+ * < (greater == 0) or > (greater != 0), over fields of scalar type `dtype`
+ * at byte offsets inside each record_bytes-byte record:
+ *   BCMI_KEY_FIELD     the field                          a.x < b.x, a.s0 < b.s3
+ *   BCMI_KEY_ABS       abs(field): integer abs is the     abs(a) < abs(b)
+ *                      unsigned magnitude (abs(INT_MIN) = 2^31, as OpenCL's
+ *                      abs returns an unsigned type); fabs for floats
+ *   BCMI_KEY_ABS_SUM2  abs(field + field at offset2), the sum in C's
+ *                      promoted type (narrow integers add in int, 32- and
+ *                      64-bit integers wrap)                 abs(a.x + a.y)
+ *   BCMI_KEY_LENGTH    Euclidean length of `count` consecutive float or
+ *                      double fields, compared through the sum of squares
+ *                      in double                             length(a) < length(b)
+ * ABS_SUM2 and LENGTH need lhs_offset == rhs_offset.  Floats compare as C
+ * does (-0 == +0); NaN fields give an unspecified order, as a comparator
+ * that is not a strict weak order does in the reference.
+ *
+ * The sort is stable (ties keep their input order), which is also a valid
+ * result of the reference's unstable merge sort.  values (may be NULL) are
+ * value_bytes-byte records moved with their records.  Records are 1, 2, 4,
+ * 8, 16, 32 or 64 bytes; n < 2^32.  Temporaries come from the queue's
+ * scratch arena. */
+typedef enum {
+    BCMI_KEY_FIELD = 0,
+    BCMI_KEY_ABS = 1,
+    BCMI_KEY_ABS_SUM2 = 2,
+    BCMI_KEY_LENGTH = 3
+} bcmi_key_form;
+
+typedef struct bcmi_comparator {
+    int32_t dtype;        /* bcmi_dtype of the compared fields */
+    int32_t form;         /* bcmi_key_form */
+    uint32_t lhs_offset;  /* field offset read from the left operand */
+    uint32_t rhs_offset;  /* field offset read from the right operand */
+    uint32_t offset2;     /* ABS_SUM2: offset of the second addend */
+    uint32_t count;       /* LENGTH: number of components (1..16) */
+    int32_t greater;      /* 0: key(a) < key(b); otherwise key(a) > key(b) */
+} bcmi_comparator;
+
+int bcmi_merge_sort(bcmi_queue queue, const bcmi_comparator *cmp,
+                    void *records, size_t record_bytes, void *values,
+                    size_t value_bytes, size_t n);
+
 /* --------------------------------------------------------- hipGraphs */
 /* Capture everything enqueued on the queue between begin and end into a
  * replayable graph (queue scratch must be reserved beforehand). */

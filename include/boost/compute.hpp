@@ -6,6 +6,7 @@
 #ifndef BOOST_COMPUTE_HPP
 #define BOOST_COMPUTE_HPP
 #include <boost/compute/algorithm.hpp>
+#include <boost/compute/algorithm/detail/merge_sort_on_gpu.hpp>
 #include <boost/compute/algorithm/detail/radix_sort.hpp>
 #include <boost/compute/async/future.hpp>
 #include <boost/compute/container.hpp>

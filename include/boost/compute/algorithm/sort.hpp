@@ -1,7 +1,9 @@
 //---------------------------------------------------------------------------//
 // sort (reference: algorithm/sort.hpp:182-202).  GPU column: n <= 32 the
 // one-work-item insertion sort, otherwise radix (less<T> ascending,
-// greater<T> descending).  Host iterators are copied to the device, sorted
+// greater<T> descending); a user comparator (a comparator expression of
+// lambda.hpp or experimental::less_by) takes the comparison sort
+// (detail/merge_sort_on_gpu.hpp).  Host iterators are copied to the device, sorted
 // and copied back (the reference maps them, sort.hpp:124-148).
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_ALGORITHM_SORT_HPP
@@ -9,6 +11,7 @@
 
 #include <iterator>
 
+#include <boost/compute/algorithm/detail/merge_sort_on_gpu.hpp>
 #include <boost/compute/algorithm/detail/radix_sort.hpp>
 #include <boost/compute/container/vector.hpp>
 #include <boost/compute/functional/operator.hpp>
@@ -18,10 +21,10 @@ namespace compute {
 namespace detail {
 
 template <class T, class Compare>
-inline void dispatch_sort(int api, buffer_iterator<T> first, buffer_iterator<T> last, Compare, command_queue &queue,
-                          std::true_type)
+inline void dispatch_sort(int api, buffer_iterator<T> first, buffer_iterator<T> last, Compare compare,
+                          command_queue &queue, std::true_type)
 {
-    bcmi_sort_call<T, T>(api, first, last, (T *)0, sort_direction<Compare, T>::descending, queue);
+    sort_call(api, first, last, (T *)0, compare, queue);
 }
 
 template <class Iterator, class Compare>

@@ -1,10 +1,12 @@
 //---------------------------------------------------------------------------//
 // sort_by_key (reference: algorithm/sort_by_key.hpp:135-163): n < 32 the
-// insertion sort by key, otherwise radix.
+// insertion sort by key, otherwise radix; a user comparator takes the
+// comparison sort (sort_by_key.hpp:88-106).
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_ALGORITHM_SORT_BY_KEY_HPP
 #define BOOST_COMPUTE_ALGORITHM_SORT_BY_KEY_HPP
 
+#include <boost/compute/algorithm/detail/merge_sort_on_gpu.hpp>
 #include <boost/compute/algorithm/detail/radix_sort.hpp>
 #include <boost/compute/functional/operator.hpp>
 
@@ -12,14 +14,12 @@ namespace boost {
 namespace compute {
 
 template <class KeyIterator, class ValueIterator, class Compare>
-inline void sort_by_key(KeyIterator keys_first, KeyIterator keys_last, ValueIterator values_first, Compare,
+inline void sort_by_key(KeyIterator keys_first, KeyIterator keys_last, ValueIterator values_first, Compare compare,
                         command_queue &queue = system::default_queue())
 {
     static_assert(is_device_iterator<KeyIterator>::value, "sort_by_key needs device iterators");
     static_assert(is_device_iterator<ValueIterator>::value, "sort_by_key needs device iterators");
-    typedef typename std::iterator_traits<KeyIterator>::value_type K;
-    detail::bcmi_sort_call(BCMI_SORT_BY_KEY, keys_first, keys_last, values_first.device_ptr(),
-                           detail::sort_direction<Compare, K>::descending, queue);
+    detail::sort_call(BCMI_SORT_BY_KEY, keys_first, keys_last, values_first.device_ptr(), compare, queue);
 }
 
 template <class KeyIterator, class ValueIterator>

@@ -1,5 +1,6 @@
 //---------------------------------------------------------------------------//
-// stable_sort (reference: algorithm/stable_sort.hpp:81-110): radix at any n.
+// stable_sort (reference: algorithm/stable_sort.hpp:81-110): radix at any n;
+// a user comparator takes the comparison sort (stable_sort.hpp:36-50).
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_ALGORITHM_STABLE_SORT_HPP
 #define BOOST_COMPUTE_ALGORITHM_STABLE_SORT_HPP

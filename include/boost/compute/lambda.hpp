@@ -12,11 +12,15 @@
 //                                     < <= > >= == !=
 //   pow(_1, 2), pow(_1 - c, 2)        element maps of transform_reduce
 //                                     (lambda/functional.hpp:521)
+//   _1 < _2, get<N>(_1) < get<M>(_2), abs(_1) < abs(_2), length(_1) <
+//   length(_2), abs(get<N>(_1) + get<M>(_1)) < ... (and >)
+//                                     comparators of the comparison sort
 // Anything else does not compile.
 //---------------------------------------------------------------------------//
 #ifndef BOOST_COMPUTE_LAMBDA_HPP
 #define BOOST_COMPUTE_LAMBDA_HPP
 
+#include <cstddef>
 #include <type_traits>
 
 #include <bcmi.h>
@@ -143,6 +147,86 @@ BOOST_COMPUTE_LAMBDA_CMP(>=, BCMI_CMP_GE)
 BOOST_COMPUTE_LAMBDA_CMP(==, BCMI_CMP_EQ)
 BOOST_COMPUTE_LAMBDA_CMP(!=, BCMI_CMP_NE)
 #undef BOOST_COMPUTE_LAMBDA_CMP
+
+// Comparator forms: the user comparators of sort / stable_sort /
+// sort_by_key that the reference's tests write as OpenCL functions
+// (test/test_merge_sort_gpu.cpp:239-242, 275-278, 311-314; test/test_sort.cpp:260;
+// test/test_stable_sort.cpp:53-56):
+//   _1 < _2, _1 > _2                       the element (a scalar)
+//   get<N>(_1) < get<M>(_2)                components (M may differ from N)
+//   abs(_1) < abs(_2), abs(get<N>(_1)) ... unsigned magnitude of integers
+//   abs(get<N>(_1) + get<M>(_1)) < abs(get<N>(_2) + get<M>(_2))
+//   length(_1) < length(_2)                Euclidean length of a float vector
+// and the same with >.
+template <int I, int FORM, std::size_t C0, std::size_t C1 = 0>
+struct key_term {
+};
+
+template <int I, std::size_t C0, std::size_t C1>
+struct sum_term {
+};
+
+template <int FORM, std::size_t LHS, std::size_t RHS, std::size_t SECOND, bool GREATER>
+struct compare_expr {
+};
+
+template <std::size_t N, int I>
+inline key_term<I, BCMI_KEY_FIELD, N> get(placeholder<I>)
+{
+    return key_term<I, BCMI_KEY_FIELD, N>();
+}
+
+template <int I>
+inline key_term<I, BCMI_KEY_ABS, 0> abs(placeholder<I>)
+{
+    return key_term<I, BCMI_KEY_ABS, 0>();
+}
+
+template <int I, std::size_t N>
+inline key_term<I, BCMI_KEY_ABS, N> abs(key_term<I, BCMI_KEY_FIELD, N>)
+{
+    return key_term<I, BCMI_KEY_ABS, N>();
+}
+
+template <int I, std::size_t A, std::size_t B>
+inline sum_term<I, A, B> operator+(key_term<I, BCMI_KEY_FIELD, A>, key_term<I, BCMI_KEY_FIELD, B>)
+{
+    return sum_term<I, A, B>();
+}
+
+template <int I, std::size_t A, std::size_t B>
+inline key_term<I, BCMI_KEY_ABS_SUM2, A, B> abs(sum_term<I, A, B>)
+{
+    return key_term<I, BCMI_KEY_ABS_SUM2, A, B>();
+}
+
+template <int I>
+inline key_term<I, BCMI_KEY_LENGTH, 0> length(placeholder<I>)
+{
+    return key_term<I, BCMI_KEY_LENGTH, 0>();
+}
+
+inline compare_expr<BCMI_KEY_FIELD, 0, 0, 0, false> operator<(placeholder<1>, placeholder<2>)
+{
+    return compare_expr<BCMI_KEY_FIELD, 0, 0, 0, false>();
+}
+
+inline compare_expr<BCMI_KEY_FIELD, 0, 0, 0, true> operator>(placeholder<1>, placeholder<2>)
+{
+    return compare_expr<BCMI_KEY_FIELD, 0, 0, 0, true>();
+}
+
+#define BOOST_COMPUTE_LAMBDA_KEY_CMP(OP, GREATER)                                                          \
+    template <int F, std::size_t A, std::size_t B, std::size_t A2, std::size_t B2>                         \
+    inline compare_expr<F, A, A2, B, GREATER> operator OP(key_term<1, F, A, B>, key_term<2, F, A2, B2>)    \
+    {                                                                                                      \
+        static_assert(F == BCMI_KEY_FIELD || F == BCMI_KEY_ABS || (A == A2 && B == B2),                    \
+                      "abs-of-sum and length comparators read the same components of both operands");     \
+        return compare_expr<F, A, A2, B, GREATER>();                                                       \
+    }
+BOOST_COMPUTE_LAMBDA_KEY_CMP(<, false)
+BOOST_COMPUTE_LAMBDA_KEY_CMP(>, true)
+#undef BOOST_COMPUTE_LAMBDA_KEY_CMP
 
 }  // namespace lambda
 

@@ -434,3 +434,165 @@ def map_sum_dd(dtype, a, b, n, m):
     lo = C.c_double()
     hi = lib().orc_map_sum_dd(NP2CODE[dt], pa, a0, pb, b0, int(b is not None), n, C.byref(m), C.byref(lo))
     return hi, lo.value
+
+
+# ------------------------------------------------------------ comparator sorts
+# sort / stable_sort / sort_by_key with a user comparator: the reference's
+# merge_sort_on_gpu (algorithm/detail/merge_sort_on_gpu.hpp:467-571) and, at
+# n < 32, serial_insertion_sort (algorithm/detail/insertion_sort.hpp:26-72).
+# The comparators are OpenCL C functions of two records; the forms restated
+# here are the ones the reference's tests write (test/test_merge_sort_gpu.cpp:
+# 239-242, 275-278, 311-314, 345-348; test/test_sort.cpp:260, 297-300;
+# test/test_stable_sort.cpp:53-56, 76-79): key(a) < key(b) or key(a) > key(b)
+# with key = a field, abs(field), abs(field + field), or length().
+KEY_FORMS = {"field": 0, "abs": 1, "abs_sum2": 2, "length": 3}
+
+
+def _as_rows(records, record_bytes):
+    """The records as an (n, record_bytes) uint8 array."""
+    raw = np.ascontiguousarray(records).view(np.uint8).reshape(-1)
+    return raw.reshape(-1, record_bytes)
+
+
+def _field(rows, dt, off):
+    return np.ascontiguousarray(rows[:, off:off + dt.itemsize]).view(dt).reshape(-1)
+
+
+def _magnitude_u64(s):
+    """abs() of a signed integer array as its unsigned magnitude (uint64)."""
+    u = s.astype(np.int64).view(np.uint64) if s.dtype != np.int64 else s.view(np.uint64)
+    return np.where(s < 0, ~u + np.uint64(1), u)
+
+
+def comparator_keys(rows, dtype, form, off, second=0, count=0):
+    """key(record) for every row, vectorised: a field; abs(field) (integer abs
+    is the unsigned magnitude, abs(INT_MIN) = 2^31; fabs for floats);
+    abs(field + second field) with C's integer promotion (8/16-bit integers
+    add in int, 32/64-bit integers wrap); length() through the sum of squares
+    in double, in component order."""
+    dt = np.dtype(dtype)
+    form = KEY_FORMS.get(form, form)
+    x = _field(rows, dt, off)
+    if form == 0:
+        return x
+    if form == 3:
+        ss = np.zeros(x.size, np.float64)
+        for k in range(count):
+            c = _field(rows, dt, off + k * dt.itemsize).astype(np.float64)
+            ss = ss + c * c
+        return ss
+    if form == 2:
+        y = _field(rows, dt, second)
+        if dt.kind == "f":
+            x = x + y
+        elif dt.itemsize < 4:
+            x = x.astype(np.int64) + y.astype(np.int64)
+        else:
+            u = (x.view(dt.str.replace("i", "u")) + y.view(dt.str.replace("i", "u")))  # wraps
+            x = u.view(dt)
+    if dt.kind == "f":
+        return np.abs(x)
+    if x.dtype.kind == "u":
+        return x.astype(np.uint64)
+    return _magnitude_u64(x)
+
+
+def _stable_order(key, greater):
+    if not greater:
+        return np.argsort(key, kind="stable")
+    n = key.size
+    return (n - 1 - np.argsort(key[::-1], kind="stable"))[::-1]
+
+
+def comparator_sort(records, record_bytes, dtype, form="field", lhs=0, rhs=None, second=0, count=0,
+                    greater=False, values=None):
+    """The stable sort of the records by the comparator (the order every
+    strict weak order gives; the reference's stable merge sort and insertion
+    sort produce it, its unstable sort one of the orders that ties allow).
+    A comparator reading different fields of its two operands (a.s0 < b.s3)
+    is sorted pairwise by comparator_less_py.  Returns (records, values) as
+    flat arrays of the input dtypes."""
+    rows = _as_rows(records, record_bytes)
+    rhs = lhs if rhs is None else rhs
+    if rhs == lhs:
+        order = _stable_order(comparator_keys(rows, dtype, form, lhs, second, count), greater)
+    else:
+        import functools
+        less = comparator_less_py(dtype, form, lhs, rhs, second, count, greater)
+        items = list(range(rows.shape[0]))
+
+        def cmp(i, j):
+            return -1 if less(rows[i], rows[j]) else (1 if less(rows[j], rows[i]) else 0)
+        order = np.array(sorted(items, key=functools.cmp_to_key(cmp)), np.int64)
+    out = np.ascontiguousarray(rows[order]).view(np.asarray(records).dtype).reshape(-1)
+    if values is None:
+        return out, None
+    v = np.asarray(values)
+    vrows = v.view(np.uint8).reshape(rows.shape[0], -1)
+    return out, np.ascontiguousarray(vrows[order]).view(v.dtype).reshape(-1)
+
+
+def comparator_less_py(dtype, form, lhs, rhs=None, second=0, count=0, greater=False):
+    """The comparator as a Python function of two records (uint8 rows), in
+    exact integer / double arithmetic: the OpenCL C semantics spelled out
+    independently of comparator_keys (Python ints, explicit wrap-around)."""
+    import struct
+    dt = np.dtype(dtype)
+    form = KEY_FORMS.get(form, form)
+    rhs = lhs if rhs is None else rhs
+    code = {"i1": "b", "u1": "B", "i2": "h", "u2": "H", "i4": "i", "u4": "I", "i8": "q", "u8": "Q",
+            "f4": "f", "f8": "d"}[dt.str[1:]]
+    sz = dt.itemsize
+
+    def fld(row, off):
+        return struct.unpack_from("<" + code, bytes(row[off:off + sz]))[0]
+
+    def wrap(v):
+        if dt.kind == "f" or sz < 4:
+            return v
+        bits = 8 * sz
+        v %= 1 << bits
+        return v - (1 << bits) if dt.kind == "i" and v >= 1 << (bits - 1) else v
+
+    def key(row, off):
+        if form == 3:
+            ss = 0.0
+            for k in range(count):
+                c = float(fld(row, off + k * sz))
+                ss = ss + c * c
+            return ss
+        v = fld(row, off)
+        if form == 0:
+            return v
+        if form == 2:
+            if dt == np.float32:
+                v = float(np.float32(v) + np.float32(fld(row, second)))
+            else:
+                v = wrap(v + fld(row, second))
+        return abs(v)
+
+    def less(a, b):
+        ka, kb = key(a, lhs), key(b, rhs)
+        return ka > kb if greater else ka < kb
+    return less
+
+
+def insertion_sort_rows(rows, less, values=None):
+    """serial_insertion_sort / serial_insertion_sort_by_key
+    (algorithm/detail/insertion_sort.hpp:26-72, :87-142): one work-item moves
+    each element left past every element it is less than -- stable."""
+    data = [r.copy() for r in rows]
+    vals = None if values is None else list(values)
+    for i in range(1, len(data)):
+        v = data[i]
+        vv = None if vals is None else vals[i]
+        j = i
+        while j > 0 and less(v, data[j - 1]):
+            data[j] = data[j - 1]
+            if vals is not None:
+                vals[j] = vals[j - 1]
+            j -= 1
+        data[j] = v
+        if vals is not None:
+            vals[j] = vv
+    return np.array(data).reshape(len(data), -1), vals

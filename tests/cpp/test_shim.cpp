@@ -369,6 +369,153 @@ int main()
         CHECK(std::equal(ref.begin(), ref.end(), back));
         CHECK(bcmi_host_free(pin) == 0 && bcmi_host_free(back) == 0);
     }
+    {  // comparison sort, less/greater given to merge_sort_on_gpu: test/test_merge_sort_gpu.cpp:36-62
+        using bc::char_;
+        char_ data[] = {'c', 'a', '0', '7', 'B', 'F', '\0', '$'};
+        bc::vector<char_> v(data, data + 8, queue);
+        bc::detail::merge_sort_on_gpu(v.begin(), v.end(), bc::less<char_>(), queue);
+        CHECK(host(v, queue) == (std::vector<char_>{'\0', '$', '0', '7', 'B', 'F', 'a', 'c'}));
+        bc::detail::merge_sort_on_gpu(v.begin(), v.end(), bc::greater<char_>(), queue);
+        CHECK(host(v, queue) == (std::vector<char_>{'c', 'a', 'F', 'B', '7', '0', '$', '\0'}));
+    }
+    {  // abs(a) < abs(b) and abs(a.x + a.y) < abs(b.x + b.y): test/test_merge_sort_gpu.cpp:224-296
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        std::vector<int> d(1024);
+        std::vector<bc::int2_> d2(1024);
+        for (int i = 0; i < 1024; ++i) {
+            d[i] = i % 2 ? 1024 - i : i - 1024;
+            d2[i] = i % 2 ? bc::int2_(i, i) : bc::int2_(i - 1024, i - 1024);
+        }
+        bc::vector<int> v(d.begin(), d.end(), queue);
+        bc::detail::merge_sort_on_gpu(v.begin(), v.end(), abs(_1) < abs(_2), queue);
+        std::vector<int> h = host(v, queue);
+        bool ok = true;
+        for (size_t i = 1; i < h.size(); ++i) ok = ok && std::abs(h[i - 1]) <= std::abs(h[i]);
+        CHECK(ok);
+        bc::vector<bc::int2_> v2(d2.begin(), d2.end(), queue);
+        bc::detail::merge_sort_on_gpu(v2.begin(), v2.end(),
+                                      abs(get<0>(_1) + get<1>(_1)) < abs(get<0>(_2) + get<1>(_2)), queue);
+        std::vector<bc::int2_> h2 = host(v2, queue);
+        ok = true;
+        for (size_t i = 1; i < h2.size(); ++i)
+            ok = ok && std::abs(h2[i - 1][0] + h2[i - 1][1]) <= std::abs(h2[i][0] + h2[i][1]);
+        CHECK(ok);
+    }
+    {  // a.s0 < b.s3 on long8_: test/test_merge_sort_gpu.cpp:299-330
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        std::vector<bc::long8_> d(256);
+        for (long i = 0; i < 256; ++i) d[i] = bc::long8_(i % 2 ? i : i * i);
+        bc::vector<bc::long8_> v(d.begin(), d.end(), queue);
+        bc::detail::merge_sort_on_gpu(v.begin(), v.end(), get<0>(_1) < get<3>(_2), queue);
+        std::vector<bc::long8_> h = host(v, queue);
+        bool ok = true;
+        for (size_t i = 1; i < h.size(); ++i) ok = ok && h[i - 1][0] <= h[i][3];
+        CHECK(ok);
+    }
+    {  // stable merge sort of int2_ by x: test/test_merge_sort_gpu.cpp:333-370
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        bc::int2_ data[] = {bc::int2_(8, 3), bc::int2_(5, 1), bc::int2_(2, 1), bc::int2_(6, 1),
+                            bc::int2_(8, 1), bc::int2_(7, 1), bc::int2_(4, 1), bc::int2_(8, 2)};
+        bc::vector<bc::int2_> v(data, data + 8, queue);
+        bc::detail::merge_sort_on_gpu(v.begin(), v.end(), get<0>(_1) < get<0>(_2), true, queue);
+        std::vector<bc::int2_> h = host(v, queue);
+        int want[8][2] = {{2, 1}, {4, 1}, {5, 1}, {6, 1}, {7, 1}, {8, 3}, {8, 1}, {8, 2}};
+        bool ok = true;
+        for (int i = 0; i < 8; ++i) ok = ok && h[i][0] == want[i][0] && h[i][1] == want[i][1];
+        CHECK(ok);
+    }
+    {  // stable_sort of int2_ by each component: test/test_stable_sort.cpp:40-95
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        bc::int2_ data[] = {bc::int2_(2, 1), bc::int2_(2, 2), bc::int2_(1, 2), bc::int2_(1, 1)};
+        bc::vector<bc::int2_> w(data, data + 4, queue);
+        bc::stable_sort(w.begin(), w.end(), get<0>(_1) < get<0>(_2), queue);
+        std::vector<bc::int2_> h = host(w, queue);
+        CHECK(h[0] == bc::int2_(1, 2) && h[1] == bc::int2_(1, 1) && h[2] == bc::int2_(2, 1) && h[3] == bc::int2_(2, 2));
+        bc::stable_sort(w.begin(), w.end(), get<1>(_1) < get<1>(_2), queue);
+        h = host(w, queue);
+        CHECK(h[0] == bc::int2_(1, 1) && h[1] == bc::int2_(2, 1) && h[2] == bc::int2_(1, 2) && h[3] == bc::int2_(2, 2));
+    }
+    {  // sort by length(_1) < length(_2): test/test_sort.cpp:238-285
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        float data[] = {1.0f, 0.2f, 1.3f, 1.0f, 6.7f, 0.0f, 5.2f, 3.4f, 1.4f, 1.4f};
+        bc::vector<bc::float2_> v(reinterpret_cast<bc::float2_ *>(data), reinterpret_cast<bc::float2_ *>(data) + 5,
+                                  queue);
+        bc::sort(v.begin(), v.end(), length(_1) < length(_2), queue);
+        bc::copy(v.begin(), v.end(), reinterpret_cast<bc::float2_ *>(data), queue);
+        float want[] = {1.0f, 0.2f, 1.3f, 1.0f, 1.4f, 1.4f, 5.2f, 3.4f, 6.7f, 0.0f};
+        CHECK(std::equal(data, data + 10, want));
+    }
+    {  // sort of a struct by a member: test/test_sort.cpp:293-326 (sort_by_x)
+        struct Particle {
+            float x, y;
+        };
+        std::vector<Particle> p = {{0.1f, 0.f}, {-0.4f, 0.f}, {10.0f, 0.f}, {0.001f, 0.f}};
+        bc::vector<Particle> v(p.begin(), p.end(), queue);
+        bc::sort(v.begin(), v.end(), bc::experimental::less_by(&Particle::x), queue);
+        std::vector<Particle> h = host(v, queue);
+        CHECK(h[0].x == -0.4f && h[1].x == 0.001f && h[2].x == 0.1f && h[3].x == 10.0f);
+        bc::sort(v.begin(), v.end(), bc::experimental::greater_by(&Particle::x), queue);
+        h = host(v, queue);
+        CHECK(h[0].x == 10.0f && h[3].x == -0.4f);
+    }
+    {  // sort_by_key with a user comparator: test/test_sort_by_key.cpp:113-142
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        const int n = 1024;
+        std::vector<int> hk(n);
+        std::vector<float> hv(n);
+        for (int i = 0; i < n; ++i) {
+            hk[i] = n - i;
+            hv[i] = (n - i) / 2.f;
+        }
+        bc::vector<int> k(hk.begin(), hk.end(), queue);
+        bc::vector<float> v(hv.begin(), hv.end(), queue);
+        bc::sort_by_key(k.begin(), k.end(), v.begin(), _1 < _2, queue);
+        std::vector<int> gk = host(k, queue);
+        std::vector<float> gv = host(v, queue);
+        CHECK(std::is_sorted(gk.begin(), gk.end()) && std::is_sorted(gv.begin(), gv.end()) && gk[0] == 1);
+    }
+    {  // stable_sort_by_key with a user comparator: test/test_stable_sort_by_key.cpp:165-208
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        std::vector<int> hk(128), hv(128);
+        for (int i = 0; i < 128; ++i) hk[i] = hv[i] = -i;
+        hk[64] = hk[126] = hk[127] = -256;
+        hv[64] = 3;
+        hv[126] = 1;
+        hv[127] = 2;
+        bc::vector<int> k(hk.begin(), hk.end(), queue);
+        bc::vector<int> v(hv.begin(), hv.end(), queue);
+        bc::stable_sort_by_key(k.begin(), k.end(), v.begin(), _1 < _2, queue);
+        std::vector<int> gk = host(k, queue), gv = host(v, queue);
+        CHECK(gk[0] == -256 && gk[1] == -256 && gk[2] == -256 && gv[0] == 3 && gv[1] == 1 && gv[2] == 2);
+        CHECK(std::is_sorted(gk.begin(), gk.end()));
+    }
+    {  // a large comparator sort against std::stable_sort: 3 M int2_ by y, descending
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        const size_t n = 3000017;
+        std::mt19937 gen(17);
+        std::vector<bc::int2_> d(n);
+        for (size_t i = 0; i < n; ++i) d[i] = bc::int2_((int)i, (int)(gen() % 20000) - 10000);
+        bc::vector<bc::int2_> v(d.begin(), d.end(), queue);
+        bc::stable_sort(v.begin(), v.end(), get<1>(_1) > get<1>(_2), queue);
+        std::stable_sort(d.begin(), d.end(), [](const bc::int2_ &a, const bc::int2_ &b) { return a[1] > b[1]; });
+        std::vector<bc::int2_> h = host(v, queue);
+        bool ok = true;
+        for (size_t i = 0; i < n; ++i) ok = ok && h[i][0] == d[i][0] && h[i][1] == d[i][1];
+        CHECK(ok);
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {

@@ -356,6 +356,69 @@ case("sort_float4_by_component", "test/test_sort_by_transform.cpp:75", "sort_by_
      records=[[1, 2, 3, 0], [9, 8, 7, 0], [4, 5, 6, 0], [0, 0, 0, 0]], get=1, fn="id", descending=False,
      expected=[[0, 0, 0, 0], [1, 2, 3, 0], [4, 5, 6, 0], [9, 8, 7, 0]])
 
+# ------------------------------------------------------ comparator sorts
+# sort / stable_sort / sort_by_key / stable_sort_by_key with a user
+# comparator (merge_sort_on_gpu).  records: lists of components (or scalars);
+# cmp = {"form": field | abs | abs_sum2 | length, "lhs"/"rhs": component read
+# from the left / right operand, "second": the second addend (abs_sum2),
+# "count": components (length), "greater": CMP is >}.  The checks are the
+# test's own: "expected" (whole output), "expected_x" (positions -> first
+# component), "expected_prefix" (+ "expected_values_prefix"), and "sorted"
+# (is_sorted by the comparator, plus values sorted where the test checks it).
+def _gen(n, f):
+    return [f(i) for i in range(n)]
+
+
+case("merge_sort_gpu_char_less", "test/test_merge_sort_gpu.cpp:36-53", "compare_sort", "int8",
+     records=[99, 97, 48, 55, 66, 70, 0, 36], cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False},
+     expected=[0, 36, 48, 55, 66, 70, 97, 99])
+case("merge_sort_gpu_char_greater", "test/test_merge_sort_gpu.cpp:55-62", "compare_sort", "int8",
+     records=[0, 36, 48, 55, 66, 70, 97, 99], cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": True},
+     expected=[99, 97, 70, 66, 55, 48, 36, 0])
+case("merge_sort_gpu_int_abs", "test/test_merge_sort_gpu.cpp:224-258", "compare_sort", "int32",
+     records=_gen(1024, lambda i: 1024 - i if i % 2 else i - 1024), cmp={"form": "abs", "lhs": 0, "rhs": 0,
+                                                                         "greater": False}, sorted=True)
+case("merge_sort_gpu_int2_abs_sum", "test/test_merge_sort_gpu.cpp:261-296", "compare_sort", "int32",
+     records=_gen(1024, lambda i: [i, i] if i % 2 else [i - 1024, i - 1024]),
+     cmp={"form": "abs_sum2", "lhs": 0, "rhs": 0, "second": 1, "greater": False}, sorted=True)
+case("merge_sort_gpu_long8_s0_s3", "test/test_merge_sort_gpu.cpp:299-330", "compare_sort", "int64",
+     records=_gen(256, lambda i: [i] * 8 if i % 2 else [i * i] * 8),
+     cmp={"form": "field", "lhs": 0, "rhs": 3, "greater": False}, sorted=True)
+case("merge_sort_gpu_stable_int2", "test/test_merge_sort_gpu.cpp:333-370", "compare_sort", "int32",
+     records=[[8, 3], [5, 1], [2, 1], [6, 1], [8, 1], [7, 1], [4, 1], [8, 2]],
+     cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False}, stable=True,
+     expected=[[2, 1], [4, 1], [5, 1], [6, 1], [7, 1], [8, 3], [8, 1], [8, 2]])
+case("stable_sort_int2_first", "test/test_stable_sort.cpp:43-73", "compare_sort", "int32",
+     records=[[2, 1], [2, 2], [1, 2], [1, 1]], cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False},
+     stable=True, expected=[[1, 2], [1, 1], [2, 1], [2, 2]])
+case("stable_sort_int2_second", "test/test_stable_sort.cpp:75-92", "compare_sort", "int32",
+     records=[[1, 2], [1, 1], [2, 1], [2, 2]], cmp={"form": "field", "lhs": 1, "rhs": 1, "greater": False},
+     stable=True, expected=[[1, 1], [2, 1], [1, 2], [2, 2]])
+case("sort_float2_by_length", "test/test_sort.cpp:238-285", "compare_sort", "float32",
+     records=[[1.0, 0.2], [1.3, 1.0], [6.7, 0.0], [5.2, 3.4], [1.4, 1.4]],
+     cmp={"form": "length", "lhs": 0, "rhs": 0, "count": 2, "greater": False},
+     expected=[[1.0, 0.2], [1.3, 1.0], [1.4, 1.4], [5.2, 3.4], [6.7, 0.0]])
+case("sort_particles_by_x", "test/test_sort.cpp:293-326", "compare_sort", "float32",
+     records=[[0.1, 0.0], [-0.4, 0.0], [10.0, 0.0], [0.001, 0.0]],
+     cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False},
+     expected=[[-0.4, 0.0], [0.001, 0.0], [0.1, 0.0], [10.0, 0.0]])
+_int2 = [[0, 0]] * 100
+_int2 = [list(r) for r in _int2]
+_int2[0], _int2[25], _int2[75], _int2[97], _int2[51] = [100, 0], [20, 0], [9, 0], [-10, 0], [-10, -1]
+case("sort_int2_by_x", "test/test_sort.cpp:328-365", "compare_sort", "int32", records=_int2,
+     cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False},
+     expected_x={"0": -10, "1": -10, "97": 9, "98": 20, "99": 100}, sorted=True)
+case("sort_by_key_int_custom_less", "test/test_sort_by_key.cpp:113-142", "compare_sort", "int32",
+     records=_gen(1024, lambda i: 1024 - i), values=_gen(1024, lambda i: (1024 - i) / 2.0), values_dtype="float32",
+     cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False}, sorted=True, values_sorted=True)
+_sk = _gen(128, lambda i: -i)
+_sv = _gen(128, lambda i: -i)
+_sk[64], _sk[126], _sk[127] = -256, -256, -256
+_sv[64], _sv[126], _sv[127] = 3, 1, 2
+case("stable_sort_by_key_int_custom_less", "test/test_stable_sort_by_key.cpp:165-208", "compare_sort", "int32",
+     records=_sk, values=_sv, values_dtype="int32", cmp={"form": "field", "lhs": 0, "rhs": 0, "greater": False},
+     stable=True, sorted=True, expected_prefix=[-256, -256, -256], expected_values_prefix=[3, 1, 2])
+
 if __name__ == "__main__":
     out = os.path.join(os.path.dirname(os.path.abspath(__file__)), "reference_vectors.json")
     with open(out, "w") as f:

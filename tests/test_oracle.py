@@ -710,3 +710,125 @@ def test_sort_by_transform_oracle_is_a_stable_sort_by_the_key():
     got = sort_by_transform_oracle(rec[:, 1].copy(), None, "abs", True)
     k = np.abs(rec[:, 1])
     assert np.array_equal(got, rec[:, 1][np.argsort(-k, kind="stable")])
+
+
+# ------------------------------------------------------------ comparator sorts
+def compare_case_arrays(c):
+    """A golden compare_sort case as (records array, record_bytes, values array or None)."""
+    dt = np.dtype(c["dtype"])
+    rec = np.array(c["records"], dt)
+    rb = dt.itemsize * (rec.shape[1] if rec.ndim == 2 else 1)
+    vals = np.array(c["values"], c["values_dtype"]) if "values" in c else None
+    return rec.reshape(-1), rb, vals
+
+
+def compare_case_cmp(c):
+    m = c["cmp"]
+    sz = np.dtype(c["dtype"]).itemsize
+    return dict(form=m["form"], lhs=m["lhs"] * sz, rhs=m["rhs"] * sz, second=m.get("second", 0) * sz,
+                count=m.get("count", 0), greater=m["greater"])
+
+
+def check_compare_case(c, out, out_vals):
+    """The reference test's own checks on a sorted output (flat arrays)."""
+    rec, rb, vals = compare_case_arrays(c)
+    dt = np.dtype(c["dtype"])
+    comps = rb // dt.itemsize
+    got = out.reshape(-1, comps) if comps > 1 else out
+    if "expected" in c:
+        assert got.tolist() == np.array(c["expected"], dt).tolist(), c["id"]
+    for pos, x in c.get("expected_x", {}).items():
+        assert got[int(pos)][0] == x, (c["id"], pos)
+    if "expected_prefix" in c:
+        assert got[:len(c["expected_prefix"])].tolist() == c["expected_prefix"], c["id"]
+        assert out_vals[:len(c["expected_values_prefix"])].tolist() == c["expected_values_prefix"], c["id"]
+    if c.get("sorted"):
+        k = compare_case_cmp(c)
+        less = O.comparator_less_py(dt, k["form"], k["lhs"], k["rhs"], k["second"], k["count"], k["greater"])
+        rows = O._as_rows(out, rb)
+        assert not any(less(rows[i + 1], rows[i]) for i in range(rows.shape[0] - 1)), c["id"]
+    if c.get("values_sorted"):
+        assert np.all(out_vals[1:] >= out_vals[:-1]), c["id"]
+    # a permutation of the input, records and values moved together
+    pairs = sorted(zip(map(bytes, O._as_rows(rec, rb)), vals.tolist() if vals is not None else [0] * len(out)))
+    got_pairs = sorted(zip(map(bytes, O._as_rows(out, rb)),
+                           out_vals.tolist() if out_vals is not None else [0] * len(out)))
+    assert pairs == got_pairs, c["id"]
+
+
+def test_golden_compare_sort(golden):
+    seen = 0
+    for c in golden:
+        if c["algo"] != "compare_sort":
+            continue
+        seen += 1
+        rec, rb, vals = compare_case_arrays(c)
+        out, out_vals = O.comparator_sort(rec, rb, c["dtype"], values=vals, **compare_case_cmp(c))
+        check_compare_case(c, out, out_vals)
+    assert seen == 13
+
+
+CMP_DTYPES = ["int8", "uint8", "int16", "uint16", "int32", "uint32", "int64", "uint64", "float32", "float64"]
+
+
+def _cmp_records(dt, n, comps, rng):
+    dt = np.dtype(dt)
+    if dt.kind == "f":
+        x = rng.integers(-6, 6, (n, comps)).astype(dt) * dt.type(0.5)  # ties, -0 and +0
+        x[rng.random((n, comps)) < 0.1] *= dt.type(-1)
+        return x
+    info = np.iinfo(dt)
+    x = rng.integers(info.min, info.max, (n, comps), endpoint=True, dtype=dt)
+    small = rng.integers(max(info.min, -3), 4, (n, comps)).astype(dt)  # ties
+    pick = rng.random((n, comps)) < 0.5
+    x[pick] = small[pick]
+    x[rng.random((n, comps)) < 0.05] = info.min
+    x[rng.random((n, comps)) < 0.05] = info.max
+    return x
+
+
+@pytest.mark.parametrize("dt", CMP_DTYPES)
+def test_comparator_keys_match_the_insertion_sort(dt):
+    # the vectorised key path against the reference's insertion sort driven by
+    # the pairwise comparator in exact Python arithmetic (wrap-around, abs of
+    # INT_MIN, -0 == +0, promotion of narrow integers)
+    rng = np.random.default_rng(99)
+    forms = ["field", "abs", "abs_sum2"] + (["length"] if np.dtype(dt).kind == "f" else [])
+    sz = np.dtype(dt).itemsize
+    for comps in (1, 2, 4):
+        for form in forms:
+            if form == "abs_sum2" and comps == 1:
+                continue
+            for greater in (False, True):
+                rec = _cmp_records(dt, 97, comps, rng)
+                rows = O._as_rows(rec, comps * sz)
+                kw = dict(form=form, lhs=sz * (comps - 1 if form == "field" else 0), second=sz * (comps - 1),
+                          count=comps, greater=greater)
+                vals = np.arange(97, dtype=np.int32)
+                out, ov = O.comparator_sort(rec, comps * sz, dt, values=vals, **kw)
+                less = O.comparator_less_py(dt, kw["form"], kw["lhs"], kw["lhs"], kw["second"], kw["count"],
+                                            greater)
+                want, wv = O.insertion_sort_rows(rows, less, list(vals))
+                assert bytes(O._as_rows(out, comps * sz)) == bytes(want.astype(np.uint8)), (comps, form, greater)
+                assert ov.tolist() == wv, (comps, form, greater)
+
+
+def test_comparator_oracle_catches_a_wrapped_abs(monkeypatch):
+    # a planted mistake: abs(INT_MIN) kept as INT_MIN (a wrapping abs) sorts
+    # INT_MIN first instead of last, which the insertion-sort check sees
+    rec = np.array([3, -2147483648, -5, 7], np.int32)
+    good, _ = O.comparator_sort(rec, 4, np.int32, form="abs")
+    assert good.tolist() == [3, -5, 7, -2147483648]
+    monkeypatch.setattr(O, "_magnitude_u64", lambda s: np.abs(s).astype(np.int64))
+    bad, _ = O.comparator_sort(rec, 4, np.int32, form="abs")
+    less = O.comparator_less_py(np.int32, "abs", 0)
+    want, _ = O.insertion_sort_rows(O._as_rows(rec, 4), less)
+    assert bytes(O._as_rows(bad, 4)) != bytes(want.astype(np.uint8))
+
+
+def test_comparator_oracle_asymmetric_pairwise():
+    # a.s0 < b.s3 on long8_ records whose components are all equal is the
+    # ordinary order of the components (test/test_merge_sort_gpu.cpp:299-330)
+    rec = np.repeat(np.array([5, -1, 9, 5, 0, 2], np.int64)[:, None], 8, axis=1)
+    out, _ = O.comparator_sort(rec, 64, np.int64, form="field", lhs=0, rhs=24)
+    assert out.reshape(-1, 8)[:, 0].tolist() == [-1, 0, 2, 5, 5, 9]
