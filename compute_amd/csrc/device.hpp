@@ -17,6 +17,24 @@ namespace bcmi {
 #define BDEV __device__ __forceinline__
 #define BHD __host__ __device__ __forceinline__
 
+// Work-group barrier that first waits for the wave's own LDS operations.
+// __syncthreads() alone lets a wave pass the barrier with LDS reads still
+// outstanding (the compiler waits for them only where it uses the data),
+// so another wave's LDS writes after the barrier can land first; with
+// several work-groups resident per CU that returned overwritten data.  Every
+// kernel here reuses LDS slots across barriers, so every barrier is this one.
+BDEV void block_barrier()
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    __syncthreads();
+}
+
+BDEV int block_barrier_or(int pred)
+{
+    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
+    return __syncthreads_or(pred);
+}
+
 constexpr int kWave = 64;
 
 BDEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
@@ -238,7 +256,7 @@ BDEV void to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t 
         c.w = __builtin_bit_cast(uint32_t, s[q][3]);
         *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
     }
-    __syncthreads();
+    block_barrier();
 #pragma unroll
     for (int e = 0; e < E; e += 4) {
         const uint4 c = *(const uint4 *)(buf + lane * ROW + e);
@@ -247,7 +265,7 @@ BDEV void to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t 
         b[e + 2] = __builtin_bit_cast(T, c.z);
         b[e + 3] = __builtin_bit_cast(T, c.w);
     }
-    __syncthreads();
+    block_barrier();
 }
 
 // this lane's row -> striped chunks of one wave
@@ -264,7 +282,7 @@ BDEV void to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t 
         c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
         *(uint4 *)(buf + lane * ROW + e) = c;
     }
-    __syncthreads();
+    block_barrier();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int p = q * 256 + (int)lane * 4;
@@ -274,7 +292,7 @@ BDEV void to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t 
         s[q][2] = __builtin_bit_cast(T, c.z);
         s[q][3] = __builtin_bit_cast(T, c.w);
     }
-    __syncthreads();
+    block_barrier();
 }
 
 // ------------------------------------------- wave-parallel decoupled look-back

@@ -70,19 +70,6 @@ BDEV bool item_less(const ItemS32 &a, const ItemS32 &b) { return a.k < b.k; }
 BDEV bool item_less(const ItemS64 &a, const ItemS64 &b) { return a.k < b.k; }
 BDEV bool item_less(const ItemA64 &a, const ItemA64 &b) { return a.kl < b.kr; }
 
-// Barrier between rounds that overwrite the LDS slots the previous round
-// read.  __syncthreads() alone lets a wave pass the barrier with its LDS
-// reads still outstanding (the compiler waits for them only where it uses the
-// data), and another wave's writes after the barrier can then land first:
-// with several work-groups resident per CU this returned next-round items
-// (duplicated and lost elements).  Waiting for this wave's LDS operations
-// first closes that window.
-BDEV void lds_barrier()
-{
-    asm volatile("s_waitcnt lgkmcnt(0)" ::: "memory");
-    __syncthreads();
-}
-
 // Field-wise loads: a whole-struct copy through a selected pointer stays a
 // memcpy into private memory.
 BDEV ItemS32 ld_item(const ItemS32 *p) { return ItemS32{p->k, p->idx}; }
@@ -269,7 +256,7 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
         const size_t g = base + i;
         lds[i] = g < n ? make_item<T, Item>(rec + g * rb, c, (uint32_t)g) : pad_item<Item>();
     }
-    lds_barrier();
+    block_barrier();
     Item x[IPT];
 #pragma unroll
     for (int k = 0; k < IPT; ++k) x[k] = lds[t * IPT + k];
@@ -289,10 +276,10 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
     }
 
     for (int w = IPT; w < TILE; w *= 2) {
-        lds_barrier();
+        block_barrier();
 #pragma unroll
         for (int k = 0; k < IPT; ++k) lds[t * IPT + k] = x[k];
-        lds_barrier();
+        block_barrier();
         const int o = t * IPT;
         const int p0 = o & ~(2 * w - 1);
         const int d = o - p0;
@@ -302,10 +289,10 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
         lds_merge<Item, IPT>(A, w, B, w, i, d - i, IPT, x);
     }
 
-    lds_barrier();
+    block_barrier();
 #pragma unroll
     for (int k = 0; k < IPT; ++k) lds[t * IPT + k] = x[k];
-    lds_barrier();
+    block_barrier();
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
         const int i = k * kMsThreads + t;
@@ -367,7 +354,7 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict_
         const size_t i = wave_path(in + p0, na_all, in + a_end, nb_all, wave ? d1 : d0, lane);
         if (lane == 0) split[wave] = i;
     }
-    lds_barrier();
+    block_barrier();
     const size_t i0 = split[0];
     // a comparator that is not a strict weak order can make the end split
     // inconsistent with the start; clamping keeps every access in bounds
@@ -376,18 +363,18 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict_
     const int na = (int)(i1 - i0), nb = (int)(j1 - j0), cnt = na + nb;
 
     for (int x = t; x < cnt; x += kMsThreads) lds[x] = ld_item(x < na ? in + (p0 + i0 + x) : in + (a_end + j0 + (x - na)));
-    lds_barrier();
+    block_barrier();
 
     // threads past the end run the merge with no outputs (d = cnt)
     Item x[IPT];
     const int dl = t * IPT < cnt ? t * IPT : cnt;
     const int il = lds_path(lds, na, lds + na, nb, dl);
     lds_merge<Item, IPT>(lds, na, lds + na, nb, il, dl - il, cnt - dl, x);
-    lds_barrier();
+    block_barrier();
 #pragma unroll
     for (int k = 0; k < IPT; ++k)
         if (dl + k < cnt) lds[dl + k] = x[k];
-    lds_barrier();
+    block_barrier();
     for (int y = t; y < cnt; y += kMsThreads) {
         if (e.rdst)
             emit_one(e, o0 + y, lds[y].idx);

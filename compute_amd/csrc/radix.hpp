@@ -148,7 +148,7 @@ BDEV uint32_t block_exclusive_sum256(uint32_t v, uint32_t *s_wsum)
     const uint32_t lane = threadIdx.x & 63, w = threadIdx.x >> 6;
     uint32_t inc = wave_inclusive_scan(v, 0u, [](uint32_t a, uint32_t b) { return a + b; });
     if (lane == 63) s_wsum[w] = inc;
-    __syncthreads();
+    block_barrier();
     uint32_t pre = inc - v;
     for (uint32_t i = 0; i < w; ++i) pre += s_wsum[i];
     return pre;
@@ -194,7 +194,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
         (&L.cnt[0][0])[i] = 0;
         (&s_peer[0][0])[i] = 0;
     }
-    __syncthreads();
+    block_barrier();
     const uint32_t base = tile * (uint32_t)TILE;
     // slot j of this lane (tile position w*64*IPT + j*64 + lane) is valid iff j*64 + lane < lim
     const int32_t lim = FULL ? TILE : (int32_t)(n - base) - (int32_t)(w * 64 * IPT);
@@ -221,7 +221,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     }
     Ranks<IPT> r;
     rank_in_wave<IPT, FULL, RANK>(k, r, xf, shift, L.cnt[w], s_peer[w], lim);
-    __syncthreads();
+    block_barrier();
 
     // per digit (thread tid == digit): wave prefixes and the tile count
     const uint32_t c0 = L.cnt[0][tid], c1 = L.cnt[1][tid], c2 = L.cnt[2][tid], c3 = L.cnt[3][tid];
@@ -231,7 +231,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     else lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_A, total));
     uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
     if (lane == 63) L.wsum[w] = inc;
-    __syncthreads();
+    block_barrier();
     uint32_t lstart = inc - total;
     for (uint32_t i = 0; i < w; ++i) lstart += L.wsum[i];
     L.cnt[0][tid] = lstart;
@@ -281,7 +281,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     // inconsistent offset (a corrupted chain) writes nothing and reports.
     const bool bad = (uint64_t)goff0[tid] + excl + total > n;
     L.goff[tid] = goff0[tid] + excl - lstart;
-    if (__syncthreads_or(bad)) {
+    if (block_barrier_or(bad)) {
         if (tid == 0) __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
@@ -291,7 +291,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     for (int j = 0; j < IPT; ++j) {
         if (valid(j)) s_keys[L.cnt[w][xf.digit(k[j], shift)] + r[j]] = k[j];
     }
-    __syncthreads();
+    block_barrier();
     const uint32_t tile_n = FULL ? (uint32_t)TILE : n - base;
     uint32_t od[VB > 0 ? IPT : 1];
 #pragma unroll
@@ -305,12 +305,12 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
         }
     }
     if constexpr (VB > 0) {
-        __syncthreads();
+        block_barrier();
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             if (valid(j)) s_vals[L.cnt[w][xf.digit(k[j], shift)] + r[j]] = v[j];
         }
-        __syncthreads();
+        block_barrier();
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             const uint32_t i = j * kSortThreads + tid;

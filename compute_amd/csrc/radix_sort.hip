@@ -46,7 +46,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
     constexpr int NW = kHistThreads / 32;  // sub-histograms: one per half-wave
     __shared__ uint32_t s_hist[NW][P][256];
     for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
-    __syncthreads();
+    block_barrier();
     uint32_t(*h)[256] = s_hist[threadIdx.x >> 5];
     auto count = [&](KB x) {
         KB k = xf.key(x);
@@ -75,7 +75,7 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
 #pragma unroll
         for (int k = 0; k < kchunk<KB>::N; ++k) count(a.v[k]);
     }
-    __syncthreads();
+    block_barrier();
     uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
     for (int i = threadIdx.x; i < P * 256; i += kHistThreads) {
         uint32_t t = 0;
@@ -98,12 +98,12 @@ __global__ __launch_bounds__(1024) void k_digit_offsets(const uint32_t *__restri
 #pragma unroll 8
     for (uint32_t b = slice; b < nparts; b += 4) c += partial[((size_t)b * passes + p) * 256 + d];
     s_part[slice][d] = c;
-    __syncthreads();
+    block_barrier();
     const uint32_t lane = d & 63, w = d >> 6;
     c = s_part[0][d] + s_part[1][d] + s_part[2][d] + s_part[3][d];
     uint32_t inc = wave_inclusive_scan(c, 0u, [](uint32_t a, uint32_t b) { return a + b; });
     if (slice == 0 && lane == 63) s_wsum[w] = inc;
-    __syncthreads();
+    block_barrier();
     if (slice == 0) {
         uint32_t pre = 0;
         for (uint32_t i = 0; i < w; ++i) pre += s_wsum[i];

@@ -121,20 +121,20 @@ BDEV S block_combine(S v, S *s_buf)
     if constexpr (sizeof(S) == 4 || sizeof(S) == 8) {
         v = wave_reduce(v, P::id(), [](S a, S b) { return P::merge(a, b); });
         if (lane == 0) s_buf[w] = v;
-        __syncthreads();
+        block_barrier();
         if (threadIdx.x == 0) {
             S acc = s_buf[0];
             for (int i = 1; i < kRedThreads / 64; ++i) acc = P::merge(acc, s_buf[i]);
             s_buf[0] = acc;
         }
-        __syncthreads();
+        block_barrier();
         return s_buf[0];
     } else {
         s_buf[threadIdx.x] = v;
-        __syncthreads();
+        block_barrier();
         for (int off = kRedThreads / 2; off > 0; off >>= 1) {
             if ((int)threadIdx.x < off) s_buf[threadIdx.x] = P::merge(s_buf[threadIdx.x], s_buf[threadIdx.x + off]);
-            __syncthreads();
+            block_barrier();
         }
         return s_buf[0];
     }
@@ -152,10 +152,10 @@ __global__ __launch_bounds__(kRedThreads) void k_reduce_final(const typename P::
     Pt p = P::part_id();
     for (uint32_t i = threadIdx.x; i < nparts; i += kRedThreads) p = P::part_merge(p, partials[i]);
     s_part[threadIdx.x] = p;
-    __syncthreads();
+    block_barrier();
     for (int off = kRedThreads / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) s_part[threadIdx.x] = P::part_merge(s_part[threadIdx.x], s_part[threadIdx.x + off]);
-        __syncthreads();
+        block_barrier();
     }
     if (threadIdx.x == 0) {
         Pt total = s_part[0];

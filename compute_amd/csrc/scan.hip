@@ -127,7 +127,7 @@ BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, ui
         carry = P::merge(carry, readlane(inc, 63));
     }
     if (lane == 0) L.wave[w] = carry;
-    __syncthreads();
+    block_barrier();
     S wpre = P::id();
     for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, L.wave[i]);
     if (w == 0) {
@@ -137,7 +137,7 @@ BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, ui
                                             [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
         if (lane == 0) L.prefix = prefix;
     }
-    __syncthreads();
+    block_barrier();
     const Cr prefix = L.prefix;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {

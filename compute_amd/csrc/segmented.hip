@@ -109,11 +109,11 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
     auto lds_pass = [&](auto dig) {
         Ranks<IPT> r;
         for (int i = lane; i < 256; i += 64) s_cnt[w][i] = 0;
-        __syncthreads();
+        block_barrier();
         // every thread read the previous pass's flag before the barrier above
         if (tid == 0) s_uniform = 0;
         rank_in_wave_by<IPT, false, kRankLdsMask>(k, r, dig, s_cnt[w], s_peer[w], lim);
-        __syncthreads();
+        block_barrier();
         const uint32_t c0 = s_cnt[0][tid], c1 = s_cnt[1][tid], c2 = s_cnt[2][tid], c3 = s_cnt[3][tid];
         const uint32_t total = c0 + c1 + c2 + c3;
         if (total == len) s_uniform = 1;
@@ -122,7 +122,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
         s_cnt[1][tid] = lstart + c0;
         s_cnt[2][tid] = lstart + c0 + c1;
         s_cnt[3][tid] = lstart + c0 + c1 + c2;
-        __syncthreads();
+        block_barrier();
         if (s_uniform) return;
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
@@ -132,7 +132,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
                 if constexpr (VB > 0) s_v[dst] = v[j];
             }
         }
-        __syncthreads();
+        block_barrier();
 #pragma unroll
         for (int j = 0; j < IPT; ++j) {
             if (valid(j)) {
@@ -162,7 +162,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
             s_mn[w] = mn;
             s_mx[w] = mx;
         }
-        __syncthreads();
+        block_barrier();
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             mn = s_mn[i] < mn ? s_mn[i] : mn;
@@ -195,7 +195,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
                 if constexpr (VB > 0) s_v[pos(j)] = v[j];
             }
         }
-        __syncthreads();
+        block_barrier();
         uint32_t rlen[IPT];
         bool big = false;
 #pragma unroll
@@ -210,7 +210,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
             rlen[j] = e - i;
             big = big || rlen[j] > kBucketMax;
         }
-        if (!__syncthreads_or(big)) {
+        if (!block_barrier_or(big)) {
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 const uint32_t b0 = pos(j);
@@ -229,7 +229,7 @@ __global__ __launch_bounds__(kSegSmallThreads) void k_seg_small(const KB *ksrc, 
                     if constexpr (VB > 0) s_v[b0 + q] = val;
                 }
             }
-            __syncthreads();
+            block_barrier();
 #pragma unroll
             for (int j = 0; j < IPT; ++j) {
                 if (valid(j)) {
@@ -309,7 +309,7 @@ __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__rest
     const SegSpan c = chunks[blockIdx.x];
     if (route && route[c.idx] != kRouteLsd) return;
     for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
-    __syncthreads();
+    block_barrier();
     uint32_t(*h)[256] = s_hist[threadIdx.x >> 6];
     const KB *kc = keys + c.base;
     constexpr int U = 8;  // loads in flight per thread
@@ -328,7 +328,7 @@ __global__ __launch_bounds__(kHistThreads) void k_seg_histogram(const KB *__rest
             }
         }
     }
-    __syncthreads();
+    block_barrier();
     uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
     for (uint32_t i = p0 * 256 + threadIdx.x; i < p1 * 256; i += kHistThreads) {
         uint32_t t = 0;
@@ -386,9 +386,9 @@ __global__ __launch_bounds__(256) void k_seg_route(const SegSpan *__restrict__ l
     const uint32_t *off = seg_off + ((size_t)s * passes + passes - 1) * 256;
     s_start[d] = off[d];
     if (d == 0) s_start[256] = sg.len;
-    __syncthreads();
+    block_barrier();
     const uint32_t size = s_start[d + 1] - s_start[d];
-    const int big = __syncthreads_or(size > cap);
+    const int big = block_barrier_or(size > cap);
     const uint32_t b0 = slot_begin[s], b1 = slot_begin[s + 1];
     if (d != 0) return;
     // count the greedy groups first, so the route is decided before anything is written

@@ -185,7 +185,7 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
         carry = mrg(carry, readlane(inc, 63));
     }
     if (lane == 0) L.wave[w] = carry;
-    __syncthreads();
+    block_barrier();
     S wpre = sid;
     for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, L.wave[i]);
     if (w == 0) {
@@ -198,7 +198,7 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
         }, ctrl);
         if (lane == 0) L.prefix = prefix;
     }
-    __syncthreads();
+    block_barrier();
     const C prefix = L.prefix;
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
@@ -345,7 +345,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     const S inc = wave_inclusive_scan(t, sid, mrg);
     const S pre = wave_shr1(inc, sid);
     if (lane == 63) L.wave[w] = inc;
-    __syncthreads();
+    block_barrier();
     S wpre = sid;
     for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, L.wave[i]);
     if (w == 0) {
@@ -358,7 +358,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
         }, ctrl);
         if (lane == 0) L.prefix = prefix;
     }
-    __syncthreads();
+    block_barrier();
     const C prefix = L.prefix;
     S s = mrg(wpre, pre);  // exclusive local pair before this lane's first element
     VT y[E];

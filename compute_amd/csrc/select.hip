@@ -291,7 +291,7 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
         }
     }
     if (lane == 0) L.wcnt[w] = running;
-    __syncthreads();
+    block_barrier();
     uint32_t wpre = 0;
     for (uint32_t i = 0; i < w; ++i) wpre += L.wcnt[i];
     const uint32_t cnt = L.wcnt[0] + L.wcnt[1] + L.wcnt[2] + L.wcnt[3];
@@ -347,7 +347,7 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
                 }
             }
         }
-        __syncthreads();
+        block_barrier();
         if (L.bad) return;
         const uint32_t prefix = L.prefix + rh;  // output index of the round's first selected element
         if constexpr (PART) {
@@ -369,7 +369,7 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
             T *o = (T *)out_a;
             for (uint32_t i = tid; i < ch; i += kSelThreads) o[(size_t)prefix + i] = L.stage[i];
         }
-        if (h + 1 < ROUNDS) __syncthreads();
+        if (h + 1 < ROUNDS) block_barrier();
     }
 }
 
@@ -458,7 +458,7 @@ __global__ __launch_bounds__(kSelThreads) void k_count_if(const T *in, size_t he
     }
     c = wave_reduce(c, 0u, [](uint32_t a, uint32_t b) { return a + b; });
     if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = c;
-    __syncthreads();
+    block_barrier();
     if (threadIdx.x == 0) partials[blockIdx.x] = (uint64_t)s_w[0] + s_w[1] + s_w[2] + s_w[3];
 }
 
@@ -469,10 +469,10 @@ __global__ __launch_bounds__(kSelThreads) void k_count_final(const uint64_t *par
     uint64_t c = 0;
     for (uint32_t i = threadIdx.x; i < nparts; i += kSelThreads) c += partials[i];
     s[threadIdx.x] = c;
-    __syncthreads();
+    block_barrier();
     for (int off = kSelThreads / 2; off > 0; off >>= 1) {
         if ((int)threadIdx.x < off) s[threadIdx.x] += s[threadIdx.x + off];
-        __syncthreads();
+        block_barrier();
     }
     if (threadIdx.x == 0) *count = s[0];
 }
