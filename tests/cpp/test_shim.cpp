@@ -516,6 +516,31 @@ int main()
         for (size_t i = 0; i < n; ++i) ok = ok && h[i][0] == d[i][0] && h[i][1] == d[i][1];
         CHECK(ok);
     }
+    {  // mapped_view over host arrays: test/test_mapped_view.cpp:24-66
+        int data[] = {1, 2, 3, 4, 5, 6, 7, 8};
+        bc::mapped_view<int> view(data, 8, queue.get_context());
+        bc::fill(view.begin(), view.end(), 4, queue);
+        view.map(CL_MAP_READ, queue);
+        CHECK(std::all_of(data, data + 8, [](int x) { return x == 4; }));
+        view.unmap(queue);
+        int d2[] = {5, 2, 3, 1, 8, 7, 4, 9};
+        bc::mapped_view<int> v2(d2, 8, queue.get_context());
+        bc::sort(v2.begin(), v2.end(), queue);
+        v2.map(CL_MAP_READ, queue);
+        CHECK(d2[0] == 1 && d2[7] == 9);
+        v2.unmap(queue);
+        int d3[] = {5, 2, 3, 1, 8, 7, 4, 9};
+        bc::mapped_view<int> v3(d3, 8, queue.get_context());
+        int sum = 0;
+        bc::reduce(v3.begin(), v3.end(), &sum, queue);
+        CHECK(sum == 39);
+        // host writes under a write mapping reach the device at unmap
+        v3.map(queue);
+        d3[0] = 100;
+        v3.unmap(queue);
+        bc::reduce(v3.begin(), v3.end(), &sum, queue);
+        CHECK(sum == 134);
+    }
     {  // errors surface as opencl_error with the OpenCL-valued code
         bool thrown = false;
         try {
