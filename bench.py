@@ -361,7 +361,7 @@ def main():
         fn = lambda: ca.merge_sort(q, cmp, work, 8, n)
         ms, _ = timed_steps(ca, q, fn, K, Wm, lambda: d2d(ca, q, work, src))
         t = float(np.mean(ms))
-        kern = kernel_breakdown(q, fn, ["merge_sort_block", "merge_sort_merge"], restore=lambda: d2d(ca, q, work, src))
+        kern = kernel_breakdown(q, fn, ["merge_sort_block", "merge_sort_split", "merge_sort_merge"], restore=lambda: d2d(ca, q, work, src))
         passes = int(np.ceil(np.log2(n / 4096)))
         merge_ms = kern.get("merge_sort_merge", float("nan")) / passes
         sec["merge_sort_int2_by_x_2p24"] = {

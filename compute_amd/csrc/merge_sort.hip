@@ -23,11 +23,13 @@
 //                thread's IPT items in registers, then log2(256) merge rounds
 //                in LDS, each thread placing its IPT outputs by a merge-path
 //                search; writes sorted runs of TILE items.
+//   k_ms_split   the merge-path split at the first output of every merge
+//                tile, one wave per tile: a 64-ary search (one ballot per
+//                round over 64 probes), all tiles at once.
 //   k_ms_merge   one merge pass of runs of length w: each block produces
-//                TILE consecutive outputs; waves 0 and 1 find the merge-path
-//                splits of the block's first and last output with a 64-ary
-//                search (one ballot per round over 64 probes), the block
-//                stages its A and B slices in LDS and merges them as above.
+//                TILE consecutive outputs between its split and the next
+//                tile's, stages its A and B slices in LDS and merges them as
+//                above.
 //   emit         the last launch (the block sort when n <= TILE, else the
 //                last merge pass) writes no items: it gathers each output's
 //                record and value from copies of the inputs, by the item's
@@ -210,15 +212,25 @@ BDEV void emit_one(const Emit &e, size_t pos, uint32_t idx)
     if (e.vdst) copy_words(e.vdst + pos * e.vb, e.vsrc + (size_t)idx * e.vb, e.vb, e.vw);
 }
 
+// LDS position of tile item i: one padding slot after every IPT items, so
+// the IPT consecutive items of consecutive threads start IPT + 1 slots apart
+// and a wave's per-thread accesses spread over the banks instead of all
+// landing on the same ones.
+template <int IPT>
+BDEV int lpos(int i)
+{
+    return i + i / IPT;
+}
+
 // Merge path inside LDS: the number of A items among the first d outputs of
-// the stable merge of A[0, na) and B[0, nb).
-template <typename Item>
-BDEV int lds_path(const Item *A, int na, const Item *B, int nb, int d)
+// the stable merge of A = tile items [a0, a0 + na) and B = [b0, b0 + nb).
+template <typename Item, int IPT>
+BDEV int lds_path(const Item *lds, int a0, int na, int b0, int nb, int d)
 {
     int lo = d > nb ? d - nb : 0, hi = d < na ? d : na;
     while (lo < hi) {
         int mid = (lo + hi) >> 1;
-        if (!item_less(B[d - 1 - mid], A[mid]))
+        if (!item_less(lds[lpos<IPT>(b0 + d - 1 - mid)], lds[lpos<IPT>(a0 + mid)]))
             lo = mid + 1;
         else
             hi = mid;
@@ -229,13 +241,13 @@ BDEV int lds_path(const Item *A, int na, const Item *B, int nb, int d)
 // Serial stable merge of IPT outputs starting at (i, j); outputs past `cnt`
 // (the merged length minus d) hold a copy of A[0] and are not to be used.
 template <typename Item, int IPT>
-BDEV void lds_merge(const Item *A, int na, const Item *B, int nb, int i, int j, int cnt, Item (&x)[IPT])
+BDEV void lds_merge(const Item *lds, int a0, int na, int b0, int nb, int i, int j, int cnt, Item (&x)[IPT])
 {
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
-        const bool take_a = j >= nb || (i < na && !item_less(B[j], A[i]));
-        const Item *src = take_a ? A + i : B + j;
-        x[k] = ld_item(k < cnt ? src : A);  // one load through the chosen address, always in bounds
+        const bool take_a = j >= nb || (i < na && !item_less(lds[lpos<IPT>(b0 + j)], lds[lpos<IPT>(a0 + i)]));
+        const int src = take_a ? a0 + i : b0 + j;
+        x[k] = ld_item(&lds[lpos<IPT>(k < cnt ? src : a0)]);  // one load, always in bounds
         i += take_a ? 1 : 0;
         j += take_a ? 0 : 1;
     }
@@ -246,7 +258,7 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
                                                         size_t n, CmpDev c, Item *__restrict__ out, Emit e)
 {
     constexpr int TILE = kMsThreads * IPT;
-    __shared__ Item lds[TILE];
+    __shared__ Item lds[TILE + TILE / IPT];
     const int t = threadIdx.x;
     const size_t base = (size_t)blockIdx.x * TILE;
 
@@ -254,12 +266,12 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
     for (int k = 0; k < IPT; ++k) {
         const int i = k * kMsThreads + t;
         const size_t g = base + i;
-        lds[i] = g < n ? make_item<T, Item>(rec + g * rb, c, (uint32_t)g) : pad_item<Item>();
+        lds[lpos<IPT>(i)] = g < n ? make_item<T, Item>(rec + g * rb, c, (uint32_t)g) : pad_item<Item>();
     }
     block_barrier();
     Item x[IPT];
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) x[k] = lds[t * IPT + k];
+    for (int k = 0; k < IPT; ++k) x[k] = lds[lpos<IPT>(t * IPT + k)];
 
     // odd-even transposition sort: IPT rounds sort IPT items; swapping only
     // strictly smaller right neighbours keeps it stable
@@ -278,20 +290,18 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
     for (int w = IPT; w < TILE; w *= 2) {
         block_barrier();
 #pragma unroll
-        for (int k = 0; k < IPT; ++k) lds[t * IPT + k] = x[k];
+        for (int k = 0; k < IPT; ++k) lds[lpos<IPT>(t * IPT + k)] = x[k];
         block_barrier();
         const int o = t * IPT;
         const int p0 = o & ~(2 * w - 1);
         const int d = o - p0;
-        const Item *A = lds + p0;
-        const Item *B = A + w;
-        const int i = lds_path(A, w, B, w, d);
-        lds_merge<Item, IPT>(A, w, B, w, i, d - i, IPT, x);
+        const int i = lds_path<Item, IPT>(lds, p0, w, p0 + w, w, d);
+        lds_merge<Item, IPT>(lds, p0, w, p0 + w, w, i, d - i, IPT, x);
     }
 
     block_barrier();
 #pragma unroll
-    for (int k = 0; k < IPT; ++k) lds[t * IPT + k] = x[k];
+    for (int k = 0; k < IPT; ++k) lds[lpos<IPT>(t * IPT + k)] = x[k];
     block_barrier();
 #pragma unroll
     for (int k = 0; k < IPT; ++k) {
@@ -299,9 +309,9 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_block(const unsigned char *__
         const size_t g = base + i;
         if (g < n) {
             if (e.rdst)
-                emit_one(e, g, lds[i].idx);
+                emit_one(e, g, lds[lpos<IPT>(i)].idx);
             else
-                out[g] = lds[i];
+                out[g] = lds[lpos<IPT>(i)];
         }
     }
 }
@@ -335,13 +345,28 @@ BDEV size_t wave_path(const Item *A, size_t na, const Item *B, size_t nb, size_t
     return lo;
 }
 
+// The merge-path split at the first output of every merge tile: one wave
+// per tile, all tiles at once, so the merge kernel starts streaming at once.
 template <typename Item, int IPT>
-__global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict__ in, Item *__restrict__ out, size_t n,
-                                                        size_t w, Emit e)
+__global__ __launch_bounds__(kMsThreads) void k_ms_split(const Item *__restrict__ in, size_t n, size_t w,
+                                                        size_t tiles, size_t *__restrict__ splits)
 {
     constexpr int TILE = kMsThreads * IPT;
-    __shared__ Item lds[TILE];
-    __shared__ size_t split[2];
+    const size_t b = (size_t)blockIdx.x * (kMsThreads / 64) + (threadIdx.x >> 6);
+    if (b >= tiles) return;  // whole waves
+    const size_t o0 = b * TILE;
+    const size_t p0 = o0 / (2 * w) * (2 * w);
+    const size_t a_end = std::min(p0 + w, n), b_end = std::min(p0 + 2 * w, n);
+    const size_t i = wave_path(in + p0, a_end - p0, in + a_end, b_end - a_end, o0 - p0, threadIdx.x & 63);
+    if ((threadIdx.x & 63) == 0) splits[b] = i;
+}
+
+template <typename Item, int IPT>
+__global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict__ in, Item *__restrict__ out, size_t n,
+                                                        size_t w, const size_t *__restrict__ splits, Emit e)
+{
+    constexpr int TILE = kMsThreads * IPT;
+    __shared__ Item lds[TILE + TILE / IPT];
     const int t = threadIdx.x;
     const size_t o0 = (size_t)blockIdx.x * TILE;
     const size_t p0 = o0 / (2 * w) * (2 * w);
@@ -349,37 +374,36 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict_
     const size_t na_all = a_end - p0, nb_all = b_end - a_end;
     const size_t d0 = o0 - p0, d1 = std::min(d0 + TILE, na_all + nb_all);
 
-    const int wave = t >> 6, lane = t & 63;
-    if (wave < 2) {
-        const size_t i = wave_path(in + p0, na_all, in + a_end, nb_all, wave ? d1 : d0, lane);
-        if (lane == 0) split[wave] = i;
-    }
-    block_barrier();
-    const size_t i0 = split[0];
+    // the tile ends where the next tile of the same pair starts, or at the end
+    // of the pair (all of A taken)
+    const bool last_in_pair = o0 + TILE >= p0 + 2 * w || o0 + TILE >= n;
+    const size_t i0 = splits[blockIdx.x];
+    const size_t i1_raw = last_in_pair ? na_all : splits[blockIdx.x + 1];
     // a comparator that is not a strict weak order can make the end split
     // inconsistent with the start; clamping keeps every access in bounds
-    const size_t i1 = std::min(std::max(split[1], i0), i0 + (d1 - d0));
+    const size_t i1 = std::min(std::max(i1_raw, i0), i0 + (d1 - d0));
     const size_t j0 = d0 - i0, j1 = d1 - i1;
     const int na = (int)(i1 - i0), nb = (int)(j1 - j0), cnt = na + nb;
 
-    for (int x = t; x < cnt; x += kMsThreads) lds[x] = ld_item(x < na ? in + (p0 + i0 + x) : in + (a_end + j0 + (x - na)));
+    for (int x = t; x < cnt; x += kMsThreads)
+        lds[lpos<IPT>(x)] = ld_item(x < na ? in + (p0 + i0 + x) : in + (a_end + j0 + (x - na)));
     block_barrier();
 
     // threads past the end run the merge with no outputs (d = cnt)
     Item x[IPT];
     const int dl = t * IPT < cnt ? t * IPT : cnt;
-    const int il = lds_path(lds, na, lds + na, nb, dl);
-    lds_merge<Item, IPT>(lds, na, lds + na, nb, il, dl - il, cnt - dl, x);
+    const int il = lds_path<Item, IPT>(lds, 0, na, na, nb, dl);
+    lds_merge<Item, IPT>(lds, 0, na, na, nb, il, dl - il, cnt - dl, x);
     block_barrier();
 #pragma unroll
     for (int k = 0; k < IPT; ++k)
-        if (dl + k < cnt) lds[dl + k] = x[k];
+        if (dl + k < cnt) lds[lpos<IPT>(dl + k)] = x[k];
     block_barrier();
     for (int y = t; y < cnt; y += kMsThreads) {
         if (e.rdst)
-            emit_one(e, o0 + y, lds[y].idx);
+            emit_one(e, o0 + y, lds[lpos<IPT>(y)].idx);
         else
-            out[o0 + y] = lds[y];
+            out[o0 + y] = lds[lpos<IPT>(y)];
     }
 }
 
@@ -416,11 +440,13 @@ static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, siz
     const size_t blocks = (n + TILE - 1) / TILE;
     const size_t rbytes = align256(n * rb), vbytes = vals ? align256(n * vb) : 0;
     const size_t ibytes = n > TILE ? align256(n * sizeof(Item)) : 0;
+    const size_t sbytes = n > TILE ? align256(blocks * sizeof(size_t)) : 0;
     void *sp = nullptr;
-    BCMI_TRY(scratch(q, rbytes + vbytes + 2 * ibytes, &sp));
+    BCMI_TRY(scratch(q, rbytes + vbytes + 2 * ibytes + sbytes, &sp));
     unsigned char *rcopy = (unsigned char *)sp;
     unsigned char *vcopy = rcopy + rbytes;
     Item *buf[2] = {(Item *)(rcopy + rbytes + vbytes), (Item *)(rcopy + rbytes + vbytes + ibytes)};
+    size_t *splits = (size_t *)(rcopy + rbytes + vbytes + 2 * ibytes);
 
     BCMI_HIP(hipMemcpyAsync(rcopy, rec, n * rb, hipMemcpyDeviceToDevice, q->stream));
     if (vals) BCMI_HIP(hipMemcpyAsync(vcopy, vals, n * vb, hipMemcpyDeviceToDevice, q->stream));
@@ -434,9 +460,13 @@ static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, siz
     int src = 0;
     for (size_t w = TILE; w < n; w *= 2) {
         const bool last = 2 * w >= n;
+        const size_t split_blocks = (blocks + kMsThreads / 64 - 1) / (kMsThreads / 64);
+        BCMI_LAUNCH(q, "merge_sort_split",
+                    hipLaunchKernelGGL((k_ms_split<Item, IPT>), dim3((unsigned)split_blocks), dim3(kMsThreads), 0,
+                                       q->stream, buf[src], n, w, blocks, splits));
         BCMI_LAUNCH(q, "merge_sort_merge",
                     hipLaunchKernelGGL((k_ms_merge<Item, IPT>), dim3((unsigned)blocks), dim3(kMsThreads), 0,
-                                       q->stream, buf[src], buf[src ^ 1], n, w, last ? emit : none));
+                                       q->stream, buf[src], buf[src ^ 1], n, w, splits, last ? emit : none));
         src ^= 1;
     }
     return 0;
