@@ -176,3 +176,22 @@ def test_graph_replay(queue):
         want, wv = O.comparator_sort(rec, 8, np.int32, values=vals, form="abs", lhs=4, greater=True)
         got = d.to_host(queue).view(np.int32).reshape(n, 2)
         assert got.tobytes() == want.tobytes() and dv.to_host(queue).tobytes() == wv.tobytes(), rep
+
+
+def test_sub_range_at_unaligned_offsets(queue):
+    # sort [3, n - 2) of int2_ records (24-byte offset: 8-byte copy words) with
+    # values at a 1-element offset (4-byte words); elements outside stay put
+    rng = np.random.default_rng(31)
+    n = 3 * 4096 + 17
+    rec = rng.integers(-300, 300, (n, 2)).astype(np.int32)
+    vals = rng.integers(0, 1 << 30, n + 1).astype(np.uint32)
+    d, dv = dev(queue, rec), dev(queue, vals)
+    m = n - 5
+    cmp = ca.comparator(np.int32, "field", 4)
+    ca.check(ca.lib().bcmi_merge_sort(queue.h, ca.C.byref(cmp), d.at(6), 8, dv.at(1), 4, m))  # record 3 = int32 6
+    got = d.to_host(queue).view(np.int32).reshape(n, 2)
+    gv = dv.to_host(queue)
+    want, wv = O.comparator_sort(rec[3:n - 2].copy(), 8, np.int32, values=vals[1:m + 1].copy(), form="field", lhs=4)
+    assert got[3:n - 2].tobytes() == want.tobytes() and gv[1:m + 1].tobytes() == wv.tobytes()
+    assert np.array_equal(got[:3], rec[:3]) and np.array_equal(got[n - 2:], rec[n - 2:])
+    assert gv[0] == vals[0] and np.array_equal(gv[m + 1:], vals[m + 1:])
