@@ -49,8 +49,8 @@ inline void radix_sort_by_key(buffer_iterator<K> keys_first, buffer_iterator<K> 
     bcmi_sort_call<K, V>(BCMI_RADIX_SORT, keys_first, keys_last, values_first.device_ptr(), !ascending, queue);
 }
 
-// less<T> -> ascending, greater<T> -> descending; other comparators are the
-// reference's merge_sort_on_gpu path, which this build does not provide.
+// less<T> -> ascending, greater<T> -> descending; comparator expressions take
+// the comparison sort instead (merge_sort_on_gpu.hpp).
 template <class C, class T>
 struct sort_direction;
 template <class T>
