@@ -500,6 +500,22 @@ int main()
         CHECK(gk[0] == -256 && gk[1] == -256 && gk[2] == -256 && gv[0] == 3 && gv[1] == 1 && gv[2] == 2);
         CHECK(std::is_sorted(gk.begin(), gk.end()));
     }
+    {  // a comparator sort of a host vector (the reference maps it, sort.hpp:124-148), and
+       // sort_by_key with _1 > _2 on a sub-range
+        using bc::lambda::_1;
+        using bc::lambda::_2;
+        using bc::lambda::get;
+        std::vector<bc::int2_> h = {bc::int2_(3, 0), bc::int2_(-1, 1), bc::int2_(3, 2), bc::int2_(0, 3),
+                                    bc::int2_(-1, 4)};
+        bc::sort(h.begin(), h.end(), get<0>(_1) < get<0>(_2), queue);
+        CHECK(h[0] == bc::int2_(-1, 1) && h[1] == bc::int2_(-1, 4) && h[2] == bc::int2_(0, 3) &&
+              h[3] == bc::int2_(3, 0) && h[4] == bc::int2_(3, 2));
+        std::vector<int> hk = {9, 5, 7, 5, 1, 8, 2}, hv = {0, 1, 2, 3, 4, 5, 6};
+        bc::vector<int> k(hk.begin(), hk.end(), queue), v(hv.begin(), hv.end(), queue);
+        bc::sort_by_key(k.begin() + 1, k.end() - 1, v.begin() + 1, _1 > _2, queue);
+        CHECK(host(k, queue) == (std::vector<int>{9, 8, 7, 5, 5, 1, 2}));
+        CHECK(host(v, queue) == (std::vector<int>{0, 5, 2, 1, 3, 4, 6}));
+    }
     {  // a large comparator sort against std::stable_sort: 3 M int2_ by y, descending
         using bc::lambda::_1;
         using bc::lambda::_2;
