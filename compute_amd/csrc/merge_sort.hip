@@ -422,6 +422,15 @@ struct MsIpt<ItemA64> {
     static constexpr int value = 8;
 };
 
+// Items per thread of the merge passes (their tiles need not match the
+// block sort's, as long as they divide it).  Half-size merge tiles made the
+// 2^24 int2_ merges 10 % faster but doubled the split searches, a net loss,
+// so they match.
+template <typename Item>
+struct MsMergeIpt {
+    static constexpr int value = MsIpt<Item>::value;
+};
+
 static uint32_t copy_word(const void *p, size_t bytes)
 {
     uint32_t w = 16;
@@ -435,12 +444,12 @@ template <typename T, typename Item>
 static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, size_t rb, unsigned char *vals,
                           size_t vb, size_t n)
 {
-    constexpr int IPT = MsIpt<Item>::value;
-    constexpr size_t TILE = (size_t)kMsThreads * IPT;
-    const size_t blocks = (n + TILE - 1) / TILE;
+    constexpr int IPT = MsIpt<Item>::value, MIPT = MsMergeIpt<Item>::value;
+    constexpr size_t TILE = (size_t)kMsThreads * IPT, MTILE = (size_t)kMsThreads * MIPT;
+    const size_t blocks = (n + TILE - 1) / TILE, mblocks = (n + MTILE - 1) / MTILE;
     const size_t rbytes = align256(n * rb), vbytes = vals ? align256(n * vb) : 0;
     const size_t ibytes = n > TILE ? align256(n * sizeof(Item)) : 0;
-    const size_t sbytes = n > TILE ? align256(blocks * sizeof(size_t)) : 0;
+    const size_t sbytes = n > TILE ? align256(mblocks * sizeof(size_t)) : 0;
     void *sp = nullptr;
     BCMI_TRY(scratch(q, rbytes + vbytes + 2 * ibytes + sbytes, &sp));
     unsigned char *rcopy = (unsigned char *)sp;
@@ -460,12 +469,12 @@ static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, siz
     int src = 0;
     for (size_t w = TILE; w < n; w *= 2) {
         const bool last = 2 * w >= n;
-        const size_t split_blocks = (blocks + kMsThreads / 64 - 1) / (kMsThreads / 64);
+        const size_t split_blocks = (mblocks + kMsThreads / 64 - 1) / (kMsThreads / 64);
         BCMI_LAUNCH(q, "merge_sort_split",
-                    hipLaunchKernelGGL((k_ms_split<Item, IPT>), dim3((unsigned)split_blocks), dim3(kMsThreads), 0,
-                                       q->stream, buf[src], n, w, blocks, splits));
+                    hipLaunchKernelGGL((k_ms_split<Item, MIPT>), dim3((unsigned)split_blocks), dim3(kMsThreads), 0,
+                                       q->stream, buf[src], n, w, mblocks, splits));
         BCMI_LAUNCH(q, "merge_sort_merge",
-                    hipLaunchKernelGGL((k_ms_merge<Item, IPT>), dim3((unsigned)blocks), dim3(kMsThreads), 0,
+                    hipLaunchKernelGGL((k_ms_merge<Item, MIPT>), dim3((unsigned)mblocks), dim3(kMsThreads), 0,
                                        q->stream, buf[src], buf[src ^ 1], n, w, splits, last ? emit : none));
         src ^= 1;
     }
