@@ -407,6 +407,25 @@ __global__ __launch_bounds__(kMsThreads) void k_ms_merge(const Item *__restrict_
     }
 }
 
+// The key route: each record's order code and input index, then (after the
+// stable radix sort of the codes) the gather of records and values.
+template <typename T, typename C>
+__global__ __launch_bounds__(kMsThreads) void k_ms_codes(const unsigned char *__restrict__ rec, uint32_t rb, size_t n,
+                                                        CmpDev c, C *__restrict__ codes, uint32_t *__restrict__ idx)
+{
+    const size_t stride = (size_t)gridDim.x * kMsThreads;
+    for (size_t i = (size_t)blockIdx.x * kMsThreads + threadIdx.x; i < n; i += stride) {
+        codes[i] = key_code<T, C>(rec + i * rb, c.lo, c);
+        idx[i] = (uint32_t)i;
+    }
+}
+
+__global__ __launch_bounds__(kMsThreads) void k_ms_gather(const uint32_t *__restrict__ idx, size_t n, Emit e)
+{
+    const size_t stride = (size_t)gridDim.x * kMsThreads;
+    for (size_t i = (size_t)blockIdx.x * kMsThreads + threadIdx.x; i < n; i += stride) emit_one(e, i, idx[i]);
+}
+
 template <typename Item>
 struct MsIpt;
 template <>
@@ -481,15 +500,56 @@ static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, siz
     return 0;
 }
 
+// Key route for comparators that read the same field of both operands:
+// key(a) < key(b) is then a strict weak order by one code per element, so
+// the stable radix sort of (code, index) pairs gives the stable comparator
+// order in 4 (32-bit codes) or 8 passes instead of log2(n / 4096) merge
+// passes, followed by one gather.  The radix sort takes the front of the
+// queue's scratch arena; this route's copies and codes sit after it.
+template <typename T, typename C>
+static int run_key_route(bcmi_queue q, const CmpDev &c, unsigned char *rec, size_t rb, unsigned char *vals,
+                         size_t vb, size_t n)
+{
+    const size_t front = align256(radix_scratch_bytes(n, sizeof(C), 4, q->cu_count));
+    const size_t rbytes = align256(n * rb), vbytes = vals ? align256(n * vb) : 0;
+    const size_t cbytes = align256(n * sizeof(C)), ibytes = align256(n * 4);
+    void *sp = nullptr;
+    BCMI_TRY(scratch(q, front + rbytes + vbytes + cbytes + ibytes, &sp));
+    unsigned char *rcopy = (unsigned char *)sp + front;
+    unsigned char *vcopy = rcopy + rbytes;
+    C *codes = (C *)(rcopy + rbytes + vbytes);
+    uint32_t *idx = (uint32_t *)(rcopy + rbytes + vbytes + cbytes);
+
+    BCMI_HIP(hipMemcpyAsync(rcopy, rec, n * rb, hipMemcpyDeviceToDevice, q->stream));
+    if (vals) BCMI_HIP(hipMemcpyAsync(vcopy, vals, n * vb, hipMemcpyDeviceToDevice, q->stream));
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kMsThreads - 1) / kMsThreads, (size_t)cu_blocks(q, 8));
+    BCMI_LAUNCH(q, "merge_sort_codes",
+                hipLaunchKernelGGL((k_ms_codes<T, C>), dim3(blocks), dim3(kMsThreads), 0, q->stream, rcopy,
+                                   (uint32_t)rb, n, c, codes, idx));
+    BCMI_TRY(bcmi_sort(q, BCMI_STABLE_SORT_BY_KEY, sizeof(C) == 4 ? BCMI_U32 : BCMI_U64, codes, idx, 4, n, 0));
+    Emit emit{rcopy, rec, (uint32_t)rb, std::min(copy_word(rec, rb), copy_word(rcopy, rb)),
+              vals ? vcopy : nullptr, vals, (uint32_t)vb, vals ? std::min(copy_word(vals, vb), copy_word(vcopy, vb)) : 1u};
+    BCMI_LAUNCH(q, "merge_sort_gather",
+                hipLaunchKernelGGL(k_ms_gather, dim3(blocks), dim3(kMsThreads), 0, q->stream, idx, n, emit));
+    return 0;
+}
+
 template <typename T>
 static int merge_sort_typed(bcmi_queue q, const CmpDev &c, bool asym, unsigned char *rec, size_t rb,
                             unsigned char *vals, size_t vb, size_t n)
 {
     if (asym) return run_merge_sort<T, ItemA64>(q, c, rec, rb, vals, vb, n);
+    // route 1 (default): key-based comparators sort their codes by radix;
+    // route 0: always the merge kernels
+    const bool key_route = tune_get("__bcmi_merge_sort", "route", 1) != 0;
     if constexpr (sizeof(T) <= 4) {
-        if (c.form != BCMI_KEY_LENGTH) return run_merge_sort<T, ItemS32>(q, c, rec, rb, vals, vb, n);
+        if (c.form != BCMI_KEY_LENGTH) {
+            if (key_route) return run_key_route<T, uint32_t>(q, c, rec, rb, vals, vb, n);
+            return run_merge_sort<T, ItemS32>(q, c, rec, rb, vals, vb, n);
+        }
     }
     if constexpr (sizeof(T) == 8 || std::is_same<T, float>::value) {
+        if (key_route) return run_key_route<T, uint64_t>(q, c, rec, rb, vals, vb, n);
         return run_merge_sort<T, ItemS64>(q, c, rec, rb, vals, vb, n);
     } else {
         set_error("merge_sort: length() needs float or double fields");

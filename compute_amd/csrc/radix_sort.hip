@@ -140,6 +140,14 @@ static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 // LASTB: the last (most significant) pass ranks by ballots on 24-key tiles.
 // Float keys of a narrow range share few top-digit values, and many lanes
 // with one digit serialise the LDS-mask ranking's same-address atomics.
+// Bytes of queue scratch radix_sort_xf takes for n keys of key_bytes with
+// value_bytes values (at most cu_count * 4 histogram blocks).  Callers that
+// keep their own temporaries in the arena beside a sort place them after it.
+size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int cu_count)
+{
+    return align256(n * key_bytes) + align256(n * value_bytes) + (size_t)cu_count * 4 * key_bytes * 256 * 4;
+}
+
 template <typename KB, int VB, int IPT, int RANK, typename XF, bool LASTB = false>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf)
 {

@@ -116,6 +116,10 @@ void prof_begin(bcmi_queue q, const char *name, ProfRec *rec);
 int prof_end(bcmi_queue q, ProfRec *rec);
 int check_launch(const char *name);
 
+// Queue scratch a radix sort of n keys takes (radix_sort.hip); temporaries
+// kept beside a sort go after it.
+size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int cu_count);
+
 // Persistent grid sizing: blocks of `threads` that fill the device.
 inline int cu_blocks(bcmi_queue q, int per_cu) { return q->cu_count * per_cu; }
 

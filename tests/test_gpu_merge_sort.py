@@ -11,6 +11,16 @@ from test_oracle import CMP_DTYPES, _cmp_records, check_compare_case, compare_ca
 
 pytestmark = pytest.mark.gpu
 
+
+@pytest.fixture(autouse=True, params=[1, 0], ids=["key_route", "merge_kernels"])
+def route(request):
+    """Every test runs on both routes of a comparator that reads the same field
+    of both operands: the radix sort of its key codes (the default) and the
+    merge kernels.  Asymmetric comparators always take the merge kernels."""
+    ca.check(ca.lib().bcmi_tune_set(b"__bcmi_merge_sort", b"route", request.param))
+    yield request.param
+    ca.check(ca.lib().bcmi_tune_set(b"__bcmi_merge_sort", b"route", 1))
+
 # tile sizes of the kernels: 4096 items for 8-byte items (fields of <= 4
 # bytes), 2048 for 16- and 24-byte items
 EDGE_SIZES = [2, 3, 31, 32, 33, 2047, 2048, 2049, 4095, 4096, 4097, 3 * 4096 + 5, 50001]
