@@ -507,9 +507,31 @@ static int run_merge_sort(bcmi_queue q, const CmpDev &c, unsigned char *rec, siz
 // passes, followed by one gather.  The radix sort takes the front of the
 // queue's scratch arena; this route's copies and codes sit after it.
 template <typename T, typename C>
+__global__ __launch_bounds__(kMsThreads) void k_ms_codes_only(const unsigned char *__restrict__ rec, uint32_t rb,
+                                                             size_t n, CmpDev c, C *__restrict__ codes)
+{
+    const size_t stride = (size_t)gridDim.x * kMsThreads;
+    for (size_t i = (size_t)blockIdx.x * kMsThreads + threadIdx.x; i < n; i += stride)
+        codes[i] = key_code<T, C>(rec + i * rb, c.lo, c);
+}
+
+template <typename T, typename C>
 static int run_key_route(bcmi_queue q, const CmpDev &c, unsigned char *rec, size_t rb, unsigned char *vals,
                          size_t vb, size_t n)
 {
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kMsThreads - 1) / kMsThreads, (size_t)cu_blocks(q, 8));
+    if (!vals && rb <= 16) {
+        // records of up to 16 bytes and no values: the records themselves are
+        // the radix sort's values, sorted in place; no copies, no gather
+        const size_t front = align256(radix_scratch_bytes(n, sizeof(C), rb, q->cu_count));
+        void *sp = nullptr;
+        BCMI_TRY(scratch(q, front + align256(n * sizeof(C)), &sp));
+        C *codes = (C *)((unsigned char *)sp + front);
+        BCMI_LAUNCH(q, "merge_sort_codes",
+                    hipLaunchKernelGGL((k_ms_codes_only<T, C>), dim3(blocks), dim3(kMsThreads), 0, q->stream, rec,
+                                       (uint32_t)rb, n, c, codes));
+        return bcmi_sort(q, BCMI_STABLE_SORT_BY_KEY, sizeof(C) == 4 ? BCMI_U32 : BCMI_U64, codes, rec, rb, n, 0);
+    }
     const size_t front = align256(radix_scratch_bytes(n, sizeof(C), 4, q->cu_count));
     const size_t rbytes = align256(n * rb), vbytes = vals ? align256(n * vb) : 0;
     const size_t cbytes = align256(n * sizeof(C)), ibytes = align256(n * 4);
@@ -522,7 +544,6 @@ static int run_key_route(bcmi_queue q, const CmpDev &c, unsigned char *rec, size
 
     BCMI_HIP(hipMemcpyAsync(rcopy, rec, n * rb, hipMemcpyDeviceToDevice, q->stream));
     if (vals) BCMI_HIP(hipMemcpyAsync(vcopy, vals, n * vb, hipMemcpyDeviceToDevice, q->stream));
-    const unsigned blocks = (unsigned)std::min<size_t>((n + kMsThreads - 1) / kMsThreads, (size_t)cu_blocks(q, 8));
     BCMI_LAUNCH(q, "merge_sort_codes",
                 hipLaunchKernelGGL((k_ms_codes<T, C>), dim3(blocks), dim3(kMsThreads), 0, q->stream, rcopy,
                                    (uint32_t)rb, n, c, codes, idx));
