@@ -371,10 +371,20 @@ def main():
         t_merge = float(np.mean(ms))
         kern = kernel_breakdown(q, fn, ["merge_sort_block", "merge_sort_split", "merge_sort_merge"], restore=restore)
         ca.check(ca.lib().bcmi_tune_set(b"__bcmi_merge_sort", b"route", 1))
+        # the sort_by_key form: uint32 values ride along (codes + (code, index) radix sort + one gather)
+        vsrc, vwork = ca.DeviceArray.from_host(np.arange(n, dtype=np.uint32), q), ca.DeviceArray(n, np.uint32, q)
+        fkv = lambda: ca.merge_sort(q, cmp, work, 8, n, vwork, 4)
+        restore_kv = lambda: (d2d(ca, q, work, src), d2d(ca, q, vwork, vsrc))
+        ms, _ = timed_steps(ca, q, fkv, K, Wm, restore_kv)
+        t_kv = float(np.mean(ms))
+        kern_kv = kernel_breakdown(q, fkv, ["merge_sort_codes", "radix_histogram", "radix_onesweep",
+                                            "merge_sort_gather"], restore=restore_kv)
+        del vsrc, vwork
         passes = int(np.ceil(np.log2(n / 4096)))
         merge_ms = kern.get("merge_sort_merge", float("nan")) / passes
         sec["merge_sort_int2_by_x_2p24"] = {
             "Mrecords_per_s": round(n / (t * 1e-3) / 1e6, 1), "ms": round(t, 4), "kernels_ms": kern_key,
+            "by_key_u32_values_ms": round(t_kv, 4), "by_key_kernels_ms": kern_kv,
             "merge_kernels_ms": round(t_merge, 4), "merge_kernels_Mrecords_per_s": round(n / (t_merge * 1e-3) / 1e6, 1),
             "merge_kernels_breakdown_ms": kern, "merge_passes": passes, "merge_pass_ms": round(merge_ms, 4),
             "merge_pass_GBps": round(gbs(16 * n, merge_ms), 1),
