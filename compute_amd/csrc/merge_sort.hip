@@ -520,9 +520,10 @@ static int run_key_route(bcmi_queue q, const CmpDev &c, unsigned char *rec, size
                          size_t vb, size_t n)
 {
     const unsigned blocks = (unsigned)std::min<size_t>((n + kMsThreads - 1) / kMsThreads, (size_t)cu_blocks(q, 8));
-    if (!vals && rb <= 16) {
-        // records of up to 16 bytes and no values: the records themselves are
-        // the radix sort's values, sorted in place; no copies, no gather
+    if (!vals && rb <= 16 && (uintptr_t)rec % rb == 0) {
+        // records of up to 16 bytes, aligned to their size, and no values:
+        // the records themselves are the radix sort's values (moved as
+        // rb-byte words), sorted in place; no copies, no gather
         const size_t front = align256(radix_scratch_bytes(n, sizeof(C), rb, q->cu_count));
         void *sp = nullptr;
         BCMI_TRY(scratch(q, front + align256(n * sizeof(C)), &sp));
@@ -561,8 +562,9 @@ static int merge_sort_typed(bcmi_queue q, const CmpDev &c, bool asym, unsigned c
 {
     if (asym) return run_merge_sort<T, ItemA64>(q, c, rec, rb, vals, vb, n);
     // route 1 (default): key-based comparators sort their codes by radix;
-    // route 0: always the merge kernels
-    const bool key_route = tune_get("__bcmi_merge_sort", "route", 1) != 0;
+    // route 0: always the merge kernels.  The radix sort takes n up to
+    // 2^32 - 2^20; above that the merge kernels serve every route.
+    const bool key_route = tune_get("__bcmi_merge_sort", "route", 1) != 0 && n <= (1ull << 32) - (1ull << 20);
     if constexpr (sizeof(T) <= 4) {
         if (c.form != BCMI_KEY_LENGTH) {
             if (key_route) return run_key_route<T, uint32_t>(q, c, rec, rb, vals, vb, n);

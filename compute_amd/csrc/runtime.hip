@@ -8,6 +8,7 @@
 #include <cstring>
 #include <dlfcn.h>
 #include <fstream>
+#include <atomic>
 #include <mutex>
 #include <sstream>
 
@@ -68,6 +69,7 @@ int scratch(bcmi_queue q, size_t bytes, void **ptr)
         size_t grow = round_up(bytes + bytes / 8, 1 << 20);
         BCMI_HIP(hipMalloc(&q->scratch, grow));
         q->scratch_bytes = grow;
+        q->arena_gen++;
     }
     *ptr = q->scratch;
     return 0;
@@ -105,6 +107,7 @@ int status_array(bcmi_queue q, size_t words, uint64_t **ptr)
         size_t grow = round_up(words + words / 8, 4096);
         BCMI_HIP(hipMalloc(&q->status, grow * sizeof(uint64_t)));
         q->status_words = grow;
+        q->arena_gen++;
         BCMI_TRY(zero_status(q));
     }
     *ptr = q->status;
@@ -251,7 +254,9 @@ int bcmi_queue_create(int device, bcmi_queue *queue)
     BCMI_CHECK(std::strncmp(p.gcnArchName, "gfx950", 6) == 0, BCMI_DEVICE_NOT_FOUND,
                "device %d is %s; this library is built for gfx950 only", device, p.gcnArchName);
     BCMI_TRY(tune_autoload());
+    static std::atomic<uint64_t> next_id{1};
     bcmi_queue q = new bcmi_queue_s();
+    q->id = next_id.fetch_add(1);
     q->device = device;
     q->cu_count = p.multiProcessorCount;
     q->lds_per_block = p.sharedMemPerBlock;
@@ -451,6 +456,8 @@ int bcmi_graph_end(bcmi_queue q, bcmi_graph *graph)
         return s;
     }
     g->lookback_launches = q->capture_lookback_launches;
+    g->queue_id = q->id;
+    g->arena_gen = q->arena_gen;
     *graph = g;
     return 0;
 }
@@ -459,6 +466,13 @@ int bcmi_graph_launch(bcmi_graph g, bcmi_queue q)
 {
     BCMI_TRY(use_queue(q));
     BCMI_CHECK(g && g->exec, BCMI_INVALID_VALUE, "null graph");
+    // the graph's kernels hold the capture queue's control block, scratch
+    // arena and status array by address
+    BCMI_CHECK(g->queue_id == q->id, BCMI_INVALID_COMMAND_QUEUE,
+               "graph launched on a queue other than the one it was captured on");
+    BCMI_CHECK(g->arena_gen == q->arena_gen, BCMI_INVALID_OPERATION,
+               "the queue's scratch arena was reallocated after this graph was captured (a later call "
+               "needed more); reserve enough with bcmi_queue_reserve before capturing, then capture again");
     BCMI_HIP(hipGraphLaunch(g->exec, q->stream));
     q->lookback_launches += g->lookback_launches;
     if (q->lookback_launches > (1ull << 29)) BCMI_TRY(zero_status(q));

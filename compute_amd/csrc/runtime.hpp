@@ -6,7 +6,11 @@
 //   * a grow-only scratch arena for temporaries.  The reference allocates a
 //     vector per call and relies on OpenCL's deferred release
 //     (radix_sort.hpp:333-336); HIP frees synchronously, so the arena is
-//     reused across calls and only grows (after a stream sync);
+//     reused across calls and only grows: a call that needs more syncs the
+//     stream, frees the old arena and allocates a larger one.  Each such
+//     reallocation (of the arena or the status array) bumps arena_gen; a
+//     graph records the generation it was captured under and refuses to
+//     replay once it changed, since its kernels hold the freed addresses;
 //   * a look-back status array (decoupled look-back scans and onesweep
 //     digit offsets).  Status words are tagged with an epoch that the last
 //     block of every look-back kernel advances on the device, so the array
@@ -45,6 +49,7 @@ struct ProfRec {
 }  // namespace bcmi
 
 struct bcmi_queue_s {
+    uint64_t id = 0;  // unique per queue created in this process
     int device = 0;
     hipStream_t stream = nullptr;
     int cu_count = 0;
@@ -56,6 +61,8 @@ struct bcmi_queue_s {
     uint64_t *status = nullptr;
     size_t status_words = 0;
     uint64_t lookback_launches = 0;
+    // bumped whenever scratch or status is reallocated
+    uint64_t arena_gen = 0;
     // control block
     bcmi::Ctrl *ctrl = nullptr;
     // pinned host staging for small blocking readbacks
@@ -73,6 +80,8 @@ struct bcmi_graph_s {
     hipGraph_t graph = nullptr;
     hipGraphExec_t exec = nullptr;
     uint64_t lookback_launches = 0;
+    uint64_t queue_id = 0;   // the queue it was captured on
+    uint64_t arena_gen = 0;  // that queue's arena generation at capture
 };
 
 namespace bcmi {

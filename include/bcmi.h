@@ -158,8 +158,11 @@ int bcmi_sort(bcmi_queue queue, int api, int key_dtype, void *keys,
 /* exclusive_scan (exclusive_scan.hpp:55-104), inclusive_scan
  * (inclusive_scan.hpp:53-87), partial_sum (partial_sum.hpp:31-41).
  * in == out is allowed.  init: host pointer to one element (exclusive only;
- * NULL means the op's identity for inclusive).  Integers wrap; float plus is
- * compensated (precision) with an FP64 carry between tiles. */
+ * NULL means the op's identity for inclusive).  op is BCMI_PLUS,
+ * BCMI_MULTIPLIES, BCMI_MIN or BCMI_MAX (MINUS and DIVIDES are not
+ * associative and give BCMI_INVALID_VALUE; the templates refuse them at
+ * compile time).  Integers wrap; float plus is compensated (precision)
+ * with an FP64 carry between tiles. */
 int bcmi_scan(bcmi_queue queue, int dtype, int op, const void *in, void *out,
               size_t n, const void *init, int exclusive, int precision);
 

@@ -19,6 +19,12 @@ inline buffer_iterator<OutT> scan(buffer_iterator<InT> first, buffer_iterator<In
                                   command_queue &queue)
 {
     static_assert(std::is_same<InT, OutT>::value, "scan: input and output element types must match");
+    // minus and divides are not associative: a parallel scan has no
+    // well-defined result for them (the reference's GPU scan folds in a tree
+    // order, scan_on_gpu.hpp:84-93), so they are refused at compile time
+    static_assert(BinaryOperator::bcmi_op == BCMI_PLUS || BinaryOperator::bcmi_op == BCMI_MULTIPLIES ||
+                      BinaryOperator::bcmi_op == BCMI_MIN || BinaryOperator::bcmi_op == BCMI_MAX,
+                  "scan: the operator must be plus, multiplies, min or max");
     const size_t n = (size_t)(last - first);
     const OutT ini = (OutT)init;
     bcmi_check(bcmi_scan(queue.get(), bcmi_dtype<OutT>(), BinaryOperator::bcmi_op, first.device_ptr(),

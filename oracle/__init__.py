@@ -120,6 +120,8 @@ def lib():
         L.orc_reduce_gpu_order_f32.restype = C.c_float
         L.orc_reduce_cpu_order_f32.argtypes = [vp, sz, C.c_uint]
         L.orc_reduce_cpu_order_f32.restype = C.c_float
+        L.orc_scan_cpu_order.argtypes = [i, vp, vp, sz, i, vp, C.c_uint]
+        L.orc_scan_cpu_order.restype = i
         L.orc_accumulate_f32.argtypes = [vp, sz, C.c_float]
         L.orc_accumulate_f32.restype = C.c_float
         L.orc_scan_gpu_order_f32.argtypes = [vp, vp, sz, i]
@@ -320,6 +322,20 @@ def reduce_gpu_order_f32(x):
 
 def reduce_cpu_order_f32(x, cu):
     return np.float32(lib().orc_reduce_cpu_order_f32(_p(np.ascontiguousarray(x, np.float32)), x.size, cu))
+
+
+def scan_cpu_order(x, exclusive, cu, init=0, fill=0):
+    """scan_on_cpu's association (plus, float32 or int32) with cu compute
+    units; elements the reference leaves unwritten keep `fill`.  Raises on
+    inputs whose result the reference leaves undefined."""
+    x = np.ascontiguousarray(x)
+    assert x.dtype in (np.float32, np.int32)
+    out = np.full_like(x, fill)
+    ini = np.array([init], dtype=x.dtype)
+    r = lib().orc_scan_cpu_order(NP2CODE[x.dtype], _p(x), _p(out), x.size, int(exclusive), _p(ini), cu)
+    if r != 0:
+        raise ValueError("scan_on_cpu result undefined for n=%d, cu=%d" % (x.size, cu))
+    return out
 
 
 def accumulate_f32(x, init=0.0):

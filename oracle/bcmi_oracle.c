@@ -744,6 +744,110 @@ float orc_reduce_cpu_order_f32(const float *x, size_t n, unsigned cu)
     return r;
 }
 
+/* scan_on_cpu (scan_on_cpu.hpp:28-199), the CPU-device column of the
+ * scans, for plus on float (ORC_F32) or int32 (ORC_I32, wrapping).  G = cu
+ * work-items; below max(16384 * sizeof(T), cu) elements it is serial_scan
+ * (serial_scan.hpp:50-80): one left fold, init first when exclusive.
+ * Otherwise block = (n + G) / (G + 1) (:76) and chunk c = [c*block,
+ * min(n, (c+1)*block)), c = 0..G:
+ *   kernel 1 (:75-124), work-item g < G, chunk g: a left fold of the chunk
+ *     from its first element (exclusive, g == 0: from init instead, :86-94);
+ *     only g == 0 writes its chunk's results (:104-118); every g writes its
+ *     fold to partial[g] (:124) and, inclusive, result[g*block] = x[g*block]
+ *     (:82-84), overwritten by kernel 2 for g >= 1;
+ *   kernel 2 (:147-183), work-item g, chunk g+1: seed = partial[0] op
+ *     partial[1] ... op partial[g] (a left fold, :153-157), then a left fold
+ *     of the chunk from the seed (exclusive writes the seed first, :161-170).
+ *   With G < 2 kernel 2 never runs (:135-137): the chunk after the first is
+ *   left unwritten, as in the reference.
+ * A work-item of kernel 1 whose chunk is empty returns before writing its
+ * partial (:79), which a later seed would read uninitialised; that needs
+ * n <= G(G-1)/2 and returns -1 here (results undefined in the reference).
+ * Returns 0 otherwise. */
+#define ORC_SCAN_CPU(NAME, T, ADD)                                                          \
+static int NAME(const T *x, T *out, size_t n, int exclusive, T init, unsigned G)            \
+{                                                                                           \
+    size_t thr = 16384 * sizeof(T);                                                         \
+    if (thr < G) thr = G;                                                                   \
+    if (n == 0) return 0;                                                                   \
+    if (n < thr) {                                                                          \
+        T sum = exclusive ? init : x[0];                                                    \
+        size_t i = 1;                                                                       \
+        if (exclusive) i = 0; else out[0] = sum;                                            \
+        for (; i < n; ++i) {                                                                \
+            const T v = x[i];                                                               \
+            if (exclusive) out[i] = sum;                                                    \
+            sum = ADD(sum, v);                                                              \
+            if (!exclusive) out[i] = sum;                                                   \
+        }                                                                                   \
+        return 0;                                                                           \
+    }                                                                                       \
+    const uint32_t cnt = (uint32_t)n;                                                       \
+    const uint32_t block = (cnt + G) / (G + 1);                                             \
+    T *part = (T *)malloc(G * sizeof(T));                                                   \
+    int undefined = 0;                                                                      \
+    for (uint32_t g = 0; g < G; ++g) {                                                      \
+        uint32_t index = g * block;                                                         \
+        const uint32_t end = cnt < index + block ? cnt : index + block;                     \
+        if (index >= end) { undefined = 1; continue; }                                      \
+        T sum;                                                                              \
+        if (!exclusive) {                                                                   \
+            sum = x[index];                                                                 \
+            out[index] = sum;                                                               \
+            index++;                                                                        \
+        } else if (index == 0) {                                                            \
+            sum = init;                                                                     \
+        } else {                                                                            \
+            sum = x[index];                                                                 \
+            index++;                                                                        \
+        }                                                                                   \
+        for (; index < end; ++index) {                                                      \
+            const T v = x[index];                                                           \
+            if (exclusive && g == 0) out[index] = sum;                                      \
+            sum = ADD(sum, v);                                                              \
+            if (!exclusive && g == 0) out[index] = sum;                                     \
+        }                                                                                   \
+        part[g] = sum;                                                                      \
+    }                                                                                       \
+    if (G >= 2) {                                                                           \
+        for (uint32_t g = 0; g < G; ++g) {                                                  \
+            uint32_t index = block + g * block;                                             \
+            const uint32_t end = cnt < index + block ? cnt : index + block;                 \
+            T sum = part[0];                                                                \
+            for (uint32_t i = 0; i < g; ++i) sum = ADD(sum, part[i + 1]);                   \
+            for (; index < end; ++index) {                                                  \
+                if (exclusive) {                                                            \
+                    const T v = x[index];                                                   \
+                    out[index] = sum;                                                       \
+                    sum = ADD(sum, v);                                                      \
+                } else {                                                                    \
+                    sum = ADD(sum, x[index]);                                               \
+                    out[index] = sum;                                                       \
+                }                                                                           \
+            }                                                                               \
+        }                                                                                   \
+    }                                                                                       \
+    free(part);                                                                             \
+    return undefined ? -1 : 0;                                                              \
+}
+
+static float orc_addf(float a, float b) { return a + b; }
+static int32_t orc_addi(int32_t a, int32_t b) { return (int32_t)((uint32_t)a + (uint32_t)b); }
+ORC_SCAN_CPU(scan_cpu_f32, float, orc_addf)
+ORC_SCAN_CPU(scan_cpu_i32, int32_t, orc_addi)
+#undef ORC_SCAN_CPU
+
+int orc_scan_cpu_order(int dtype, const void *in, void *out, size_t n, int exclusive, const void *init,
+                       unsigned cu)
+{
+    if (dtype == ORC_F32)
+        return scan_cpu_f32((const float *)in, (float *)out, n, exclusive, init ? *(const float *)init : 0.0f, cu);
+    if (dtype == ORC_I32)
+        return scan_cpu_i32((const int32_t *)in, (int32_t *)out, n, exclusive, init ? *(const int32_t *)init : 0,
+                            cu);
+    return -2;
+}
+
 /* serial_accumulate (serial_accumulate.hpp:38-40): result = init; result += xi */
 float orc_accumulate_f32(const float *x, size_t n, float init)
 {

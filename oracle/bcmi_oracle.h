@@ -90,6 +90,12 @@ double orc_sum_f64_dd(const double *x, size_t n, double *lo);
 float  orc_reduce_gpu_order_f32(const float *x, size_t n);
 float  orc_reduce_cpu_order_f32(const float *x, size_t n, unsigned cu);
 float  orc_accumulate_f32(const float *x, size_t n, float init);
+/* scan_on_cpu (CPU-device column) association with cu compute units, plus
+ * on ORC_F32 or ORC_I32; out must be pre-filled by the caller where the
+ * reference leaves it unwritten (cu < 2).  0 ok, -1 undefined (empty chunk),
+ * -2 bad dtype. */
+int    orc_scan_cpu_order(int dtype, const void *in, void *out, size_t n,
+                          int exclusive, const void *init, unsigned cu);
 float  orc_scan_gpu_order_f32_last(const float *x, size_t n);
 void   orc_scan_gpu_order_f32(const float *x, float *out, size_t n, int exclusive);
 

@@ -205,3 +205,22 @@ def test_sub_range_at_unaligned_offsets(queue):
     assert got[3:n - 2].tobytes() == want.tobytes() and gv[1:m + 1].tobytes() == wv.tobytes()
     assert np.array_equal(got[:3], rec[:3]) and np.array_equal(got[n - 2:], rec[n - 2:])
     assert gv[0] == vals[0] and np.array_equal(gv[m + 1:], vals[m + 1:])
+
+
+@pytest.mark.parametrize("cols", [2, 4])
+def test_records_at_misaligned_offset_no_values(queue, cols):
+    # int2 / int4 records starting 4 bytes into the buffer (aligned to the
+    # int32 field, not to the 8- or 16-byte record) with no values: the key
+    # route must not move them as 8- or 16-byte words
+    rng = np.random.default_rng(40 + cols)
+    n = 2 * 4096 + 77
+    flat = rng.integers(-500, 500, n * cols + 1).astype(np.int32)
+    d = dev(queue, flat)
+    rb = 4 * cols
+    cmp = ca.comparator(np.int32, "field", 4)
+    ca.check(ca.lib().bcmi_merge_sort(queue.h, ca.C.byref(cmp), d.at(1), rb, None, 0, n))
+    got = d.to_host(queue)
+    rec = flat[1:].reshape(n, cols).copy()
+    want, _ = O.comparator_sort(rec, rb, np.int32, form="field", lhs=4)
+    assert got[1:].tobytes() == want.tobytes()
+    assert got[0] == flat[0]

@@ -508,6 +508,44 @@ def test_graph_replay_matches_eager(queue):
         queue.finish()
 
 
+def test_graph_replay_after_arena_growth_is_refused():
+    """A graph holds its queue's scratch arena and status array by address.
+    When a later eager call grows them (the old buffers are freed), a replay
+    is refused with an error instead of touching freed memory; a graph
+    captured again afterwards replays correctly.  A graph only replays on
+    the queue it was captured on."""
+    q = ca.Queue(0)
+    other = ca.Queue(0)
+    try:
+        n_small, n_big = 5000, 3 << 20
+        k = random_keys("uint32", n_small)
+        dk = dev(q, k)
+        ca.sort(q, dk)  # sizes the arena for n_small
+        q.finish()
+        g = ca.Graph(q)
+        with g:
+            ca.sort(q, dk)
+        with pytest.raises(ca.BcmiError):
+            g.launch(other)
+        big = dev(q, random_keys("uint32", n_big))
+        ca.sort(q, big)  # grows the arena and the status array
+        q.finish()
+        with pytest.raises(ca.BcmiError, match="reallocated"):
+            g.launch()
+        g2 = ca.Graph(q)
+        with g2:
+            ca.sort(q, dk)
+        for rep in range(2):
+            ca.check(ca.lib().bcmi_memcpy(q.h, dk.ptr, k.ctypes.data_as(ca.C.c_void_p), k.nbytes, ca.H2D, 1))
+            g2.launch()
+            q.finish()
+            assert np.array_equal(dk.to_host(q), np.sort(k)), rep
+        del g, g2
+    finally:
+        other.close()
+        q.close()
+
+
 # ================================================ tunable radix variants
 def _tune_get(key, param):
     v = ca.C.c_uint()

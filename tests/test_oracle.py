@@ -370,6 +370,61 @@ def test_reference_association_orders_closed_forms():
     assert inc[3] == big + 2 and inc[256] == big + 4
 
 
+@pytest.mark.parametrize("cu", [2, 3, 8, 64, 255])
+@pytest.mark.parametrize("n", [65535, 65536, 65537, 100003, 196608])
+def test_scan_cpu_order_integer_exact(cu, n):
+    # int32 plus is exact (wrapping), so every chunk boundary, seed and
+    # partial of scan_on_cpu (scan_on_cpu.hpp:75-183) must reproduce the
+    # plain prefix sum; an off-by-one in a chunk or seed changes it
+    rng = np.random.default_rng(n + cu)
+    x = rng.integers(-(1 << 30), 1 << 30, n).astype(np.int32)
+    inc = np.cumsum(x.astype(np.int64)).astype(np.int64)
+    want_inc = (inc & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+    assert np.array_equal(O.scan_cpu_order(x, False, cu), want_inc)
+    init = -12345
+    exc = np.concatenate([[0], inc[:-1]]) + init
+    want_exc = (exc & 0xFFFFFFFF).astype(np.uint32).view(np.int32)
+    assert np.array_equal(O.scan_cpu_order(x, True, cu, init=init), want_exc)
+
+
+def test_scan_cpu_order_float_closed_forms():
+    big = np.float32(2 ** 24)
+    # below max(16384 * sizeof(float), cu) elements: serial_scan, one left
+    # fold (numpy's add.accumulate is a sequential float32 left fold)
+    rng = np.random.default_rng(5)
+    x = rng.normal(size=60000).astype(np.float32)
+    assert np.array_equal(O.scan_cpu_order(x, False, 8), np.add.accumulate(x, dtype=np.float32))
+    # cu = 2, n = 3 * 65536: block = (n + 2) / 3 = 65536, chunks c0 c1 c2.
+    # c0 = [3, 0, ...], c1 = [2^24, 1.5 x 65535], c2 = zeros.  Each 1.5 added
+    # at [2^24, 2^25) (ulp 2) rounds up by 2, so c1's own fold is
+    # p1 = 2^24 + 131070 = 16908286 and c2's seed is 3 + p1 = 16908289, a tie
+    # rounded to even: 16908288.  The serial fold instead gives 3 + 2^24 ->
+    # 16777220 (tie to even), then + 131070 = 16908290.
+    n = 3 * 65536
+    x = np.zeros(n, np.float32)
+    x[0] = 3
+    x[65536] = big
+    x[65537:131072] = 1.5
+    inc = O.scan_cpu_order(x, False, 2)
+    assert inc[65535] == 3
+    assert inc[65536] == 16777220 and inc[131071] == 16777220 + 2 * 65535
+    assert np.all(inc[131072:] == 16908288)
+    assert np.add.accumulate(x, dtype=np.float32)[131072] == 16908290  # the serial fold differs
+    exc = O.scan_cpu_order(x, True, 2, init=0.0)
+    assert exc[0] == 0 and exc[1] == 3 and exc[65536] == 3 and exc[65537] == 16777220
+    assert exc[131072] == 16908288
+    # with one compute unit kernel 2 never runs (scan_on_cpu.hpp:135-137):
+    # block = (n + 1) / 2 and the second half keeps what was there
+    y = np.ones(131072, np.float32)
+    out = O.scan_cpu_order(y, False, 1, fill=-1.0)
+    assert np.array_equal(out[:65536], np.arange(1, 65537, dtype=np.float32))
+    assert np.all(out[65536:] == -1.0)
+    # a work-item whose chunk is empty leaves its partial unwritten (:79):
+    # cu = 1000, n = 65536 -> block 66, chunks from 993 on are empty
+    with pytest.raises(ValueError):
+        O.scan_cpu_order(np.ones(65536, np.float32), False, 1000)
+
+
 # ------------------------------------------------------------ reduce_by_key
 def test_golden_reduce_by_key(golden):
     n = 0
