@@ -103,6 +103,92 @@ def load_traffic(name):
         return None
 
 
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def cpu_baseline():
+    """The reference's CPU-device column (what Boost.Compute runs on a CPU
+    OpenCL device such as PoCL), restated on OpenMP threads in
+    oracle/cpu_column.c, timed per primitive on bounded samples of the same
+    workloads: one untimed warm-up, then the minimum of 3 trials
+    (perf/perf.hpp conventions).  Neither Boost nor an OpenCL CPU device is
+    on this machine (the probe is kept with the profiles), so the reference
+    itself cannot run here."""
+    import oracle as O
+    import compute_amd.workloads as W
+    threads = int(os.environ.get("OMP_NUM_THREADS", "0") or 0) or (os.cpu_count() or 1)
+
+    def best(fn, trials=3):
+        fn()
+        ts = []
+        for _ in range(trials):
+            t0 = time.perf_counter()
+            fn()
+            ts.append(time.perf_counter() - t0)
+        return min(ts)
+
+    per = {}
+    m = 1 << 24
+    keys = W.sort_u32(m)
+    t_sort = best(lambda: O.cc_sort(keys, threads=threads))
+    per["sort_u32"] = {"Gkeys_per_s": round(m / t_sort / 1e9, 5), "sample": "2^24 keys", "s": round(t_sort, 3)}
+    mk = 1 << 22
+    k, v = W.sort_by_key_f32(mk)
+    t = best(lambda: O.cc_sort(k, v, threads=threads))
+    per["sort_by_key_f32_u32"] = {"Gpairs_per_s": round(mk / t / 1e9, 5), "sample": "2^22 pairs",
+                                  "s": round(t, 3)}
+    ms = 1 << 26
+    x = W.scan_i32(ms)
+    t = best(lambda: O.cc_scan_i32(x, True, 0, threads=threads))
+    per["exclusive_scan_i32"] = {"GBps": round(8 * ms / t / 1e9, 2), "sample": "2^26 elements", "s": round(t, 4)}
+    f = W.lognormal_f32(ms)
+    t = best(lambda: O.cc_reduce_f32(f, threads=threads))
+    per["reduce_f32"] = {"GBps": round(4 * ms / t / 1e9, 2), "sample": "2^26 elements", "s": round(t, 4)}
+    t = best(lambda: O.cc_accumulate_f32(f, 0.0), trials=1)
+    per["accumulate_f32_serial_fold"] = {"GBps": round(4 * ms / t / 1e9, 2), "sample": "2^26 elements",
+                                         "threads": 1, "s": round(t, 4)}
+    a, b = W.saxpy_f32(ms)
+    t = best(lambda: O.cc_saxpy(2.5, a, b, threads=threads))
+    per["saxpy_f32"] = {"GBps": round(12 * ms / t / 1e9, 2), "sample": "2^26 elements", "s": round(t, 4)}
+    mr = 1 << 25
+    rk, rv = W.runs_i32_f32(mr)
+    t = best(lambda: O.cc_reduce_by_key(rk, rv), trials=1)
+    per["reduce_by_key_i32_f32_serial"] = {"GBps": round(8 * mr / t / 1e9, 2), "sample": "2^25 elements",
+                                           "threads": 1, "s": round(t, 4)}
+    off = W.segments(65536)
+    nseg = 65536
+    sub = off
+    sk, sv = W.segment_data(int(sub[-1]))
+    t = best(lambda: O.cc_segmented_sort(sk, sv, sub, threads=threads), trials=1)
+    per["segmented_sort_per_call_loop"] = {
+        "Gpairs_per_s": round(int(sub[-1]) / t / 1e9, 5), "s": round(t, 3),
+        "sample": "all %d segments (%d pairs), one merge_sort_by_key_on_cpu call per segment, "
+                  "each on all threads" % (nseg, int(sub[-1]))}
+    del keys, k, v, x, f, a, b, rk, rv, sk, sv
+    # divergence A5: the reference's float accumulate is one serial FP32
+    # fold; its error on the full 2^28 precision workload, against the
+    # double-double sum
+    full = W.lognormal_f32(1 << 28)
+    fold = float(O.cc_accumulate_f32(full, 0.0))
+    ref = W.sum_dd(full)
+    ulp = float(np.spacing(np.float32(abs(ref))))
+    per["accumulate_f32_serial_fold"]["err_on_2p28_vs_fp64"] = {
+        "ulp": round(abs(fold - ref) / ulp, 3), "abs": float("%.4g" % abs(fold - ref)),
+        "rel": float("%.4g" % (abs(fold - ref) / abs(ref)))}
+    return {"value": round(m / t_sort / 1e9, 5), "unit": "Gkeys/s", "cores": threads, "kind": "port",
+            "sample": "first 2^24 of the same mt19937(0xB0057001) keys, sorted by oracle/cpu_column.c's "
+                      "merge_sort_on_cpu on %d OpenMP threads (min of 3 after a warm-up); %.3f s" % (threads, t_sort),
+            "cpu_model": cpu_model(), "nproc": os.cpu_count(), "per_primitive": per}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -221,18 +307,35 @@ def main():
         got = float(r.to_host(q)[0])
         ca.reduce(q, dx, r, precision=ca.NAIVE)
         naive = float(r.to_host(q)[0])
-        ref = float(np.sum(x, dtype=np.float64))  # pairwise FP64: error << 1 float ulp here
+        # errors against the double-double value of the sum (workloads.py)
+        ref = W.sum_dd(x)
         ulp = float(np.spacing(np.float32(abs(ref))))
+        ca.reduce(q, dx, r, init=np.float32(1.5))
+        acc = float(r.to_host(q)[0])
+        ref_acc = W.sum_dd(x, 1.5)
         do = ca.DeviceArray(n, np.float32, q)
         fs = lambda: ca.scan(q, dx, do, False)
         ms2, _ = timed_steps(ca, q, fs, max(2, K // 2), 1)
         t2 = float(np.mean(ms2))
+        scan_err = W.err_scan(x, do.to_host(q), False)
+        ca.scan(q, dx, do, False, precision=ca.NAIVE)
+        scan_err_naive = W.err_scan(x, do.to_host(q), False)
+        rnd = lambda e: {k: float("%.4g" % v) for k, v in e.items()}
         sec["reduce_f32_lognormal_2p28"] = {
             "GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4), "frac": round(gbs(4 * n, t) / PEAK_GBS, 4),
+            "fp64_ref": "double-double sum",
             "compensated_err_ulp_vs_fp64": round(abs(got - ref) / ulp, 3),
+            "compensated_err_abs": float("%.4g" % abs(got - ref)),
+            "compensated_err_rel": float("%.4g" % (abs(got - ref) / abs(ref))),
             "naive_fp32_err_ulp_vs_fp64": round(abs(naive - ref) / ulp, 3),
+            "naive_fp32_err_abs": float("%.4g" % abs(naive - ref)),
+            "naive_fp32_err_rel": float("%.4g" % (abs(naive - ref) / abs(ref))),
+            "accumulate_init_1p5_err_ulp_vs_fp64": round(abs(acc - ref_acc) /
+                                                         float(np.spacing(np.float32(abs(ref_acc)))), 3),
             "inclusive_scan_GBps": round(gbs(8 * n, t2), 1), "inclusive_scan_ms": round(t2, 4),
-            "inclusive_scan_frac": round(gbs(8 * n, t2) / PEAK_GBS, 4)}
+            "inclusive_scan_frac": round(gbs(8 * n, t2) / PEAK_GBS, 4),
+            "inclusive_scan_err_vs_fp64_compensated": rnd(scan_err),
+            "inclusive_scan_err_vs_fp64_naive": rnd(scan_err_naive)}
         del dx, do, x
 
     # ---------------------------------------------------------------- saxpy
@@ -244,8 +347,16 @@ def main():
         fn = lambda: ca.saxpy(q, 2.5, dx, dy, do)
         ms, _ = timed_steps(ca, q, fn, K, Wm)
         t = float(np.mean(ms))
+        # FP64 reference: 2.5*x is exact in double, so one double rounding
+        # separates it from the exact a*x + y
+        ref = 2.5 * x.astype(np.float64) + y.astype(np.float64)
+        err = np.abs(do.to_host(q).astype(np.float64) - ref)
+        ulps = err / np.spacing(np.abs(ref).astype(np.float32)).astype(np.float64)
         sec["saxpy_f32_2p28"] = {"GBps": round(gbs(12 * n, t), 1), "ms": round(t, 4),
-                                 "frac": round(gbs(12 * n, t) / PEAK_GBS, 4)}
+                                 "frac": round(gbs(12 * n, t) / PEAK_GBS, 4),
+                                 "err_vs_fp64": {"max_abs": float("%.4g" % err.max()),
+                                                 "max_ulp": float("%.4g" % ulps.max())}}
+        del ref, err, ulps
         del dx, dy, do, x, y
 
     # -------------------------------------------- reduce_by_key / seg scan
@@ -261,13 +372,17 @@ def main():
         t = float(np.mean(ms))
         runs = int(cnt.to_host(q)[0])
         alg = 8 * n + 8 * runs
+        rnd = lambda e: {k: float("%.4g" % v) for k, v in e.items()}
+        assert np.array_equal(ko.to_host(q, count=runs), np.arange(runs, dtype=np.int32)), "reduce_by_key keys"
         sec["reduce_by_key_i32_f32_2p27"] = {"GBps": round(gbs(alg, t), 1), "ms": round(t, 4),
-                                             "frac": round(gbs(alg, t) / PEAK_GBS, 4), "runs": runs}
+                                             "frac": round(gbs(alg, t) / PEAK_GBS, 4), "runs": runs,
+                                             "run_sum_err_vs_fp64": rnd(W.err_runsum(k, v, vo.to_host(q, count=runs)))}
         fs = lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0)
         ms, _ = timed_steps(ca, q, fs, K, Wm)
         t = float(np.mean(ms))
         sec["segmented_exclusive_scan_2p27"] = {"GBps": round(gbs(12 * n, t), 1), "ms": round(t, 4),
-                                                "frac": round(gbs(12 * n, t) / PEAK_GBS, 4)}
+                                                "frac": round(gbs(12 * n, t) / PEAK_GBS, 4),
+                                                "err_vs_fp64": rnd(W.err_segscan(k, v, vo.to_host(q), True))}
         del dk, dv, ko, vo, k, v
 
     # -------------------------------------------------- stream compaction
@@ -345,8 +460,12 @@ def main():
         fv = lambda: ca.transform_reduce(q, np.float32, dx, None, n, mv, res1)
         ms, _ = timed_steps(ca, q, fv, K, Wm)
         t = float(np.mean(ms))
+        got = float(res1.to_host(q)[0])
+        c32 = float(np.float32(mean))
+        ref = float(np.sum(np.square(x.astype(np.float64) - c32)))
         sec["sum_sq_dev_f32_2p28"] = {"GBps": round(gbs(4 * n, t), 1), "ms": round(t, 4),
-                                      "frac": round(gbs(4 * n, t) / PEAK_GBS, 4)}
+                                      "frac": round(gbs(4 * n, t) / PEAK_GBS, 4),
+                                      "err_ulp_vs_fp64": round(abs(got - ref) / float(np.spacing(np.float32(ref))), 3)}
         del dx, dy, x, y, res1
 
     # ------------------------------------------- comparison (merge) sort
@@ -428,19 +547,29 @@ def main():
                 if ln > 1:
                     L.bcmi_sort(q.h, ca.STABLE_SORT_BY_KEY, ca.U64, wk.at(s_), wv.at(s_), 4, ln, 0)
 
+        # the same loop issued from C: no interpreter time between calls
+        per_call_c = lambda: ca.check(W.sort_per_segment(L.bcmi_sort, q.h, ca.STABLE_SORT_BY_KEY, ca.U64,
+                                                         wk.ptr, 8, wv.ptr, 4, off))
+
         parts = kernel_breakdown(q, fn, ["segsort_small", "segsort_insertion", "segsort_histogram",
                                          "segsort_digit_offsets", "segsort_route", "segsort_onesweep_msd",
                                          "segsort_groups", "segsort_onesweep", "segsort_copyback"],
                                  restore=restore)
         if args.no_per_call:
-            t_call = float("nan")
+            t_call = t_call_py = float("nan")
         else:
+            ms, wall = timed_steps(ca, q, per_call_c, 2, 1, restore)
+            t_call = float(min(ms))
+            got_c = wk.to_host(q)
+            assert np.array_equal(got_c, got), "per-call loop and batched plan disagree"
             ms, wall = timed_steps(ca, q, per_call, 1, 1, restore)
-            t_call = float(ms[0])
+            t_call_py = float(ms[0])
         sec["segmented_sort_u64_u32_65536seg"] = {
             "pairs": n, "segments": 65536,
             "batched_graph_ms": round(t_graph, 4), "batched_eager_ms": round(t_eager, 4),
             "per_call_ms": round(t_call, 2) if t_call == t_call else None,
+            "per_call_note": "one stable_sort_by_key call per segment, looped in C (workloads.cpp)",
+            "per_call_python_loop_ms": round(t_call_py, 2) if t_call_py == t_call_py else None,
             "speedup_graph_vs_per_call": round(t_call / t_graph, 1) if t_call == t_call else None,
             "kernels_ms_per_call": parts,
             "Gpairs_per_s": round(n / (t_graph * 1e-3) / 1e9, 3),
@@ -457,16 +586,7 @@ def main():
 
     # ---------------------------------------------------------- CPU baseline
     if not args.no_cpu_baseline and "sort" in only:
-        import oracle as O
-        m = 1 << 24
-        sample = O.gen_sort_u32(m)
-        t0 = time.perf_counter()
-        O.compare_stable_sort(sample)
-        dt = time.perf_counter() - t0
-        out["cpu_baseline"] = {"value": round(m / dt / 1e9, 5), "unit": "Gkeys/s", "cores": 1, "kind": "port",
-                               "sample": "first 2^24 of the same mt19937(0xB0057001) keys, sorted by the "
-                                         "oracle's restatement of merge_sort_on_cpu (the reference's "
-                                         "CPU-device sort path), one thread; %.2f s" % dt}
+        out["cpu_baseline"] = cpu_baseline()
     print(json.dumps(out), flush=True)
 
 

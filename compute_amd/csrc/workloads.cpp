@@ -32,9 +32,128 @@ double normal64(std::mt19937_64 &g)
     return std::sqrt(-2.0 * std::log(u1)) * std::cos(kTwoPi * u2);
 }
 
+// Double-double accumulator (Knuth two-sum, then a fast renormalisation):
+// about 106 bits, so its value stands in for the exact sum when the float
+// results' errors are measured.
+struct DD {
+    double hi = 0.0, lo = 0.0;
+    void add(double x)
+    {
+        const double s = hi + x;
+        const double bp = s - hi;
+        double e = (hi - (s - bp)) + (x - bp);
+        e += lo;
+        hi = s + e;
+        lo = e - (hi - s);
+    }
+    double value() const { return hi + lo; }
+};
+
+// spacing of float32 at |r| (the float ulp the error is counted in)
+double ulp_f32(double r)
+{
+    r = std::fabs(r);
+    if (r < 0x1p-126) return 0x1p-149;
+    int e;
+    std::frexp(r, &e);  // r = m * 2^e, m in [0.5, 1)
+    return std::ldexp(1.0, e - 24);
+}
+
+// Running maxima of |got - ref| (absolute, relative to |ref| where ref != 0,
+// and in float ulps of ref).
+struct ErrMax {
+    double abs = 0.0, rel = 0.0, ulp = 0.0;
+    void add(float got, double ref)
+    {
+        const double e = std::fabs((double)got - ref);
+        if (e > abs) abs = e;
+        if (ref != 0.0 && e / std::fabs(ref) > rel) rel = e / std::fabs(ref);
+        const double u = e / ulp_f32(ref);
+        if (u > ulp) ulp = u;
+    }
+    void out(double *o) const
+    {
+        o[0] = abs;
+        o[1] = rel;
+        o[2] = ulp;
+    }
+};
+
 }  // namespace
 
 extern "C" {
+
+// ---------------------------------------------- float error measurement
+// Each writes {max abs, max rel, max ulp} of a float32 result against the
+// double-double value of the same operation into err[3].
+
+// sum of x with init folded in first; returns the double-double value
+double wl_sum_f32_dd(const float *x, size_t n, double init)
+{
+    DD a;
+    a.add(init);
+    for (size_t i = 0; i < n; ++i) a.add((double)x[i]);
+    return a.value();
+}
+
+// inclusive (exclusive != 0: exclusive, seeded with init) prefix sums
+void wl_err_scan_f32(const float *x, const float *got, size_t n, int exclusive, double init, double *err)
+{
+    DD a;
+    a.add(init);
+    ErrMax m;
+    for (size_t i = 0; i < n; ++i) {
+        if (exclusive) {
+            m.add(got[i], a.value());
+            a.add((double)x[i]);
+        } else {
+            a.add((double)x[i]);
+            m.add(got[i], a.value());
+        }
+    }
+    m.out(err);
+}
+
+// segmented prefix sums over runs of equal keys (restarting at 0 per run)
+void wl_err_segscan_f32(const int32_t *keys, const float *x, const float *got, size_t n, int exclusive, double *err)
+{
+    DD a;
+    ErrMax m;
+    for (size_t i = 0; i < n; ++i) {
+        if (i > 0 && keys[i] != keys[i - 1]) a = DD();
+        if (exclusive) {
+            m.add(got[i], a.value());
+            a.add((double)x[i]);
+        } else {
+            a.add((double)x[i]);
+            m.add(got[i], a.value());
+        }
+    }
+    m.out(err);
+}
+
+// per-run sums (reduce_by_key): got[r] against run r's sum; returns the
+// number of runs found in keys
+size_t wl_err_runsum_f32(const int32_t *keys, const float *x, size_t n, const float *got, size_t runs, double *err)
+{
+    ErrMax m;
+    size_t r = 0;
+    DD a;
+    for (size_t i = 0; i < n; ++i) {
+        if (i > 0 && keys[i] != keys[i - 1]) {
+            if (r < runs) m.add(got[r], a.value());
+            ++r;
+            a = DD();
+        }
+        a.add((double)x[i]);
+    }
+    if (n) {
+        if (r < runs) m.add(got[r], a.value());
+        ++r;
+    }
+    m.out(err);
+    return r;
+}
 
 // uint32 sort keys: raw mt19937(seed) output
 void wl_sort_u32(uint32_t seed, uint32_t *keys, size_t n)
@@ -134,6 +253,27 @@ size_t wl_runs_i32_f32(int32_t *keys, float *vals, size_t n, size_t max_run)
         ++run;
     }
     return run;
+}
+
+// ------------------------------------------------- per-call baseline loop
+// One sort call per segment issued from C (the per-call baseline of the
+// segmented sort, without interpreter overhead between calls).  `sort` is
+// the library's bcmi_sort, passed in so this helper links nothing.
+typedef int (*wl_sort_fn)(void *queue, int api, int key_dtype, void *keys, void *values, size_t value_bytes,
+                          size_t n, int descending);
+
+int wl_sort_per_segment(void *sort, void *queue, int api, int key_dtype, char *keys, size_t key_bytes, char *vals,
+                        size_t value_bytes, const uint64_t *offsets, size_t nseg)
+{
+    wl_sort_fn fn = (wl_sort_fn)sort;
+    for (size_t s = 0; s < nseg; ++s) {
+        const size_t len = (size_t)(offsets[s + 1] - offsets[s]);
+        if (len < 2) continue;
+        const int r = fn(queue, api, key_dtype, keys + offsets[s] * key_bytes,
+                         vals ? vals + offsets[s] * value_bytes : nullptr, value_bytes, len, 0);
+        if (r != 0) return r;
+    }
+    return 0;
 }
 
 }  // extern "C"

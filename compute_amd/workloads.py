@@ -25,6 +25,14 @@ def lib():
         L.wl_segment_data.argtypes = [vp, vp, sz]
         L.wl_runs_i32_f32.argtypes = [vp, vp, sz, sz]
         L.wl_runs_i32_f32.restype = sz
+        dp = C.POINTER(C.c_double)
+        L.wl_sum_f32_dd.argtypes = [vp, sz, C.c_double]
+        L.wl_sum_f32_dd.restype = C.c_double
+        L.wl_err_scan_f32.argtypes = [vp, vp, sz, C.c_int, C.c_double, dp]
+        L.wl_err_segscan_f32.argtypes = [vp, vp, vp, sz, C.c_int, dp]
+        L.wl_err_runsum_f32.argtypes = [vp, vp, sz, vp, sz, dp]
+        L.wl_err_runsum_f32.restype = sz
+        L.wl_sort_per_segment.argtypes = [vp, vp, C.c_int, C.c_int, vp, sz, vp, sz, vp, sz]
         _lib = L
     return _lib
 
@@ -79,3 +87,37 @@ def runs_i32_f32(n, max_run=4096):
     k, v = np.empty(n, np.int32), np.empty(n, np.float32)
     lib().wl_runs_i32_f32(_p(k), _p(v), n, max_run)
     return k, v
+
+
+# ------------------------------------------------- float error measurement
+def _err(fn, *args):
+    e = (C.c_double * 3)()
+    r = fn(*args, e)
+    return {"max_abs": e[0], "max_rel": e[1], "max_ulp": e[2]}, r
+
+
+def sum_dd(x, init=0.0):
+    """Double-double value of init + sum(x) (float32 x)."""
+    return lib().wl_sum_f32_dd(_p(np.ascontiguousarray(x, np.float32)), x.size, float(init))
+
+
+def err_scan(x, got, exclusive, init=0.0):
+    """Max abs / rel / float-ulp error of a float32 prefix sum."""
+    return _err(lib().wl_err_scan_f32, _p(x), _p(got), x.size, int(exclusive), float(init))[0]
+
+
+def err_segscan(keys, x, got, exclusive):
+    return _err(lib().wl_err_segscan_f32, _p(keys), _p(x), _p(got), x.size, int(exclusive))[0]
+
+
+def err_runsum(keys, x, got):
+    e, runs = _err(lib().wl_err_runsum_f32, _p(keys), _p(x), x.size, _p(got), got.size)
+    assert runs == got.size, (runs, got.size)
+    return e
+
+
+def sort_per_segment(sort_fn, queue_handle, api, key_dtype, keys_ptr, key_bytes, vals_ptr, value_bytes, offsets):
+    """One library sort call per segment, looped in C; returns the status."""
+    off = np.ascontiguousarray(offsets, np.uint64)
+    return lib().wl_sort_per_segment(C.cast(sort_fn, C.c_void_p), queue_handle, api, key_dtype, keys_ptr,
+                                     key_bytes, vals_ptr, value_bytes, _p(off), off.size - 1)

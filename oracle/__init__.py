@@ -14,7 +14,10 @@ import os
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB_PATH = os.path.join(_HERE, "build", "liboracle.so")
+# BCMI_ORACLE_BUILD selects another build directory (the sanitizer build,
+# oracle/Makefile target "sanitize")
+_BUILD = os.environ.get("BCMI_ORACLE_BUILD", os.path.join(_HERE, "build"))
+_LIB_PATH = os.path.join(_BUILD, "liboracle.so")
 _lib = None
 
 # dtype codes (shared with the product C-ABI)
@@ -612,3 +615,84 @@ def insertion_sort_rows(rows, less, values=None):
         if vals is not None:
             vals[j] = vv
     return np.array(data).reshape(len(data), -1), vals
+
+
+# ------------------------------------------------- CPU-column baseline port
+# cpu_column.c: the reference's CPU-device algorithms on OpenMP threads, the
+# benchmark's timed CPU baseline (never part of the library).
+_cc = None
+_CC_PATH = os.path.join(_BUILD, "libcpucol.so")
+
+
+def cpucol():
+    global _cc
+    if _cc is None:
+        if not os.path.exists(_CC_PATH):
+            build()
+        L = C.CDLL(_CC_PATH)
+        vp, sz, i = C.c_void_p, C.c_size_t, C.c_int
+        L.cc_sort_u32.argtypes = [vp, vp, sz, i, i]
+        L.cc_sort_by_key_f32_u32.argtypes = [vp, vp, sz, i, i]
+        L.cc_sort_by_key_u64_u32.argtypes = [vp, vp, sz, i, i]
+        L.cc_segmented_sort_u64_u32.argtypes = [vp, vp, vp, sz, i]
+        L.cc_scan_i32.argtypes = [vp, vp, sz, i, C.c_int32, i]
+        L.cc_reduce_f32.argtypes = [vp, sz, i]
+        L.cc_reduce_f32.restype = C.c_float
+        L.cc_accumulate_f32.argtypes = [vp, sz, C.c_float]
+        L.cc_accumulate_f32.restype = C.c_float
+        L.cc_reduce_by_key_i32_f32.argtypes = [vp, vp, sz, vp, vp]
+        L.cc_reduce_by_key_i32_f32.restype = sz
+        L.cc_saxpy_f32.argtypes = [C.c_float, vp, vp, vp, sz, i]
+        _cc = L
+    return _cc
+
+
+def cc_sort(keys, vals=None, threads=1):
+    """merge_sort_on_cpu / merge_sort_by_key_on_cpu on `threads` threads
+    (uint32 keys alone, float32 or uint64 keys with uint32 values)."""
+    k = np.ascontiguousarray(keys).copy()
+    if vals is None:
+        assert k.dtype == np.uint32
+        cpucol().cc_sort_u32(_p(k), None, k.size, threads, 1)
+        return k
+    v = np.ascontiguousarray(vals, np.uint32).copy()
+    fn = {np.dtype(np.float32): cpucol().cc_sort_by_key_f32_u32,
+          np.dtype(np.uint64): cpucol().cc_sort_by_key_u64_u32}[k.dtype]
+    fn(_p(k), _p(v), k.size, threads, 0)
+    return k, v
+
+
+def cc_segmented_sort(keys, vals, offsets, threads=1):
+    k, v = keys.copy(), vals.copy()
+    off = np.ascontiguousarray(offsets, np.uint64)
+    cpucol().cc_segmented_sort_u64_u32(_p(k), _p(v), _p(off), off.size - 1, threads)
+    return k, v
+
+
+def cc_scan_i32(x, exclusive, init=0, threads=1):
+    x = np.ascontiguousarray(x, np.int32)
+    out = np.empty_like(x)
+    cpucol().cc_scan_i32(_p(x), _p(out), x.size, int(exclusive), int(init), threads)
+    return out
+
+
+def cc_reduce_f32(x, threads=1):
+    return np.float32(cpucol().cc_reduce_f32(_p(np.ascontiguousarray(x, np.float32)), x.size, threads))
+
+
+def cc_accumulate_f32(x, init=0.0):
+    return np.float32(cpucol().cc_accumulate_f32(_p(np.ascontiguousarray(x, np.float32)), x.size, init))
+
+
+def cc_reduce_by_key(keys, vals):
+    keys = np.ascontiguousarray(keys, np.int32)
+    vals = np.ascontiguousarray(vals, np.float32)
+    ko, vo = np.empty_like(keys), np.empty_like(vals)
+    r = cpucol().cc_reduce_by_key_i32_f32(_p(keys), _p(vals), keys.size, _p(ko), _p(vo))
+    return ko[:r], vo[:r]
+
+
+def cc_saxpy(a, x, y, threads=1):
+    out = np.empty_like(x)
+    cpucol().cc_saxpy_f32(a, _p(x), _p(y), _p(out), x.size, threads)
+    return out
