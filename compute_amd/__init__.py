@@ -34,7 +34,7 @@ EXPORTS = [
     "bcmi_queue_create", "bcmi_queue_destroy", "bcmi_queue_finish", "bcmi_queue_device",
     "bcmi_queue_native", "bcmi_queue_reserve", "bcmi_malloc", "bcmi_free", "bcmi_memcpy",
     "bcmi_fill", "bcmi_event_create", "bcmi_event_destroy", "bcmi_event_record",
-    "bcmi_event_synchronize", "bcmi_event_elapsed_ms", "bcmi_sort", "bcmi_scan", "bcmi_reduce",
+    "bcmi_event_synchronize", "bcmi_event_elapsed_ms", "bcmi_sort", "bcmi_sort_noalloc", "bcmi_scan", "bcmi_reduce",
     "bcmi_reduce_by_key", "bcmi_transform", "bcmi_saxpy", "bcmi_segmented_sort",
     "bcmi_segmented_plan_create", "bcmi_segmented_sort_plan", "bcmi_segmented_plan_destroy",
     "bcmi_segmented_scan", "bcmi_graph_begin", "bcmi_graph_end", "bcmi_graph_launch",
@@ -157,6 +157,7 @@ def lib():
         L.bcmi_event_synchronize.argtypes = [vp]
         L.bcmi_event_elapsed_ms.argtypes = [vp, vp, C.POINTER(C.c_float)]
         L.bcmi_sort.argtypes = [vp, i, i, vp, vp, sz, sz, i]
+        L.bcmi_sort_noalloc.argtypes = [vp, i, i, vp, vp, sz, sz, i]
         L.bcmi_scan.argtypes = [vp, i, i, vp, vp, sz, vp, i, i]
         L.bcmi_reduce.argtypes = [vp, i, i, vp, sz, vp, vp, C.POINTER(C.c_double), i]
         L.bcmi_reduce_by_key.argtypes = [vp, i, i, i, vp, vp, sz, vp, vp, vp, i]

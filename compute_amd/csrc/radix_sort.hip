@@ -19,6 +19,7 @@
 #include "radix.hpp"
 
 #include <cstring>
+#include <mutex>
 
 namespace bcmi {
 
@@ -137,6 +138,56 @@ __global__ void k_insertion(T *keys, typename ValT<VB>::type *vals, uint32_t n, 
 
 static size_t align256(size_t x) { return (x + 255) & ~(size_t)255; }
 
+// Workspace of one radix sort: the queue's arena and status array (base ==
+// nullptr), or a fixed region the caller hands in (the allocation-free entry
+// point's device scratch), carved into [status words | partial histograms |
+// key temporaries | value temporaries].
+struct SortWork {
+    unsigned char *base;
+    size_t bytes;
+};
+
+// Device scratch of bcmi_sort_noalloc: one copy per GPU, held until the
+// process ends.  Host code never passes `scratch` itself to a kernel (in
+// host code the name is a host-side stand-in); it uses the device address
+// hipGetSymbolAddress returned for the current device, kept per device.
+constexpr size_t kNoAllocScratchBytes = (size_t)64 << 20;
+constexpr int kNoAllocMaxDevices = 64;
+}  // namespace bcmi
+static __device__ __align__(256) unsigned char scratch[bcmi::kNoAllocScratchBytes];
+namespace bcmi {
+static unsigned char *g_noalloc_ptr[kNoAllocMaxDevices];
+static std::mutex g_noalloc_mu;
+
+// Called by bcmi_queue_create with `device` current: looks up this device's
+// copy of the scratch array once.
+int noalloc_scratch_setup(int device)
+{
+    BCMI_CHECK(device >= 0 && device < kNoAllocMaxDevices, BCMI_DEVICE_NOT_FOUND,
+               "device %d beyond the %d devices the allocation-free sort supports", device, kNoAllocMaxDevices);
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    if (g_noalloc_ptr[device]) return 0;
+    void *p = nullptr;
+    BCMI_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(::scratch)));
+    BCMI_CHECK(p != nullptr, BCMI_RUNTIME_ERROR, "no device address for the sort scratch on device %d", device);
+    g_noalloc_ptr[device] = (unsigned char *)p;
+    return 0;
+}
+
+static unsigned char *noalloc_scratch(int device)
+{
+    if (device < 0 || device >= kNoAllocMaxDevices) return nullptr;
+    std::lock_guard<std::mutex> lk(g_noalloc_mu);
+    return g_noalloc_ptr[device];
+}
+
+__global__ void k_clear_words(uint64_t *p, size_t n)
+{
+    size_t i = (size_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (; i < n; i += stride) p[i] = 0;
+}
+
 // LASTB: the last (most significant) pass ranks by ballots on 24-key tiles.
 // Float keys of a narrow range share few top-digit values, and many lanes
 // with one digit serialise the LDS-mask ranking's same-address atomics.
@@ -149,7 +200,7 @@ size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int c
 }
 
 template <typename KB, int VB, int IPT, int RANK, typename XF, bool LASTB = false>
-static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf)
+static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
@@ -165,17 +216,35 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     if ((uintptr_t)keys & 15) head = (uint32_t)((16 - ((uintptr_t)keys & 15)) / sizeof(KB));
     if (((uintptr_t)keys & 15) % sizeof(KB)) head = n;  // not element-aligned to a chunk
     if (head > n) head = n;
-    void *scr;
     const size_t kbytes = align256((size_t)n * sizeof(KB));
     const size_t vbytes = align256(VB ? (size_t)n * VB : 0);
-    BCMI_TRY(scratch(q, kbytes + vbytes + (size_t)hblocks * P * 256 * 4, &scr));
-    KB *ktmp = (KB *)scr;
-    V *vtmp = (V *)((char *)scr + kbytes);
-    uint32_t *partial = (uint32_t *)((char *)scr + kbytes + vbytes);
+    const size_t pbytes = align256((size_t)hblocks * P * 256 * 4);
     constexpr uint32_t TILEB = kSortThreads * 24;
     const uint32_t tilesb = LASTB ? (n + TILEB - 1) / TILEB : 0;
+    const size_t swords = (size_t)(tiles > tilesb ? tiles : tilesb) * 256;
+    unsigned char *scr;
     uint64_t *status;
-    BCMI_TRY(status_array(q, (size_t)(tiles > tilesb ? tiles : tilesb) * 256, &status));
+    if (work) {
+        const size_t need = align256(swords * 8) + pbytes + kbytes + vbytes;
+        BCMI_CHECK(need <= work->bytes, BCMI_INVALID_BUFFER_SIZE,
+                   "sort_noalloc: %u keys of %zu bytes with %d-byte values need %zu bytes of scratch; "
+                   "the allocation-free sort has %zu", n, sizeof(KB), VB, need, work->bytes);
+        status = (uint64_t *)work->base;
+        scr = work->base + align256(swords * 8);
+        // the words may carry another queue's epochs: clear them (a fill
+        // kernel, so the call stays capturable)
+        BCMI_LAUNCH(q, "radix_clear_status",
+                    hipLaunchKernelGGL(k_clear_words, dim3((unsigned)((swords + 255) / 256 < 4096 ? (swords + 255) / 256 : 4096)),
+                                       dim3(256), 0, q->stream, status, swords));
+    } else {
+        void *a;
+        BCMI_TRY(scratch(q, pbytes + kbytes + vbytes, &a));
+        scr = (unsigned char *)a;
+        BCMI_TRY(status_array(q, swords, &status));
+    }
+    uint32_t *partial = (uint32_t *)scr;
+    KB *ktmp = (KB *)(scr + pbytes);
+    V *vtmp = (V *)(scr + pbytes + kbytes);
 
     BCMI_LAUNCH(q, "radix_histogram",
                 hipLaunchKernelGGL((k_histogram<KB, XF>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
@@ -210,13 +279,14 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
 
 template <typename KB, int VB, int IPT, int RANK>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
-                      bool last_ballot = false)
+                      const SortWork *work, bool last_ballot = false)
 {
     return with_key_xf<KB>(kind, descending, [&](auto xf) {
         if constexpr (VB == 4 && RANK == kRankLdsMask) {
-            if (last_ballot) return radix_sort_xf<KB, VB, IPT, RANK, decltype(xf), true>(q, keys, vals, n, xf);
+            if (last_ballot)
+                return radix_sort_xf<KB, VB, IPT, RANK, decltype(xf), true>(q, keys, vals, n, xf, work);
         }
-        return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf);
+        return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf, work);
     });
 }
 
@@ -227,20 +297,23 @@ static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind
 // default of 32 items per thread with LDS peer-mask ranking was the fastest
 // measured on 2^27 uint32 keys (16: 3.00 ms, 24: 2.39 ms, 32: 2.25 ms).
 template <typename KB, int VB>
-static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc)
+static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc, const SortWork *work)
 {
     constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
     constexpr int IPT = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
+        // The allocation-free sort (work != nullptr) reads no tuning cache:
+        // its sizes are bounded by the scratch, where the size rule below
+        // picks 16 or 32 keys per thread anyway.
         const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
-        unsigned ipt = tune_get(key, "ipt", 32);
-        unsigned rank = tune_get(key, "rank", kRankLdsMask);
+        unsigned ipt = work ? 32 : tune_get(key, "ipt", 32);
+        unsigned rank = work ? (unsigned)kRankLdsMask : tune_get(key, "rank", kRankLdsMask);
         // The tuned tile is chosen on 2^27 keys; smaller sorts need more,
         // smaller tiles to fill the chip.  Measured on MI355X (u32 keys; f32
         // keys + u32 values): 16 keys per thread is fastest up to 2^19 keys
         // (2^20 with values), 32 with LDS ranking up to 2^23 (2^25 with
         // values).  Param "fit_size" = 0 keeps the tuned tile at every size.
-        const bool fit = tune_get(key, "fit_size", 1) != 0;
+        const bool fit = work ? true : tune_get(key, "fit_size", 1) != 0;
         if (fit) {
             const uint32_t small = VB ? (1u << 20) : (1u << 19), mid = VB ? (1u << 25) : (1u << 23);
             if (n <= small) {
@@ -253,32 +326,34 @@ static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int 
         }
         // pairs: param "last_ballot" = 1 ranks the top digit by ballots
         // (chosen on 2^27 pairs; with fit_size, only above 2^25)
-        const bool lb = VB && rank == kRankLdsMask && tune_get(key, "last_ballot", 0) && (!fit || n > (1u << 25));
-        if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc);
-        if (ipt == 16) return radix_sort<KB, VB, 16, kRankLdsMask>(q, keys, vals, n, kind, desc);
-        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
-        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
-        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
-        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc, lb);
+        const bool lb = !work && VB && rank == kRankLdsMask && tune_get(key, "last_ballot", 0) &&
+                        (!fit || n > (1u << 25));
+        if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc, work);
+        if (ipt == 16) return radix_sort<KB, VB, 16, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
+        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
+        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
+        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
+        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
         if constexpr (VB == 0) {
-            if (ipt == 64) return radix_sort<KB, VB, 64, kRankLdsMask>(q, keys, vals, n, kind, desc);
+            if (ipt == 64) return radix_sort<KB, VB, 64, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
         }
         // any other value falls through to the generic tile below
     }
-    return radix_sort<KB, VB, IPT, kRankLdsMask>(q, keys, vals, n, kind, desc);
+    return radix_sort<KB, VB, IPT, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
 }
 
 template <typename KB>
-static int radix_sort_vb(bcmi_queue q, void *keys, void *vals, size_t vb, uint32_t n, int kind, int desc)
+static int radix_sort_vb(bcmi_queue q, void *keys, void *vals, size_t vb, uint32_t n, int kind, int desc,
+                         const SortWork *work)
 {
     if (!vals) vb = 0;
     switch (vb) {
-    case 0: return radix_sort_ipt<KB, 0>(q, keys, nullptr, n, kind, desc);
-    case 1: return radix_sort_ipt<KB, 1>(q, keys, vals, n, kind, desc);
-    case 2: return radix_sort_ipt<KB, 2>(q, keys, vals, n, kind, desc);
-    case 4: return radix_sort_ipt<KB, 4>(q, keys, vals, n, kind, desc);
-    case 8: return radix_sort_ipt<KB, 8>(q, keys, vals, n, kind, desc);
-    case 16: return radix_sort_ipt<KB, 16>(q, keys, vals, n, kind, desc);
+    case 0: return radix_sort_ipt<KB, 0>(q, keys, nullptr, n, kind, desc, work);
+    case 1: return radix_sort_ipt<KB, 1>(q, keys, vals, n, kind, desc, work);
+    case 2: return radix_sort_ipt<KB, 2>(q, keys, vals, n, kind, desc, work);
+    case 4: return radix_sort_ipt<KB, 4>(q, keys, vals, n, kind, desc, work);
+    case 8: return radix_sort_ipt<KB, 8>(q, keys, vals, n, kind, desc, work);
+    case 16: return radix_sort_ipt<KB, 16>(q, keys, vals, n, kind, desc, work);
     default: set_error("sort: value size %zu unsupported (1, 2, 4, 8 or 16 bytes)", vb); return BCMI_INVALID_VALUE;
     }
 }
@@ -320,8 +395,8 @@ static int dtype_kind(int dt)
 
 using namespace bcmi;
 
-extern "C" int bcmi_sort(bcmi_queue q, int api, int key_dtype, void *keys, void *values, size_t value_bytes, size_t n,
-                         int descending)
+static int sort_entry(bcmi_queue q, int api, int key_dtype, void *keys, void *values, size_t value_bytes, size_t n,
+                      int descending, const SortWork *work)
 {
     BCMI_TRY(use_queue(q));
     BCMI_CHECK(dtype_valid(key_dtype), BCMI_INVALID_VALUE, "sort: bad key dtype %d", key_dtype);
@@ -352,9 +427,33 @@ extern "C" int bcmi_sort(bcmi_queue q, int api, int key_dtype, void *keys, void 
     }
     const int kind = dtype_kind(key_dtype);
     switch (dtype_size(key_dtype)) {
-    case 1: return radix_sort_vb<uint8_t>(q, keys, values, value_bytes, n32, kind, desc);
-    case 2: return radix_sort_vb<uint16_t>(q, keys, values, value_bytes, n32, kind, desc);
-    case 4: return radix_sort_vb<uint32_t>(q, keys, values, value_bytes, n32, kind, desc);
-    default: return radix_sort_vb<uint64_t>(q, keys, values, value_bytes, n32, kind, desc);
+    case 1: return radix_sort_vb<uint8_t>(q, keys, values, value_bytes, n32, kind, desc, work);
+    case 2: return radix_sort_vb<uint16_t>(q, keys, values, value_bytes, n32, kind, desc, work);
+    case 4: return radix_sort_vb<uint32_t>(q, keys, values, value_bytes, n32, kind, desc, work);
+    default: return radix_sort_vb<uint64_t>(q, keys, values, value_bytes, n32, kind, desc, work);
     }
+}
+
+extern "C" int bcmi_sort(bcmi_queue q, int api, int key_dtype, void *keys, void *values, size_t value_bytes, size_t n,
+                         int descending)
+{
+    return sort_entry(q, api, key_dtype, keys, values, value_bytes, n, descending, nullptr);
+}
+
+// The same sort with every temporary in this device's copy of `scratch`:
+// no allocation on any path.  Calls that overlap on one GPU (two queues at
+// once) share the array and race; the array keeps the last call's contents.
+extern "C" int bcmi_sort_noalloc(bcmi_queue q, int api, int key_dtype, void *keys, void *values, size_t value_bytes,
+                                 size_t n, int descending)
+{
+    BCMI_TRY(use_queue(q));
+    unsigned char *p = noalloc_scratch(q->device);
+    BCMI_CHECK(p != nullptr, BCMI_INVALID_COMMAND_QUEUE, "sort_noalloc: no scratch set up for device %d", q->device);
+    const SortWork w{p, kNoAllocScratchBytes};
+    // the queue's kernel profiler creates its events on demand: off here
+    const bool prof = q->profile;
+    q->profile = false;
+    const int r = sort_entry(q, api, key_dtype, keys, values, value_bytes, n, descending, &w);
+    q->profile = prof;
+    return r;
 }

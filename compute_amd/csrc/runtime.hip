@@ -254,6 +254,7 @@ int bcmi_queue_create(int device, bcmi_queue *queue)
     BCMI_CHECK(std::strncmp(p.gcnArchName, "gfx950", 6) == 0, BCMI_DEVICE_NOT_FOUND,
                "device %d is %s; this library is built for gfx950 only", device, p.gcnArchName);
     BCMI_TRY(tune_autoload());
+    BCMI_TRY(noalloc_scratch_setup(device));
     static std::atomic<uint64_t> next_id{1};
     bcmi_queue q = new bcmi_queue_s();
     q->id = next_id.fetch_add(1);

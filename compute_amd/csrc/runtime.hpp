@@ -129,6 +129,10 @@ int check_launch(const char *name);
 // kept beside a sort go after it.
 size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int cu_count);
 
+// Looks up the calling device's copy of the allocation-free sort's scratch
+// (radix_sort.hip); bcmi_queue_create calls it once per queue.
+int noalloc_scratch_setup(int device);
+
 // Persistent grid sizing: blocks of `threads` that fill the device.
 inline int cu_blocks(bcmi_queue q, int per_cu) { return q->cu_count * per_cu; }
 

@@ -155,6 +155,20 @@ int bcmi_event_elapsed_ms(bcmi_event start, bcmi_event end, float *ms);
 int bcmi_sort(bcmi_queue queue, int api, int key_dtype, void *keys,
               void *values, size_t value_bytes, size_t n, int descending);
 
+/* bcmi_sort with no allocation on any path.  No counterpart in the
+ * reference: it is for callers that cannot allocate device memory (code
+ * under HIP graph capture, tight serving loops, harnesses that forbid
+ * hipMalloc).  Same arguments, order and results as bcmi_sort; every
+ * temporary (key and value copies, histograms, look-back words) lives in a
+ * 64 MiB device array of the library, one copy per GPU, held until the
+ * process ends.  A sort whose temporaries exceed it (about 15 M 4-byte keys,
+ * 7 M 4-byte pairs) fails with BCMI_INVALID_BUFFER_SIZE before any work is
+ * enqueued.  All calls on one GPU share the array: calls that overlap on
+ * that GPU (two queues at once) race.  The queue's kernel profiler does not
+ * record these calls. */
+int bcmi_sort_noalloc(bcmi_queue queue, int api, int key_dtype, void *keys,
+                      void *values, size_t value_bytes, size_t n, int descending);
+
 /* exclusive_scan (exclusive_scan.hpp:55-104), inclusive_scan
  * (inclusive_scan.hpp:53-87), partial_sum (partial_sum.hpp:31-41).
  * in == out is allowed.  init: host pointer to one element (exclusive only;
