@@ -41,7 +41,8 @@ EXPORTS = [
     "bcmi_graph_destroy", "bcmi_tune_get", "bcmi_tune_set", "bcmi_tuning_cache_load",
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
-    "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free", "bcmi_merge_sort",
+    "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free", "bcmi_merge_sort", "bcmi_scan_iter",
+    "bcmi_transform_iter",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -61,6 +62,31 @@ class Seq(C.Structure):
 class Map(C.Structure):
     """bcmi_map: y = g(a op b) (binary) or g(a - c) (unary)."""
     _fields_ = [("op", C.c_int32), ("fn", C.c_int32), ("offset", C.c_uint64)]
+
+
+ITER_BUFFER, ITER_COUNTING, ITER_CONSTANT = 0, 1, 2
+MAP_ID, MAP_ABS, MAP_SQUARE, MAP_LENGTH, MAP_CMP = 0, 1, 2, 3, 8
+
+
+class Iter(C.Structure):
+    """bcmi_iter: an array, counting or constant sequence and its element map."""
+    _fields_ = [("ptr", C.c_void_p), ("kind", C.c_int32), ("fn", C.c_int32), ("value", C.c_uint64),
+                ("offset", C.c_uint64)]
+
+
+def _elem_bits(dtype, v):
+    a = np.zeros(1, np.uint64)
+    a.view(np.uint8)[:np.dtype(dtype).itemsize] = np.array([v], dtype=dtype).view(np.uint8)
+    return int(a[0])
+
+
+def iterator(dtype, array=None, kind=None, value=0, fn=MAP_ID, offset=0):
+    """Build a bcmi_iter: an array (DeviceArray or device pointer), or a
+    counting / constant sequence starting at / equal to value."""
+    if kind is None:
+        kind = ITER_BUFFER if array is not None else ITER_COUNTING
+    ptr = array.ptr if hasattr(array, "ptr") else array
+    return Iter(ptr, kind, fn, _elem_bits(dtype, value), _elem_bits(dtype, offset))
 
 
 def mapping(dtype, fn="id", op="multiplies", c=0):
@@ -186,6 +212,8 @@ def lib():
         L.bcmi_partition.argtypes = [vp, i, vp, vp, vp, sz, pp, vp]
         L.bcmi_extract_keys.argtypes = [vp, i, i, vp, sz, sz, sz, vp]
         L.bcmi_merge_sort.argtypes = [vp, C.POINTER(Comparator), vp, sz, vp, sz, sz]
+        L.bcmi_scan_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), vp, sz, vp, i, i]
+        L.bcmi_transform_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), C.POINTER(Iter), vp, sz]
         L.bcmi_host_malloc.argtypes = [sz, C.POINTER(vp)]
         L.bcmi_host_free.argtypes = [vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,

@@ -27,35 +27,6 @@ struct MapDev {
     uint64_t c;
 };
 
-template <typename T>
-BDEV T map_bits(uint64_t b)
-{
-    T v;
-    memcpy(&v, &b, sizeof(T));
-    return v;
-}
-
-template <typename T>
-BDEV T map_abs(T x)
-{
-    if constexpr (std::is_floating_point<T>::value) {
-        return __builtin_fabs(x);
-    } else if constexpr (std::is_signed<T>::value) {
-        using U = typename wrap_t<T>::type;
-        return x < 0 ? (T)(U)(0u - (U)x) : x;  // abs(INT_MIN) wraps to INT_MIN, as OpenCL's abs bits
-    } else {
-        return x;
-    }
-}
-
-template <typename T>
-BDEV T map_sub(T a, T b)
-{
-    using U = typename wrap_t<T>::type;
-    if constexpr (std::is_floating_point<T>::value) return a - b;
-    else return (T)(U)((U)a - (U)b);
-}
-
 // Fold y = map(a, b) into the state.  FN is the map's g, OPB the binary op
 // (kMapUnary: y = g(a - c)).  The last multiplication of a product or square goes
 // through add_prod, so compensated policies keep its rounding error.

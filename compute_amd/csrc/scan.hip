@@ -63,6 +63,17 @@ struct ScanCompF32 {
     }
 };
 
+// Where a tile's elements come from.  ArraySrc: an array (every plain
+// scan).  IterIn (ops.hpp): a sequence as the reference's iterators give
+// it, mapped element by element.
+template <typename T>
+struct ArraySrc {
+    const T *ptr;
+    BDEV bool array() const { return true; }
+    BDEV T gen(size_t) const { return T(); }
+    BDEV T map(T x) const { return x; }
+};
+
 template <typename T>
 struct alignas(16) sc_chunk {
     static constexpr int N = 16 / sizeof(T);
@@ -80,8 +91,8 @@ struct ScanLds {
 // w*WSEG + q*64*V + l*V .. +V-1 of the tile.  FULL: the tile lies below n, so
 // no element needs a bounds test.  VEC: input and output are 16-byte
 // aligned, so full chunks move as one 16-byte load / store.
-template <typename T, typename P, bool VEC, int Q, bool FULL>
-BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
+template <typename T, typename P, bool VEC, int Q, bool FULL, typename Src>
+BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
                     typename P::Cr init, int exclusive, uint32_t back)
 {
     using S = typename P::S;
@@ -94,23 +105,30 @@ BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, ui
     const uint32_t tile = blockIdx.x;
     // wave-uniform base, so every access is one 32-bit lane offset + immediates
     const size_t wstart = (size_t)tile * TILE + (size_t)__builtin_amdgcn_readfirstlane(w) * WSEG;
-    const T *win = in + wstart;
+    const T *win = src.ptr + wstart;
     T *wout = out + wstart;
     const int64_t rem = FULL ? (int64_t)WSEG : (int64_t)n - (int64_t)wstart;
     auto ok = [&](int q, int k) { return FULL || (int64_t)(q * 64 * V + lane * V + k) < rem; };
 
     T x[Q][V];
+    if (src.array()) {
 #pragma unroll
-    for (int q = 0; q < Q; ++q) {
-        const T *src = win + q * 64 * V + lane * V;
-        if (VEC && FULL) {
-            const sc_chunk<T> c = *(const sc_chunk<T> *)src;
+        for (int q = 0; q < Q; ++q) {
+            const T *p = win + q * 64 * V + lane * V;
+            if (VEC && FULL) {
+                const sc_chunk<T> c = *(const sc_chunk<T> *)p;
 #pragma unroll
-            for (int k = 0; k < V; ++k) x[q][k] = c.v[k];
-        } else {
+                for (int k = 0; k < V; ++k) x[q][k] = src.map(c.v[k]);
+            } else {
 #pragma unroll
-            for (int k = 0; k < V; ++k) x[q][k] = ok(q, k) ? src[k] : T(0);
+                for (int k = 0; k < V; ++k) x[q][k] = ok(q, k) ? src.map(p[k]) : T();
+            }
         }
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q)
+#pragma unroll
+            for (int k = 0; k < V; ++k) x[q][k] = src.map(src.gen(wstart + q * 64 * V + lane * V + k));
     }
     // chunk folds, then wave scans of the folds carried across q
     auto mrg = [](S a, S b) { return P::merge(a, b); };
@@ -163,58 +181,63 @@ BDEV void scan_body(ScanLds<P> &L, const T *in, T *out, size_t n, Ctrl *ctrl, ui
     }
 }
 
-template <typename T, typename P, bool VEC, int Q>
-__global__ __launch_bounds__(kScanThreads) void k_scan(const T *in, T *out, size_t n, Ctrl *ctrl,
+template <typename T, typename P, bool VEC, int Q, typename Src>
+__global__ __launch_bounds__(kScanThreads) void k_scan(const Src src, T *out, size_t n, Ctrl *ctrl,
                                                        uint64_t *status, typename P::Cr init, int exclusive,
                                                        uint32_t back)
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
     __shared__ ScanLds<P> lds;
     if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        scan_body<T, P, VEC, Q, true>(lds, in, out, n, ctrl, status, init, exclusive, back);
+        scan_body<T, P, VEC, Q, true>(lds, src, out, n, ctrl, status, init, exclusive, back);
     else
-        scan_body<T, P, VEC, Q, false>(lds, in, out, n, ctrl, status, init, exclusive, back);
+        scan_body<T, P, VEC, Q, false>(lds, src, out, n, ctrl, status, init, exclusive, back);
 }
 
-template <typename T, typename P, int Q>
-static int launch_scan(bcmi_queue q, const T *in, T *out, size_t n, uint64_t *status, typename P::Cr ini,
+template <typename T, typename P, int Q, typename Src>
+static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t *status, typename P::Cr ini,
                        int exclusive)
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
     size_t tiles = (n + TILE - 1) / TILE;
     BCMI_CHECK(tiles < (1ull << 31), BCMI_INVALID_BUFFER_SIZE, "scan: n=%zu too large", n);
-    bool vec = (((uintptr_t)in | (uintptr_t)out) & 15) == 0;
+    bool vec = (((uintptr_t)src.ptr | (uintptr_t)out) & 15) == 0;
     BCMI_TRY(begin_lookback(q, 1));
     if (vec) {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, true, Q>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
-                                       q->stream, in, out, n, q->ctrl, status, ini, exclusive, 1u));
+                    hipLaunchKernelGGL((k_scan<T, P, true, Q, Src>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                                       q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     } else {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, false, Q>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
-                                       q->stream, in, out, n, q->ctrl, status, ini, exclusive, 1u));
+                    hipLaunchKernelGGL((k_scan<T, P, false, Q, Src>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                                       q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     }
     return 0;
 }
 
 // Chunks per thread come from the tuning cache (key "__bcmi_scan", param
-// "q": 4, 8 or 16) for 4-byte types; 4 otherwise.
-template <typename T, typename P>
-static int run_scan(bcmi_queue q, const void *in_, void *out_, size_t n, const void *init, int exclusive)
+// "q": 4, 8 or 16) for 4-byte array scans; 4 otherwise.
+template <typename T, typename P, typename Src>
+static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, const void *init, int exclusive)
 {
-    const T *in = (const T *)in_;
     T *out = (T *)out_;
     const size_t tiles_max = (n + (size_t)4 * 4 * 64 - 1) / ((size_t)4 * 4 * 64) + 1;
     uint64_t *status;
     BCMI_TRY(status_array(q, tiles_max * (sizeof(typename P::Cr) / 4), &status));
     typename P::Cr ini = P::cid();
     if (exclusive && init) ini = P::host_init(init);
-    if constexpr (sizeof(T) == 4) {
+    if constexpr (sizeof(T) == 4 && std::is_same<Src, ArraySrc<T>>::value) {
         const unsigned qq = tune_get("__bcmi_scan", "q", 16);
-        if (qq == 16) return launch_scan<T, P, 16>(q, in, out, n, status, ini, exclusive);
-        if (qq == 8) return launch_scan<T, P, 8>(q, in, out, n, status, ini, exclusive);
+        if (qq == 16) return launch_scan<T, P, 16>(q, src, out, n, status, ini, exclusive);
+        if (qq == 8) return launch_scan<T, P, 8>(q, src, out, n, status, ini, exclusive);
     }
-    return launch_scan<T, P, 4>(q, in, out, n, status, ini, exclusive);
+    return launch_scan<T, P, 4>(q, src, out, n, status, ini, exclusive);
+}
+
+template <typename T, typename P>
+static int run_scan(bcmi_queue q, const void *in, void *out, size_t n, const void *init, int exclusive)
+{
+    return run_scan_src<T, P>(q, ArraySrc<T>{(const T *)in}, out, n, init, exclusive);
 }
 
 template <typename T>
@@ -254,5 +277,79 @@ extern "C" int bcmi_scan(bcmi_queue q, int dtype, int op, const void *in, void *
         return scan_plain<float>(q, op, in, out, n, init, exclusive);
     case BCMI_F64: return scan_plain<double>(q, op, in, out, n, init, exclusive);
     default: set_error("scan: bad dtype %d", dtype); return BCMI_INVALID_VALUE;
+    }
+}
+
+// ------------------------------------------------ fancy-iterator inputs
+namespace bcmi {
+
+template <typename T>
+static int scan_iter_scalar(bcmi_queue q, int op, const bcmi_iter *in, void *out, size_t n, const void *init,
+                            int excl, int precision)
+{
+    const IterIn<T> src = iter_in<T>(in);
+    if constexpr (std::is_same<T, float>::value) {
+        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED)
+            return run_scan_src<T, ScanCompF32>(q, src, out, n, init, excl);
+    }
+    switch (op) {
+    case BCMI_PLUS: return run_scan_src<T, ScanPlain<T, BCMI_PLUS>>(q, src, out, n, init, excl);
+    case BCMI_MULTIPLIES: return run_scan_src<T, ScanPlain<T, BCMI_MULTIPLIES>>(q, src, out, n, init, excl);
+    case BCMI_MIN: return run_scan_src<T, ScanPlain<T, BCMI_MIN>>(q, src, out, n, init, excl);
+    case BCMI_MAX: return run_scan_src<T, ScanPlain<T, BCMI_MAX>>(q, src, out, n, init, excl);
+    default: set_error("scan: unsupported op %d", op); return BCMI_INVALID_VALUE;
+    }
+}
+
+// vector elements: plus, component by component, from an array
+template <typename E, int W>
+static int scan_iter_vector(bcmi_queue q, const bcmi_iter *in, void *out, size_t n, const void *init, int excl)
+{
+    using T = vecn<E, W>;
+    return run_scan<T, ScanPlain<T, BCMI_PLUS>>(q, in->ptr, out, n, init, excl);
+}
+
+}  // namespace bcmi
+
+extern "C" int bcmi_scan_iter(bcmi_queue q, int dtype, int width, int op, const bcmi_iter *in, void *out, size_t n,
+                              const void *init, int exclusive, int precision)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(in, BCMI_INVALID_VALUE, "scan: null input descriptor");
+    BCMI_CHECK(in->kind >= BCMI_ITER_BUFFER && in->kind <= BCMI_ITER_CONSTANT, BCMI_INVALID_VALUE,
+               "scan: bad iterator kind %d", in->kind);
+    BCMI_CHECK((in->fn >= BCMI_MAP_ID && in->fn <= BCMI_MAP_SQUARE) ||
+                   (in->fn >= BCMI_MAP_CMP && in->fn <= BCMI_MAP_CMP + BCMI_CMP_NE),
+               BCMI_INVALID_VALUE, "scan: bad element map %d", in->fn);
+    if (n == 0) return 0;
+    BCMI_CHECK(out && (in->kind != BCMI_ITER_BUFFER || in->ptr), BCMI_INVALID_VALUE, "scan: null pointer");
+    if (width == 2 || width == 4) {
+        BCMI_CHECK(op == BCMI_PLUS && in->kind == BCMI_ITER_BUFFER && in->fn == BCMI_MAP_ID && in->offset == 0,
+                   BCMI_INVALID_VALUE, "scan: vector elements take plus over an array only");
+        switch (dtype) {
+        case BCMI_I32: return width == 2 ? scan_iter_vector<int32_t, 2>(q, in, out, n, init, exclusive)
+                                         : scan_iter_vector<int32_t, 4>(q, in, out, n, init, exclusive);
+        case BCMI_U32: return width == 2 ? scan_iter_vector<uint32_t, 2>(q, in, out, n, init, exclusive)
+                                         : scan_iter_vector<uint32_t, 4>(q, in, out, n, init, exclusive);
+        case BCMI_F32: return width == 2 ? scan_iter_vector<float, 2>(q, in, out, n, init, exclusive)
+                                         : scan_iter_vector<float, 4>(q, in, out, n, init, exclusive);
+        default: set_error("scan: vector elements of dtype %d unsupported (int, uint, float)", dtype);
+                 return BCMI_INVALID_VALUE;
+        }
+    }
+    BCMI_CHECK(width == 1, BCMI_INVALID_VALUE, "scan: element width %d unsupported (1, 2 or 4)", width);
+    // a plain array: the tuned array scan
+    if (in->kind == BCMI_ITER_BUFFER && in->fn == BCMI_MAP_ID && in->offset == 0)
+        return bcmi_scan(q, dtype, op, in->ptr, out, n, init, exclusive, precision);
+    switch (dtype) {
+    case BCMI_I32: return scan_iter_scalar<int32_t>(q, op, in, out, n, init, exclusive, precision);
+    case BCMI_U32: return scan_iter_scalar<uint32_t>(q, op, in, out, n, init, exclusive, precision);
+    case BCMI_I64: return scan_iter_scalar<int64_t>(q, op, in, out, n, init, exclusive, precision);
+    case BCMI_U64: return scan_iter_scalar<uint64_t>(q, op, in, out, n, init, exclusive, precision);
+    case BCMI_F32: return scan_iter_scalar<float>(q, op, in, out, n, init, exclusive, precision);
+    case BCMI_F64: return scan_iter_scalar<double>(q, op, in, out, n, init, exclusive, precision);
+    default: set_error("scan: generated or mapped inputs of dtype %d unsupported (32/64-bit int, float, double)",
+                       dtype);
+             return BCMI_INVALID_VALUE;
     }
 }

@@ -227,3 +227,106 @@ extern "C" int bcmi_transform(bcmi_queue q, int dtype, int op, const void *a, co
     default: set_error("transform: bad dtype %d", dtype); return BCMI_INVALID_VALUE;
     }
 }
+
+// ------------------------------------------------ fancy-iterator inputs
+// out[i] = op(a_i, b_i) (or a_i alone) where each operand is an array, a
+// counting or a constant sequence, mapped element by element (IterIn).  A
+// grid-stride loop: generated operands read nothing, arrays are read with
+// coalesced scalar loads.
+template <typename T, int OP>
+__global__ __launch_bounds__(kTfThreads) void k_transform_iter(const IterIn<T> a, const IterIn<T> b, int binary,
+                                                               T *__restrict__ out, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * kTfThreads;
+    for (size_t i = (size_t)blockIdx.x * kTfThreads + threadIdx.x; i < n; i += stride) {
+        const T x = a.at(i);
+        out[i] = binary ? BinOp<T, OP>()(x, b.at(i)) : x;
+    }
+}
+
+// length<floatW_>: the Euclidean length of each W-component float element
+template <int W>
+__global__ __launch_bounds__(kTfThreads) void k_vector_length(const float *__restrict__ a, float *__restrict__ out,
+                                                              size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * kTfThreads;
+    for (size_t i = (size_t)blockIdx.x * kTfThreads + threadIdx.x; i < n; i += stride) {
+        float s = 0.0f;
+#pragma unroll
+        for (int c = 0; c < W; ++c) s = __builtin_fmaf(a[i * W + c], a[i * W + c], s);
+        out[i] = __builtin_sqrtf(s);
+    }
+}
+
+template <typename T>
+static int transform_iter_typed(bcmi_queue q, int op, const bcmi_iter *a, const bcmi_iter *b, void *out, size_t n)
+{
+    const IterIn<T> ia = iter_in<T>(a);
+    IterIn<T> ib = ia;
+    if (b) ib = iter_in<T>(b);
+    const int binary = b ? 1 : 0;
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kTfThreads - 1) / kTfThreads, (size_t)cu_blocks(q, 8));
+    T *o = (T *)out;
+#define BCMI_TF_ITER(OPV)                                                                                       \
+    BCMI_LAUNCH(q, "transform_iter", hipLaunchKernelGGL((k_transform_iter<T, OPV>), dim3(blocks), dim3(kTfThreads), \
+                                                        0, q->stream, ia, ib, binary, o, n));                   \
+    return 0
+    switch (op) {
+    case BCMI_PLUS: BCMI_TF_ITER(BCMI_PLUS);
+    case BCMI_MINUS: BCMI_TF_ITER(BCMI_MINUS);
+    case BCMI_MULTIPLIES: BCMI_TF_ITER(BCMI_MULTIPLIES);
+    case BCMI_DIVIDES: BCMI_TF_ITER(BCMI_DIVIDES);
+    case BCMI_MIN: BCMI_TF_ITER(BCMI_MIN);
+    case BCMI_MAX: BCMI_TF_ITER(BCMI_MAX);
+    default: set_error("transform: unsupported op %d", op); return BCMI_INVALID_VALUE;
+    }
+#undef BCMI_TF_ITER
+}
+
+static int check_iter(const bcmi_iter *it, bool allow_length)
+{
+    BCMI_CHECK(it->kind >= BCMI_ITER_BUFFER && it->kind <= BCMI_ITER_CONSTANT, BCMI_INVALID_VALUE,
+               "transform: bad iterator kind %d", it->kind);
+    BCMI_CHECK((it->fn >= BCMI_MAP_ID && it->fn <= (allow_length ? BCMI_MAP_LENGTH : BCMI_MAP_SQUARE)) ||
+                   (it->fn >= BCMI_MAP_CMP && it->fn <= BCMI_MAP_CMP + BCMI_CMP_NE),
+               BCMI_INVALID_VALUE, "transform: bad element map %d", it->fn);
+    BCMI_CHECK(it->kind != BCMI_ITER_BUFFER || it->ptr, BCMI_INVALID_VALUE, "transform: null input array");
+    return 0;
+}
+
+extern "C" int bcmi_transform_iter(bcmi_queue q, int dtype, int width, int op, const bcmi_iter *a,
+                                   const bcmi_iter *b, void *out, size_t n)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(a, BCMI_INVALID_VALUE, "transform: null input descriptor");
+    BCMI_TRY(check_iter(a, width > 1));
+    if (b) BCMI_TRY(check_iter(b, false));
+    if (n == 0) return 0;
+    BCMI_CHECK(out, BCMI_INVALID_VALUE, "transform: null output");
+    if (width > 1) {
+        BCMI_CHECK((width == 2 || width == 4) && dtype == BCMI_F32 && !b && a->kind == BCMI_ITER_BUFFER &&
+                       a->fn == BCMI_MAP_LENGTH,
+                   BCMI_INVALID_VALUE, "transform: vector elements take length<floatN_> over an array only");
+        const unsigned blocks =
+            (unsigned)std::min<size_t>((n + kTfThreads - 1) / kTfThreads, (size_t)cu_blocks(q, 8));
+        if (width == 2)
+            BCMI_LAUNCH(q, "vector_length", hipLaunchKernelGGL((k_vector_length<2>), dim3(blocks), dim3(kTfThreads),
+                                                                0, q->stream, (const float *)a->ptr, (float *)out, n));
+        else
+            BCMI_LAUNCH(q, "vector_length", hipLaunchKernelGGL((k_vector_length<4>), dim3(blocks), dim3(kTfThreads),
+                                                                0, q->stream, (const float *)a->ptr, (float *)out, n));
+        return 0;
+    }
+    BCMI_CHECK(width == 1, BCMI_INVALID_VALUE, "transform: element width %d unsupported", width);
+    BCMI_CHECK(a->fn != BCMI_MAP_LENGTH, BCMI_INVALID_VALUE, "transform: length needs vector elements");
+    switch (dtype) {
+    case BCMI_I32: return transform_iter_typed<int32_t>(q, op, a, b, out, n);
+    case BCMI_U32: return transform_iter_typed<uint32_t>(q, op, a, b, out, n);
+    case BCMI_I64: return transform_iter_typed<int64_t>(q, op, a, b, out, n);
+    case BCMI_U64: return transform_iter_typed<uint64_t>(q, op, a, b, out, n);
+    case BCMI_F32: return transform_iter_typed<float>(q, op, a, b, out, n);
+    case BCMI_F64: return transform_iter_typed<double>(q, op, a, b, out, n);
+    default: set_error("transform: iterator inputs of dtype %d unsupported (32/64-bit int, float, double)", dtype);
+             return BCMI_INVALID_VALUE;
+    }
+}

@@ -341,6 +341,52 @@ int bcmi_transform_reduce(bcmi_queue queue, int dtype, const bcmi_seq *a,
                           int reduce_op, const void *init, void *out,
                           double *out_f64, int precision);
 
+/* ------------------------------------------------- fancy-iterator inputs */
+/* An input range as the reference's iterators present it:
+ *   BCMI_ITER_BUFFER    element i = ptr[i]           (buffer_iterator)
+ *   BCMI_ITER_COUNTING  element i = value + i        (counting_iterator,
+ *                                                    iterator/counting_iterator.hpp)
+ *   BCMI_ITER_CONSTANT  element i = value            (constant_iterator,
+ *                                                    iterator/constant_iterator.hpp)
+ * then y = g(element - offset) with g = fn: identity, abs or square
+ * (transform_iterator over _1 - c, _1 + c (offset -c), abs<T>, _1 * _1,
+ * pow(_1, 2), pown(_1, 2), pow(_1 - c, 2); iterator/transform_iterator.hpp),
+ * or, for fn = BCMI_MAP_CMP + a bcmi_cmp code, y = (element cmp offset) ?
+ * 1 : 0 (_1 > c and the other comparisons, which OpenCL C evaluates to an
+ * int).  value and offset hold one element of the range's type in their
+ * low bytes; arithmetic is in that type (integers wrap). */
+enum bcmi_iter_kind { BCMI_ITER_BUFFER = 0, BCMI_ITER_COUNTING = 1, BCMI_ITER_CONSTANT = 2 };
+typedef struct bcmi_iter {
+    const void *ptr;
+    int32_t kind;
+    int32_t fn;
+    uint64_t value;
+    uint64_t offset;
+} bcmi_iter;
+
+/* exclusive_scan / inclusive_scan over an iterator range
+ * (test/test_scan.cpp:140-286).  width 1: dtype is I32, U32, I64, U64, F32
+ * or F64 with any kind and map (a plain array goes to bcmi_scan).  width 2
+ * or 4: OpenCL vector elements (int2_, uint4_, float4_ ...) of dtype I32,
+ * U32 or F32 from an array, op BCMI_PLUS, added component by component;
+ * init then points at one vector element. */
+int bcmi_scan_iter(bcmi_queue queue, int dtype, int width, int op,
+                   const bcmi_iter *in, void *out, size_t n,
+                   const void *init, int exclusive, int precision);
+
+/* transform over iterator ranges (transform.hpp:38-82 reading
+ * transform_iterator / counting_iterator / constant_iterator /
+ * zip_iterator inputs): out[i] = a_i op b_i (op: plus, minus, multiplies,
+ * divides, min, max), or out[i] = a_i when b is NULL (copy of a generated
+ * or mapped range).  width 2 or 4 with dtype F32, b NULL and a->fn ==
+ * BCMI_MAP_LENGTH: out[i] = length(a[i]), the Euclidean length of vector
+ * element i (length<float4_>, functional/geometry.hpp). */
+#define BCMI_MAP_LENGTH 3
+#define BCMI_MAP_CMP 8
+int bcmi_transform_iter(bcmi_queue queue, int dtype, int width, int op,
+                        const bcmi_iter *a, const bcmi_iter *b, void *out,
+                        size_t n);
+
 /* The key extraction of experimental::sort_by_transform
  * (experimental/sort_by_transform.hpp): keys[i] = f(field of element i),
  * where element i is record_bytes bytes at records + i * record_bytes, the

@@ -11,6 +11,7 @@
 #include <vector>
 
 #include <boost/compute/async/future.hpp>
+#include <boost/compute/detail/iter_desc.hpp>
 #include <boost/compute/event.hpp>
 #include <boost/compute/system.hpp>
 #include <boost/compute/type_traits/is_device_iterator.hpp>
@@ -49,6 +50,20 @@ inline buffer_iterator<T> copy_dispatch(buffer_iterator<T> first, buffer_iterato
     const size_t n = (size_t)(last - first);
     if (n)
         bcmi_check(bcmi_memcpy(queue.get(), result.device_ptr(), first.device_ptr(), n * sizeof(T), BCMI_D2D, 0));
+    return result + (std::ptrdiff_t)n;
+}
+
+// counting / constant / transform iterator into a device range: the
+// sequence generated (and mapped) on the device (test/test_constant_iterator.cpp:60-91)
+template <class InputIterator, class T>
+inline buffer_iterator<T> copy_dispatch(InputIterator first, InputIterator last, buffer_iterator<T> result,
+                                        command_queue &queue, std::true_type, std::true_type)
+{
+    static_assert(std::is_same<typename iter_desc<InputIterator>::value_type, T>::value,
+                  "copy: a generated range is copied into elements of its own type");
+    const size_t n = (size_t)(last - first);
+    const bcmi_iter d = iter_desc<InputIterator>::get(first);
+    if (n) bcmi_check(bcmi_transform_iter(queue.get(), bcmi_dtype<T>(), 1, BCMI_PLUS, &d, 0, result.device_ptr(), n));
     return result + (std::ptrdiff_t)n;
 }
 

@@ -5,8 +5,10 @@
 #ifndef BOOST_COMPUTE_ALGORITHM_DETAIL_SCAN_HPP
 #define BOOST_COMPUTE_ALGORITHM_DETAIL_SCAN_HPP
 
+#include <boost/compute/detail/iter_desc.hpp>
 #include <boost/compute/functional/operator.hpp>
 #include <boost/compute/system.hpp>
+#include <boost/compute/types/fundamental.hpp>
 #include <boost/compute/type_traits/is_device_iterator.hpp>
 
 namespace boost {
@@ -30,6 +32,56 @@ inline buffer_iterator<OutT> scan(buffer_iterator<InT> first, buffer_iterator<In
     bcmi_check(bcmi_scan(queue.get(), bcmi_dtype<OutT>(), BinaryOperator::bcmi_op, first.device_ptr(),
                          result.device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
                          BCMI_PRECISION_COMPENSATED));
+    return result + (std::ptrdiff_t)n;
+}
+
+template <class BinaryOperator>
+inline void scan_op_check()
+{
+    static_assert(BinaryOperator::bcmi_op == BCMI_PLUS || BinaryOperator::bcmi_op == BCMI_MULTIPLIES ||
+                      BinaryOperator::bcmi_op == BCMI_MIN || BinaryOperator::bcmi_op == BCMI_MAX,
+                  "scan: the operator must be plus, multiplies, min or max");
+}
+
+// A counting_iterator, constant_iterator or transform_iterator input: the
+// kernel generates or maps the elements as it reads them
+// (test/test_scan.cpp:190-286).
+template <class InputIterator, class OutT, class T, class BinaryOperator>
+inline buffer_iterator<OutT> scan(InputIterator first, InputIterator last, buffer_iterator<OutT> result,
+                                  bool exclusive, T init, BinaryOperator, command_queue &queue)
+{
+    static_assert(std::is_same<typename iter_desc<InputIterator>::value_type, OutT>::value,
+                  "scan: input and output element types must match");
+    scan_op_check<BinaryOperator>();
+    const size_t n = (size_t)(last - first);
+    const OutT ini = (OutT)init;
+    const bcmi_iter d = iter_desc<InputIterator>::get(first);
+    bcmi_check(bcmi_scan_iter(queue.get(), bcmi_dtype<OutT>(), 1, BinaryOperator::bcmi_op, &d, result.device_ptr(),
+                              n, exclusive ? &ini : 0, exclusive ? 1 : 0, BCMI_PRECISION_COMPENSATED));
+    return result + (std::ptrdiff_t)n;
+}
+
+// OpenCL vector elements (int2_, float4_ ...): plus, component by
+// component (test/test_scan.cpp:140-180).
+template <class S, std::size_t N, class T, class BinaryOperator>
+inline buffer_iterator<vector_type<S, N> > scan(buffer_iterator<vector_type<S, N> > first,
+                                                buffer_iterator<vector_type<S, N> > last,
+                                                buffer_iterator<vector_type<S, N> > result, bool exclusive, T init,
+                                                BinaryOperator, command_queue &queue)
+{
+    static_assert(N == 2 || N == 4, "scan: vector elements of 2 or 4 components");
+    static_assert(BinaryOperator::bcmi_op == BCMI_PLUS, "scan: vector elements take plus");
+    typedef vector_type<S, N> V;
+    const size_t n = (size_t)(last - first);
+    const V ini = V(init);
+    bcmi_iter d;
+    d.ptr = first.device_ptr();
+    d.kind = BCMI_ITER_BUFFER;
+    d.fn = BCMI_MAP_ID;
+    d.value = 0;
+    d.offset = 0;
+    bcmi_check(bcmi_scan_iter(queue.get(), bcmi_dtype<S>(), (int)N, BCMI_PLUS, &d, result.device_ptr(), n,
+                              exclusive ? &ini : 0, exclusive ? 1 : 0, BCMI_PRECISION_COMPENSATED));
     return result + (std::ptrdiff_t)n;
 }
 

@@ -9,6 +9,7 @@
 #define BOOST_COMPUTE_ALGORITHM_REDUCE_HPP
 
 #include <boost/compute/container/vector.hpp>
+#include <boost/compute/detail/materialize.hpp>
 #include <boost/compute/functional/operator.hpp>
 
 namespace boost {
@@ -37,6 +38,26 @@ inline void dispatch_reduce(buffer_iterator<T> first, buffer_iterator<T> last, T
     ::boost::compute::vector<T> tmp(1, queue.get_context());
     bcmi_reduce_to(first, last, tmp.begin().device_ptr(), (const T *)0, f, queue);
     *result = tmp.begin().read(queue);
+}
+
+// counting / constant / transform iterator input: written out first
+template <class InputIterator, class OutputIterator, class BinaryFunction>
+inline void dispatch_reduce(InputIterator first, InputIterator last, OutputIterator result, BinaryFunction f,
+                            command_queue &queue)
+{
+    const ::boost::compute::vector<typename iter_desc<InputIterator>::value_type> tmp =
+        materialize(first, (size_t)(last - first), queue);
+    dispatch_reduce(tmp.begin(), tmp.end(), result, f, queue);
+    queue.finish();  // tmp is released on return
+}
+
+template <class InputIterator, class T, class BinaryFunction>
+inline void bcmi_reduce_to(InputIterator first, InputIterator last, T *out_device, const T *init, BinaryFunction f,
+                           command_queue &queue)
+{
+    const ::boost::compute::vector<T> tmp = materialize(first, (size_t)(last - first), queue);
+    bcmi_reduce_to(tmp.begin(), tmp.end(), out_device, init, f, queue);
+    queue.finish();
 }
 
 }  // namespace detail

@@ -9,7 +9,10 @@
 
 #include <iterator>
 
+#include <boost/compute/algorithm/reduce.hpp>
+#include <boost/compute/algorithm/transform.hpp>
 #include <boost/compute/detail/mapreduce.hpp>
+#include <boost/compute/functional/geometry.hpp>
 #include <boost/compute/system.hpp>
 #include <boost/compute/type_traits/is_device_iterator.hpp>
 
@@ -45,6 +48,21 @@ inline void transform_reduce(InputIterator first, InputIterator last, OutputIter
     const bcmi_map m = detail::unary_map_of<T, UnaryTransformFunction>::get(transform_function);
     detail::store_result<T>(detail::seq_of(first), (const bcmi_seq *)0, n, m, BinaryReduceFunction::bcmi_op, result,
                             queue);
+}
+
+// length<floatN_>: the lengths written out (one launch), then reduced
+// (test/test_transform_reduce.cpp:42-60)
+template <class S, std::size_t N, class OutputIterator, class BinaryReduceFunction>
+inline void transform_reduce(buffer_iterator<vector_type<S, N> > first, buffer_iterator<vector_type<S, N> > last,
+                             OutputIterator result, length<vector_type<S, N> > f, BinaryReduceFunction op,
+                             command_queue &queue = system::default_queue())
+{
+    const size_t n = (size_t)(last - first);
+    if (n == 0) return;
+    ::boost::compute::vector<S> lengths(n, queue.get_context());
+    ::boost::compute::transform(first, last, lengths.begin(), f, queue);
+    ::boost::compute::reduce(lengths.begin(), lengths.end(), result, op, queue);
+    queue.finish();  // lengths is released on return
 }
 
 template <class InputIterator1, class InputIterator2, class OutputIterator, class BinaryTransformFunction,

@@ -1,5 +1,6 @@
 #ifndef BOOST_COMPUTE_FUNCTIONAL_HPP
 #define BOOST_COMPUTE_FUNCTIONAL_HPP
+#include <boost/compute/functional/geometry.hpp>
 #include <boost/compute/functional/get.hpp>
 #include <boost/compute/functional/integer.hpp>
 #include <boost/compute/functional/logical.hpp>

@@ -5,5 +5,8 @@
 #ifndef BOOST_COMPUTE_ITERATOR_HPP
 #define BOOST_COMPUTE_ITERATOR_HPP
 #include <boost/compute/iterator/buffer_iterator.hpp>
+#include <boost/compute/iterator/constant_iterator.hpp>
 #include <boost/compute/iterator/counting_iterator.hpp>
+#include <boost/compute/iterator/transform_iterator.hpp>
+#include <boost/compute/iterator/zip_iterator.hpp>
 #endif

@@ -11,7 +11,9 @@
 //                                     a * _1, _1 % a, _1 & a and cmp one of
 //                                     < <= > >= == !=
 //   pow(_1, 2), pow(_1 - c, 2)        element maps of transform_reduce
-//                                     (lambda/functional.hpp:521)
+//                                     (lambda/functional.hpp:521) and of
+//   _1 + c, _1 - c, _1 * _1,          transform_iterator (scan, reduce,
+//   pown(_1, 2), _1 cmp c             transform, copy)
 //   _1 < _2, get<N>(_1) < get<M>(_2), abs(_1) < abs(_2), length(_1) <
 //   length(_2), abs(get<N>(_1) + get<M>(_1)) < ... (and >)
 //                                     comparators of the comparison sort
@@ -111,6 +113,30 @@ inline square_expr<T> pow(minus_arg1<T> m, int e)
 inline square_expr<int> pow(placeholder<1>, int e)
 {
     if (e != 2) throw opencl_error(BCMI_INVALID_VALUE, "pow: only the exponent 2 is supported");
+    return square_expr<int>{0};
+}
+
+// _1 + c, _1 * _1 and pown(_1, 2): more element maps of transform_iterator
+// (test/test_reduce.cpp:216-234, test/test_scan.cpp:273-276)
+template <class T>
+struct plus_arg1 {
+    T c;
+};
+
+template <class T, class = typename std::enable_if<std::is_arithmetic<T>::value>::type>
+inline plus_arg1<T> operator+(placeholder<1>, T c)
+{
+    return plus_arg1<T>{c};
+}
+
+inline square_expr<int> operator*(placeholder<1>, placeholder<1>)
+{
+    return square_expr<int>{0};
+}
+
+inline square_expr<int> pown(placeholder<1>, int e)
+{
+    if (e != 2) throw opencl_error(BCMI_INVALID_VALUE, "pown: only the exponent 2 is supported");
     return square_expr<int>{0};
 }
 

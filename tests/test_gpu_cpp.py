@@ -19,3 +19,20 @@ def test_cpp_shim_reference_cases():
     print(r.stdout)
     assert r.returncode == 0, r.stdout + r.stderr
     assert "0 failures" in r.stdout
+
+
+ITER_EXE = os.path.join(os.path.dirname(os.path.abspath(__file__)), "cpp", "build", "test_iterators")
+
+
+@pytest.mark.gpu
+def test_cpp_fancy_iterators_reference_cases():
+    """tests/cpp/test_iterators.cpp: counting / constant / transform / zip
+    iterators and int2_ / float4_ elements through the templates, on the
+    reference's test vectors (test_scan.cpp, test_transform_reduce.cpp,
+    test_reduce.cpp, test_zip_iterator.cpp, test_constant_iterator.cpp)."""
+    if not os.path.exists(ITER_EXE):
+        pytest.fail("tests/cpp/build/test_iterators missing: run __graft_entry__.build() first")
+    r = subprocess.run([ITER_EXE], capture_output=True, text=True, timeout=120)
+    print(r.stdout)
+    assert r.returncode == 0, r.stdout + r.stderr
+    assert "0 failures" in r.stdout
