@@ -22,29 +22,12 @@
 // A plan owns its tables and temporaries, so running it allocates nothing
 // and can be captured in a hipGraph and replayed.
 #include "radix.hpp"
+#include "segplan.hpp"
 
 #include <cstring>
 #include <vector>
 
 namespace bcmi {
-
-constexpr int kSegSmallThreads = 256;
-constexpr int kSegSmallLds = 48 * 1024;  // key + value bytes a small-segment block keeps in LDS
-constexpr uint32_t kSegChunk = 1u << 16;  // keys per histogram chunk of a large segment
-
-struct SegSpan {
-    uint64_t base;
-    uint32_t len;
-    uint32_t idx;  // large-segment index (tiles, chunks) or unused
-};
-
-struct SegTile {
-    uint64_t base;
-    uint32_t len;
-    uint32_t tile;  // tile index within the segment
-    uint32_t seg;   // large-segment index
-    uint32_t pad;
-};
 
 // ------------------------------------------------------------ small path
 constexpr uint32_t kBucketMax = 64;  // largest bucket the one-pass sort finishes by insertion
@@ -624,45 +607,17 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
                BCMI_INVALID_VALUE, "segmented plan: value size %zu unsupported", value_bytes);
     const int kb = dtype_size(key_dtype);
     const bool flt = key_dtype == BCMI_F32 || key_dtype == BCMI_F64;
-    const size_t eb = (size_t)kb + value_bytes;
-    // the LDS-resident path holds (key + value) bytes x capacity in LDS
-    int cmax = 4;
-    while (cmax > 0 && eb * (size_t)kSegSmallThreads * ((size_t)1 << cmax) > (size_t)kSegSmallLds) --cmax;
-    const uint64_t small_max = (uint64_t)kSegSmallThreads << cmax;
-    const int ipt = large_ipt(kb, value_bytes);
-    const uint64_t tile = (uint64_t)kSortThreads * ipt;
-    std::vector<SegSpan> small[5], insert, large, chunks;
-    std::vector<uint32_t> chunk_begin(1, 0), slot_begin(1, 0);
-    std::vector<SegTile> tiles;
-    uint64_t total = nseg ? host_offsets[nseg] : 0;
-    for (size_t s = 0; s < nseg; ++s) {
-        BCMI_CHECK(host_offsets[s + 1] >= host_offsets[s], BCMI_INVALID_VALUE,
-                   "segmented plan: offsets decrease at segment %zu", s);
-        const uint64_t base = host_offsets[s], len = host_offsets[s + 1] - host_offsets[s];
-        BCMI_CHECK(len <= (1ull << 32) - (1ull << 20), BCMI_INVALID_BUFFER_SIZE,
-                   "segmented plan: segment %zu has %llu elements", s, (unsigned long long)len);
-        if (len < 2) continue;
-        if (flt && len < 32) {
-            insert.push_back(SegSpan{base, (uint32_t)len, 0});
-        } else if (len <= small_max) {
-            int c = 0;
-            while (((uint64_t)kSegSmallThreads << c) < len) ++c;
-            small[c].push_back(SegSpan{base, (uint32_t)len, 0});
-        } else {
-            const uint32_t li = (uint32_t)large.size();
-            large.push_back(SegSpan{base, (uint32_t)len, li});
-            for (uint64_t t = 0; t * tile < len; ++t) tiles.push_back(SegTile{base, (uint32_t)len, (uint32_t)t, li, 0});
-            for (uint64_t c = 0; c < len; c += kSegChunk) {
-                uint64_t cl = len - c < kSegChunk ? len - c : kSegChunk;
-                chunks.push_back(SegSpan{base + c, (uint32_t)cl, li});
-            }
-            chunk_begin.push_back((uint32_t)chunks.size());
-            // greedy groups of <= small_max keys: any two consecutive ones
-            // hold more than small_max, so at most 2*len/small_max + 1
-            const uint64_t gmax = std::min<uint64_t>(256, 2 * (len / small_max) + 2);
-            slot_begin.push_back(slot_begin.back() + (uint32_t)gmax);
-        }
-    }
+    const uint64_t tile = (uint64_t)kSortThreads * large_ipt(kb, value_bytes);
+    SegTables tb;
+    std::string err;
+    const int st0 = build_seg_tables(host_offsets, nseg, (size_t)kb + value_bytes, flt, tile, tb, err);
+    BCMI_CHECK(st0 == 0, st0, "%s", err.c_str());
+    const int cmax = tb.small_cls;
+    std::vector<SegSpan>(&small)[5] = tb.small;
+    std::vector<SegSpan> &insert = tb.insert, &large = tb.large, &chunks = tb.chunks;
+    std::vector<uint32_t> &chunk_begin = tb.chunk_begin, &slot_begin = tb.slot_begin;
+    std::vector<SegTile> &tiles = tb.tiles;
+    const uint64_t total = tb.total;
     bcmi_segplan pl = new bcmi_segplan_s();
     pl->device = q->device;
     pl->key_dtype = key_dtype;
