@@ -9,10 +9,11 @@ is checked, the fastest correct candidate is kept, and the result is written
 with bcmi_tuning_cache_save.  libbcmi.so loads the file at its first queue:
 the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
 
-Tunables (the digit width is fixed at 8 bits and the block size at 256
-threads by the kernels' LDS layout; what varies is the work per thread):
-  __bcmi_radix_sort|ipt, |rank         keys-only 4-byte radix sort
-  __bcmi_radix_sort_by_key|ipt, |rank  4-byte keys with 4-byte values
+Tunables (the digit width is fixed at 8 bits by the per-digit LDS words;
+what varies is the tile shape and the work per thread):
+  __bcmi_radix_sort|threads, |ipt       keys-only 4-byte radix sort: workgroup size, keys per thread
+  __bcmi_radix_sort_by_key|threads, |ipt  4-byte keys with 4-byte values
+  __bcmi_radix_hist|unroll, |subs, |blocks_per_cu  the sort's histogram kernel
   __bcmi_scan|q                        4-byte scans, items per lane
   __bcmi_reduce_by_key|q               reduce_by_key, items per lane
   __bcmi_segscan|q                     segmented scans, items per lane
@@ -30,9 +31,8 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
-SORT_CANDIDATES = [(0, 16), (0, 24), (0, 32), (1, 24)]  # (rank, ipt)
-SORT_KEYS_ONLY_EXTRA = [(0, 40), (0, 48), (0, 64)]
-SORT_PAIRS_EXTRA = [(0, 40), (0, 48), (0, 32, 1), (0, 40, 1), (0, 48, 1)]  # (rank, ipt, last_ballot)
+SORT_SHAPES = [(256, 32), (256, 64), (512, 16), (512, 24), (512, 32), (1024, 16)]  # (threads, ipt)
+HIST_SHAPES = [(u, nw, b) for u in (2, 4, 8) for nw in (8, 16) for b in (2, 4)]  # (unroll, subs, blocks/CU)
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 
@@ -64,18 +64,44 @@ def tune_sort(ca, W, q, log2n, log):
     work = ca.DeviceArray(n, np.uint32, q)
     restore = lambda: ca.check(L.bcmi_memcpy(q.h, work.ptr, src.ptr, src.nbytes, ca.D2D, 0))
     best = None
-    for rank, ipt in SORT_CANDIDATES + SORT_KEYS_ONLY_EXTRA:
-        _set(ca, "__bcmi_radix_sort", "rank", rank)
+    for th, ipt in SORT_SHAPES:
+        _set(ca, "__bcmi_radix_sort", "threads", th)
         _set(ca, "__bcmi_radix_sort", "ipt", ipt)
         ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
         ok = bool(np.array_equal(work.to_host(q), expect))
-        log({"tunable": "__bcmi_radix_sort", "rank": rank, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_radix_sort", "threads": th, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, rank, ipt)
+            best = (ms, th, ipt)
     if best is None:
         raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
-    _set(ca, "__bcmi_radix_sort", "rank", best[1])
+    _set(ca, "__bcmi_radix_sort", "threads", best[1])
     _set(ca, "__bcmi_radix_sort", "ipt", best[2])
+
+    # the histogram kernel, timed inside the tuned sort (queue profiler)
+    best = None
+    for u, nw, b in HIST_SHAPES:
+        _set(ca, "__bcmi_radix_hist", "unroll", u)
+        _set(ca, "__bcmi_radix_hist", "subs", nw)
+        _set(ca, "__bcmi_radix_hist", "blocks_per_cu", b)
+        q.profile_reset()
+        q.profile(True)
+        for _ in range(5):
+            restore()
+            ca.sort(q, work)
+        q.profile(False)
+        cnt, tot = q.profile_query("radix_histogram")
+        q.profile_reset()
+        ms = tot / max(cnt, 1)
+        ok = bool(np.array_equal(work.to_host(q), expect))
+        log({"tunable": "__bcmi_radix_hist", "unroll": u, "subs": nw, "blocks_per_cu": b, "ms": round(ms, 4),
+             "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, u, nw, b)
+    if best is None:
+        raise RuntimeError("autotune: no histogram shape produced the sorted output")
+    _set(ca, "__bcmi_radix_hist", "unroll", best[1])
+    _set(ca, "__bcmi_radix_hist", "subs", best[2])
+    _set(ca, "__bcmi_radix_hist", "blocks_per_cu", best[3])
 
     k, v = W.sort_by_key_f32(n)
     sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
@@ -87,12 +113,9 @@ def tune_sort(ca, W, q, log2n, log):
 
     first = None
     best = None
-    for cand in SORT_CANDIDATES + SORT_PAIRS_EXTRA:
-        rank, ipt = cand[:2]
-        lb = cand[2] if len(cand) > 2 else 0
-        _set(ca, "__bcmi_radix_sort_by_key", "rank", rank)
+    for th, ipt in SORT_SHAPES:
+        _set(ca, "__bcmi_radix_sort_by_key", "threads", th)
         _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
-        _set(ca, "__bcmi_radix_sort_by_key", "last_ballot", lb)
         ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)
         got = (wk.to_host(q).view(np.uint32), wv.to_host(q))
         if first is None:
@@ -103,15 +126,13 @@ def tune_sort(ca, W, q, log2n, log):
             first = got
         else:
             ok = bool(np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]))
-        log({"tunable": "__bcmi_radix_sort_by_key", "rank": rank, "ipt": ipt, "last_ballot": lb, "ms": round(ms, 4),
-             "ok": ok})
+        log({"tunable": "__bcmi_radix_sort_by_key", "threads": th, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, rank, ipt, lb)
+            best = (ms, th, ipt)
     if best is None:
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
-    _set(ca, "__bcmi_radix_sort_by_key", "rank", best[1])
+    _set(ca, "__bcmi_radix_sort_by_key", "threads", best[1])
     _set(ca, "__bcmi_radix_sort_by_key", "ipt", best[2])
-    _set(ca, "__bcmi_radix_sort_by_key", "last_ballot", best[3])
 
 
 def tune_scan(ca, W, q, log2n, log):

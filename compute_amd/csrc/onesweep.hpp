@@ -1,0 +1,294 @@
+// onesweep.hpp -- one 8-bit digit pass of the LSD radix sort, ranked by
+// quarter-waves.
+//
+// A workgroup of THREADS threads owns tile `blockIdx.x` of THREADS * IPT
+// keys.  Each 16-lane quarter-wave ("sub") owns a contiguous run of 16 * IPT
+// positions of the tile, loaded quarter-striped: slot j of lane l holds
+// position sub * 16 * IPT + j * 16 + (l & 15), so every load instruction
+// reads four 64-byte segments and ranking slot by slot, lane by lane, walks
+// the sub's keys in input order (stability).
+//
+// Per sub and digit, one 32-bit LDS word [position:16 | peer mask:16]:
+//   1. early counts: every key adds 1 << 16 to its (sub, digit) word;
+//   2. a scan in digit-major, sub-minor order turns the counts into the tile
+//      positions where each (sub, digit) run starts, and gives the tile's
+//      per-digit totals, which are published for the look-back right away;
+//   3. ranking, per slot: each lane ORs its lane bit into the low half of its
+//      word, reads the word back (peers = same-digit lanes of the slot),
+//      takes position = start + peers below it, and every peer writes the
+//      same advanced start with the mask cleared.  LDS executes one wave's
+//      instructions in order, so slot j + 1 sees slot j's update.
+// The look-back (its first reads issued before ranking) then fetches the
+// global offset of each digit, the tile is
+// reordered in LDS by position (digit runs contiguous) and written out
+// coalesced: keys, then values.  Three LDS round trips per key to rank, one
+// atomic add to count, one write and two reads to reorder.
+//
+// LDS: the count words alias the staging buffer (ranking finishes before
+// the first staging write), so a tile needs max(TILE * key bytes,
+// THREADS * 64) bytes plus 2-3 KB.
+#pragma once
+
+#include "radix.hpp"
+
+namespace bcmi {
+
+template <int THREADS, int IPT, typename KB, int VB>
+struct QSweepLds {
+    static constexpr int NSUB = THREADS / 16;
+    static constexpr int TILE = THREADS * IPT;
+    static constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+    static constexpr int WORDS = NSUB * 256 * 4;
+    static constexpr int STAGE = TILE * SB > WORDS ? TILE * SB : WORDS;
+    alignas(16) unsigned char stage[STAGE];
+    uint32_t goff[256];
+    uint32_t part[THREADS / 256][256];
+    uint32_t wsum[4];
+};
+
+// Tile positions (below 2^16), two per register.
+template <int IPT>
+struct Pos16 {
+    uint32_t w[(IPT + 1) / 2];
+    BDEV void set(int j, uint32_t v)
+    {
+        if (j & 1) {
+            w[j >> 1] |= v << 16;
+            // materialise the packed pair here (see Ranks in radix.hpp)
+            asm volatile("" : "+v"(w[j >> 1]));
+        } else {
+            w[j >> 1] = v;
+        }
+    }
+    BDEV uint32_t operator[](int j) const { return (j & 1) ? (w[j >> 1] >> 16) : (w[j >> 1] & 0xFFFFu); }
+};
+
+// Digits of up to 4 keys per register (the value pass reuses the key pass's
+// digits).
+template <int IPT>
+struct Dig8 {
+    uint32_t w[(IPT + 3) / 4];
+    BDEV void set(int j, uint32_t d)
+    {
+        if ((j & 3) == 0) w[j >> 2] = d;
+        else w[j >> 2] |= d << (8 * (j & 3));
+    }
+    BDEV uint32_t operator[](int j) const { return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
+};
+
+// Look-back for digit d of tile `tile` (tile > 0).  The first kQLookback
+// predecessor words are read early (before the tile ranks its keys, so the
+// round trip overlaps the ranking) and handed in as wd; then the
+// predecessors' aggregates are folded in tile order until an inclusive
+// prefix, kQLookback words per round trip; at a word not yet published only
+// that word is polled.  A word read early may be older than the
+// predecessor's latest state, which is harmless: an aggregate instead of an
+// inclusive prefix only sends the walk further back.  Returns the exclusive
+// prefix (a bounded wait: after kSpinLimit polls it records a fault and
+// returns what it has).
+constexpr int kQLookback = 4;
+
+BDEV void digit_lookback_issue(const uint64_t *status, uint32_t tile, uint32_t d, uint32_t epoch,
+                               uint64_t (&wd)[kQLookback])
+{
+#pragma unroll
+    for (int i = 0; i < kQLookback; ++i) {
+        const int64_t t = (int64_t)tile - 1 - i;
+        wd[i] = t >= 0 ? lb_load(&status[(size_t)t * 256 + d]) : lb_word(epoch, LB_P, 0);
+    }
+}
+
+BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, uint32_t epoch, Ctrl *ctrl,
+                             uint64_t (&wd)[kQLookback])
+{
+    uint32_t excl = 0, spins = 0;
+    int64_t kt = (int64_t)tile - 1;
+    while (true) {
+        int used = 0;
+        bool found = false, stalled = false;
+#pragma unroll
+        for (int i = 0; i < kQLookback; ++i) {
+            if (found || stalled) continue;
+            if (lb_epoch(wd[i]) != epoch) {
+                stalled = true;
+                continue;
+            }
+            excl += lb_payload(wd[i]);
+            ++used;
+            if (lb_flag(wd[i]) == LB_P) found = true;
+        }
+        if (found) return excl;
+        kt -= used;
+        if (stalled && used == 0) {
+            while (lb_epoch(lb_load(&status[(size_t)kt * 256 + d])) != epoch)
+                if (spin_expired(ctrl, spins)) return excl;
+        }
+        digit_lookback_issue(status, (uint32_t)(kt + 1), d, epoch, wd);
+    }
+}
+
+template <int THREADS, int IPT, typename KB, int VB, typename XF, bool FULL>
+BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
+                      const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
+                      uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf, Ctrl *ctrl,
+                      uint64_t *__restrict__ status, uint32_t back)
+{
+    using V = typename ValT<VB>::type;
+    constexpr int NSUB = THREADS / 16, TILE = THREADS * IPT, PARTS = THREADS / 256, SPP = NSUB / PARTS;
+    static_assert(THREADS % 256 == 0 && TILE <= 32768, "256-multiple blocks, tile positions in 15 bits");
+    uint32_t *words = (uint32_t *)L.stage;
+    KB *s_keys = (KB *)L.stage;
+    V *s_vals = (V *)L.stage;
+    const uint32_t tid = threadIdx.x, ql = tid & 15, sub = tid >> 4;
+    // zero the count words: 16 per thread, four 16-byte stores
+#pragma unroll
+    for (int i = 0; i < 4; ++i) ((uint4 *)words)[i * THREADS + tid] = make_uint4(0, 0, 0, 0);
+    const uint32_t base = tile * (uint32_t)TILE;
+    const int32_t lim = FULL ? TILE : (int32_t)(n - base) - (int32_t)(sub * 16 * IPT);
+    auto valid = [&](int j) { return FULL || (int32_t)(j * 16 + ql) < lim; };
+    const KB *kq = kin + ((size_t)base + sub * 16 * IPT + ql);
+    KB k[IPT];
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) k[j] = valid(j) ? kq[j * 16] : KB(0);
+    V v[VB ? IPT : 1];
+    if constexpr (VB > 0) {
+        const V *vq = vin + ((size_t)base + sub * 16 * IPT + ql);
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (valid(j)) v[j] = vq[j * 16];
+    }
+    block_barrier();
+    // 1. early counts
+    uint32_t *my = words + sub * 256;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        if (valid(j))
+            __hip_atomic_fetch_add(&my[xf.digit(k[j], shift)], 1u << 16, __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_WORKGROUP);
+    block_barrier();
+    // 2. thread (part, d) scans digit d over subs [part * SPP, part * SPP + SPP)
+    const uint32_t d = tid & 255, part = tid >> 8;
+    uint32_t c[SPP];
+    uint32_t psum = 0;
+#pragma unroll
+    for (int i = 0; i < SPP; ++i) {
+        c[i] = words[(part * SPP + i) * 256 + d] >> 16;
+        psum += c[i];
+    }
+    uint32_t total = psum, before = 0;
+    if constexpr (PARTS > 1) {
+        L.part[part][d] = psum;
+        block_barrier();
+        total = 0;
+#pragma unroll
+        for (int p = 0; p < PARTS; ++p) {
+            const uint32_t x = L.part[p][d];
+            if (p < (int)part) before += x;
+            total += x;
+        }
+    }
+    const uint32_t lane = tid & 63, w = tid >> 6;
+    const uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
+    if (lane == 63 && w < 4) L.wsum[w] = inc;
+    uint32_t epoch = 0;
+    if (part == 0) {
+        epoch = launch_epoch_after(ctrl, back, total);
+        if (tile == 0) lb_store(&status[d], lb_word(epoch, LB_P, total));
+        else lb_store(&status[(size_t)tile * 256 + d], lb_word(epoch, LB_A, total));
+    }
+    block_barrier();
+    uint32_t dstart = inc - total;
+    for (uint32_t i = 0; i < (w & 3); ++i) dstart += L.wsum[i];
+    {
+        uint32_t s0 = dstart + before;
+#pragma unroll
+        for (int i = 0; i < SPP; ++i) {
+            words[(part * SPP + i) * 256 + d] = s0 << 16;
+            s0 += c[i];
+        }
+    }
+    block_barrier();
+    // the look-back's first reads, in flight while the tile ranks
+    uint64_t lbw[kQLookback];
+    if (part == 0 && tile > 0) digit_lookback_issue(status, tile, d, epoch, lbw);
+    // 3. rank
+    Pos16<IPT> pos;
+    const uint32_t qbit = 1u << ql, qlt = qbit - 1u;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const bool ok = valid(j);
+        uint32_t *a = &my[xf.digit(k[j], shift)];
+        if (ok) __hip_atomic_fetch_or(a, qbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t wd = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        const uint32_t run = wd >> 16;
+        pos.set(j, run + __builtin_popcount(wd & qlt));
+        if (ok)
+            __hip_atomic_store(a, (run + __builtin_popcount(wd & 0xFFFFu)) << 16, __ATOMIC_RELAXED,
+                               __HIP_MEMORY_SCOPE_WORKGROUP);
+    }
+    // global offsets: digit d's keys go to [goff0 + excl, goff0 + excl + total),
+    // checked once per digit; a tile with an inconsistent offset (a corrupted
+    // chain) writes nothing and reports
+    bool bad = false;
+    if (part == 0) {
+        uint32_t excl = 0;
+        if (tile > 0) {
+            excl = digit_lookback(status, tile, d, epoch, ctrl, lbw);
+            lb_store(&status[(size_t)tile * 256 + d], lb_word(epoch, LB_P, excl + total));
+        }
+        L.goff[d] = goff0[d] + excl - dstart;
+        bad = (uint64_t)goff0[d] + excl + total > n;
+    }
+    if (block_barrier_or(bad)) {
+        if (tid == 0) __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        return;
+    }
+    // 4. reorder by position, write out coalesced
+#pragma unroll
+    for (int j = 0; j < IPT; ++j)
+        if (valid(j)) s_keys[pos[j]] = k[j];
+    block_barrier();
+    const uint32_t tile_n = FULL ? (uint32_t)TILE : n - base;
+    Dig8<(VB > 0 ? IPT : 1)> od;
+#pragma unroll
+    for (int j = 0; j < IPT; ++j) {
+        const uint32_t i = j * THREADS + tid;
+        if (FULL || i < tile_n) {
+            const KB key = s_keys[i];
+            const uint32_t dg = xf.digit(key, shift);
+            if constexpr (VB > 0) od.set(j, dg);
+            kout[L.goff[dg] + i] = key;
+        }
+    }
+    if constexpr (VB > 0) {
+        block_barrier();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j)
+            if (valid(j)) s_vals[pos[j]] = v[j];
+        block_barrier();
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t i = j * THREADS + tid;
+            if (FULL || i < tile_n) vout[L.goff[od[j]] + i] = s_vals[i];
+        }
+    }
+}
+
+template <int THREADS, int IPT, typename KB, int VB, typename XF>
+__global__ __launch_bounds__(THREADS) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
+                                                    const typename ValT<VB>::type *__restrict__ vin,
+                                                    typename ValT<VB>::type *__restrict__ vout, uint32_t n,
+                                                    uint32_t shift, uint32_t pass, XF xf, Ctrl *ctrl,
+                                                    uint64_t *__restrict__ status, uint32_t back)
+{
+    __shared__ QSweepLds<THREADS, IPT, KB, VB> lds;
+    const uint32_t tile = blockIdx.x;
+    if ((uint64_t)(tile + 1) * (THREADS * IPT) <= n)
+        qsweep_body<THREADS, IPT, KB, VB, XF, true>(lds, kin, kout, vin, vout, n, tile, shift, ctrl->offsets[pass],
+                                                    xf, ctrl, status, back);
+    else
+        qsweep_body<THREADS, IPT, KB, VB, XF, false>(lds, kin, kout, vin, vout, n, tile, shift, ctrl->offsets[pass],
+                                                     xf, ctrl, status, back);
+}
+
+}  // namespace bcmi

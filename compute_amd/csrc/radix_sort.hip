@@ -16,7 +16,7 @@
 // reorders the tile in LDS so each digit run is contiguous and writes keys
 // (then values) out coalesced.  The order equals the reference's: a stable
 // sort by the transformed key radix(x) (radix_sort.hpp:98-129).
-#include "radix.hpp"
+#include "onesweep.hpp"
 
 #include <cstring>
 #include <mutex>
@@ -35,20 +35,20 @@ struct alignas(16) kchunk {
     KB v[N];
 };
 
-// Each half-wave counts into its own LDS sub-histogram: lanes adding to one
-// bin in the same instruction serialise, which matters for skewed digits
-// (the few top-digit values of floats in a narrow range); keys are read as
-// 16-byte chunks, two chunks in flight.
-template <typename KB, typename XF>
+// Each group of kHistThreads / NW lanes counts into its own LDS
+// sub-histogram: lanes adding to one bin in the same instruction serialise,
+// which matters for skewed digits (the few top-digit values of floats in a
+// narrow range); keys are read as 16-byte chunks, U chunks in flight per
+// thread.
+template <typename KB, typename XF, int U, int NW>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
                                                             XF xf, uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
-    constexpr int NW = kHistThreads / 32;  // sub-histograms: one per half-wave
     __shared__ uint32_t s_hist[NW][P][256];
     for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
     block_barrier();
-    uint32_t(*h)[256] = s_hist[threadIdx.x >> 5];
+    uint32_t(*h)[256] = s_hist[threadIdx.x / (kHistThreads / NW)];
     auto count = [&](KB x) {
         KB k = xf.key(x);
 #pragma unroll
@@ -64,15 +64,17 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
     const kchunk<KB> *body = (const kchunk<KB> *)(keys + head);
     const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
     uint64_t c = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x;
-    for (; c + stride < nchunks; c += 2 * stride) {
-        kchunk<KB> a = body[c], b = body[c + stride];
+    for (; c + (U - 1) * stride < nchunks; c += U * stride) {
+        kchunk<KB> a[U];
 #pragma unroll
-        for (int k = 0; k < kchunk<KB>::N; ++k) count(a.v[k]);
+        for (int u = 0; u < U; ++u) a[u] = body[c + u * stride];
 #pragma unroll
-        for (int k = 0; k < kchunk<KB>::N; ++k) count(b.v[k]);
+        for (int u = 0; u < U; ++u)
+#pragma unroll
+            for (int k = 0; k < kchunk<KB>::N; ++k) count(a[u].v[k]);
     }
-    if (c < nchunks) {
-        kchunk<KB> a = body[c];
+    for (; c < nchunks; c += stride) {
+        const kchunk<KB> a = body[c];
 #pragma unroll
         for (int k = 0; k < kchunk<KB>::N; ++k) count(a.v[k]);
     }
@@ -84,6 +86,49 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
         for (int w = 0; w < NW; ++w) t += (&s_hist[w][0][0])[i];
         dst[i] = t;
     }
+}
+
+// The histogram's shape: chunks in flight per thread, sub-histograms per
+// block (LDS: subs * key bytes KB), blocks per CU.  Tuning cache key
+// "__bcmi_radix_hist", params "unroll" (1, 2, 4, 8), "subs" (8 or 16; 8 for
+// 8-byte keys) and "blocks_per_cu"; the defaults were the fastest measured on
+// 2^27 uint32 keys.
+struct HistShape {
+    int unroll, subs, blocks_per_cu;
+};
+
+static HistShape hist_shape(size_t key_bytes, bool tuned)
+{
+    HistShape h{4, 16, 2};
+    if (tuned) {
+        h.unroll = (int)tune_get("__bcmi_radix_hist", "unroll", 4);
+        h.subs = (int)tune_get("__bcmi_radix_hist", "subs", 16);
+        h.blocks_per_cu = (int)tune_get("__bcmi_radix_hist", "blocks_per_cu", 2);
+    }
+    if (h.unroll != 1 && h.unroll != 2 && h.unroll != 8) h.unroll = 4;
+    if (h.subs != 8 || key_bytes >= 8) h.subs = key_bytes >= 8 ? 8 : 16;
+    if (h.blocks_per_cu < 1 || h.blocks_per_cu > 8) h.blocks_per_cu = 2;
+    return h;
+}
+
+template <typename KB, typename XF>
+static int launch_histogram(bcmi_queue q, const HistShape &hs, uint32_t hblocks, const KB *keys, uint32_t n,
+                            uint32_t head, XF xf, uint32_t *partial)
+{
+#define BCMI_HIST(U, NW)                                                                                         \
+    if (hs.unroll == U && hs.subs == NW) {                                                                      \
+        BCMI_LAUNCH(q, "radix_histogram",                                                                       \
+                    hipLaunchKernelGGL((k_histogram<KB, XF, U, NW>), dim3(hblocks), dim3(kHistThreads), 0,      \
+                                       q->stream, keys, n, head, xf, partial));                                 \
+        return 0;                                                                                               \
+    }
+    if constexpr (sizeof(KB) < 8) {
+        BCMI_HIST(1, 16) BCMI_HIST(2, 16) BCMI_HIST(4, 16) BCMI_HIST(8, 16)
+    }
+    BCMI_HIST(1, 8) BCMI_HIST(2, 8) BCMI_HIST(4, 8) BCMI_HIST(8, 8)
+#undef BCMI_HIST
+    set_error("sort: histogram shape unroll=%d subs=%d not built", hs.unroll, hs.subs);
+    return BCMI_INVALID_VALUE;
 }
 
 // One block of 1024 threads per pass: thread (slice, d) sums digit d over a
@@ -188,28 +233,26 @@ __global__ void k_clear_words(uint64_t *p, size_t n)
     for (; i < n; i += stride) p[i] = 0;
 }
 
-// LASTB: the last (most significant) pass ranks by ballots on 24-key tiles.
-// Float keys of a narrow range share few top-digit values, and many lanes
-// with one digit serialise the LDS-mask ranking's same-address atomics.
 // Bytes of queue scratch radix_sort_xf takes for n keys of key_bytes with
-// value_bytes values (at most cu_count * 4 histogram blocks).  Callers that
+// value_bytes values (at most cu_count * 8 histogram blocks).  Callers that
 // keep their own temporaries in the arena beside a sort place them after it.
 size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int cu_count)
 {
-    return align256(n * key_bytes) + align256(n * value_bytes) + (size_t)cu_count * 4 * key_bytes * 256 * 4;
+    return align256(n * key_bytes) + align256(n * value_bytes) + align256((size_t)cu_count * 8 * key_bytes * 256 * 4);
 }
 
-template <typename KB, int VB, int IPT, int RANK, typename XF, bool LASTB = false>
+template <typename KB, int VB, int TH, int IPT, typename XF>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
-    constexpr uint32_t TILE = kSortThreads * IPT;
+    constexpr uint32_t TILE = TH * IPT;
     KB *keys = (KB *)keys_;
     V *vals = (V *)vals_;
     const uint32_t tiles = (n + TILE - 1) / TILE;
+    const HistShape hs = hist_shape(sizeof(KB), work == nullptr);
     uint32_t hblocks = (n + kHistThreads * 32 - 1) / (kHistThreads * 32);
-    uint32_t hmax = (uint32_t)q->cu_count * 4;
+    uint32_t hmax = (uint32_t)(q->cu_count * hs.blocks_per_cu);
     if (hblocks > hmax) hblocks = hmax;
     if (hblocks < 1) hblocks = 1;
     uint32_t head = 0;
@@ -219,9 +262,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     const size_t kbytes = align256((size_t)n * sizeof(KB));
     const size_t vbytes = align256(VB ? (size_t)n * VB : 0);
     const size_t pbytes = align256((size_t)hblocks * P * 256 * 4);
-    constexpr uint32_t TILEB = kSortThreads * 24;
-    const uint32_t tilesb = LASTB ? (n + TILEB - 1) / TILEB : 0;
-    const size_t swords = (size_t)(tiles > tilesb ? tiles : tilesb) * 256;
+    const size_t swords = (size_t)tiles * 256;
     unsigned char *scr;
     uint64_t *status;
     if (work) {
@@ -246,9 +287,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     KB *ktmp = (KB *)(scr + pbytes);
     V *vtmp = (V *)(scr + pbytes + kbytes);
 
-    BCMI_LAUNCH(q, "radix_histogram",
-                hipLaunchKernelGGL((k_histogram<KB, XF>), dim3(hblocks), dim3(kHistThreads), 0, q->stream, keys, n, head,
-                                   xf, partial));
+    BCMI_TRY(launch_histogram(q, hs, hblocks, keys, n, head, xf, partial));
     BCMI_LAUNCH(q, "radix_digit_offsets",
                 hipLaunchKernelGGL(k_digit_offsets, dim3(P), dim3(1024), 0, q->stream, partial, hblocks,
                                    (uint32_t)P, q->ctrl));
@@ -256,16 +295,10 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     KB *ks = keys, *kd = ktmp;
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
-        if (LASTB && p == P - 1)
-            BCMI_LAUNCH(q, "radix_onesweep",
-                        hipLaunchKernelGGL((k_onesweep<KB, VB, 24, kRankBallot, XF>), dim3(tilesb), dim3(kSortThreads),
-                                           0, q->stream, ks, kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf,
-                                           q->ctrl, status, (uint32_t)(P - p)));
-        else
-            BCMI_LAUNCH(q, "radix_onesweep",
-                        hipLaunchKernelGGL((k_onesweep<KB, VB, IPT, RANK, XF>), dim3(tiles), dim3(kSortThreads), 0,
-                                           q->stream, ks, kd, vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl,
-                                           status, (uint32_t)(P - p)));
+        BCMI_LAUNCH(q, "radix_onesweep",
+                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
+                                       vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
+                                       (uint32_t)(P - p)));
         std::swap(ks, kd);
         std::swap(vs, vd);
     }
@@ -277,69 +310,55 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     return 0;
 }
 
-template <typename KB, int VB, int IPT, int RANK>
+template <typename KB, int VB, int TH, int IPT>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
-                      const SortWork *work, bool last_ballot = false)
+                      const SortWork *work)
 {
-    return with_key_xf<KB>(kind, descending, [&](auto xf) {
-        if constexpr (VB == 4 && RANK == kRankLdsMask) {
-            if (last_ballot)
-                return radix_sort_xf<KB, VB, IPT, RANK, decltype(xf), true>(q, keys, vals, n, xf, work);
-        }
-        return radix_sort_xf<KB, VB, IPT, RANK>(q, keys, vals, n, xf, work);
-    });
+    return with_key_xf<KB>(kind, descending,
+                           [&](auto xf) { return radix_sort_xf<KB, VB, TH, IPT>(q, keys, vals, n, xf, work); });
 }
 
-// Tile size (items per thread) and ranking method come from the tuning cache
-// (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
-// params "ipt" and "rank"); the variants are
-// compiled for the benchmarked 4-byte keys with 0- or 4-byte values.  The
-// default of 32 items per thread with LDS peer-mask ranking was the fastest
-// measured on 2^27 uint32 keys (16: 3.00 ms, 24: 2.39 ms, 32: 2.25 ms).
+// Tile shape: threads per workgroup and keys per thread.  For 4-byte keys
+// with no or 4-byte values it comes from the tuning cache (key
+// "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values; params
+// "threads" and "ipt"), chosen on 2^27 keys; smaller sorts take smaller
+// tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape at
+// every size).  Other key and value widths take a fixed shape whose staging
+// buffer stays within 32-64 KB (two workgroups per CU).
 template <typename KB, int VB>
-static int radix_sort_ipt(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc, const SortWork *work)
+static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc,
+                            const SortWork *work)
 {
     constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
-    constexpr int IPT = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
-        // The allocation-free sort (work != nullptr) reads no tuning cache:
-        // its sizes are bounded by the scratch, where the size rule below
-        // picks 16 or 32 keys per thread anyway.
+        // the allocation-free sort (work != nullptr) reads no tuning cache
         const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
+        unsigned th = work ? 256 : tune_get(key, "threads", 512);
         unsigned ipt = work ? 32 : tune_get(key, "ipt", 32);
-        unsigned rank = work ? (unsigned)kRankLdsMask : tune_get(key, "rank", kRankLdsMask);
-        // The tuned tile is chosen on 2^27 keys; smaller sorts need more,
-        // smaller tiles to fill the chip.  Measured on MI355X (u32 keys; f32
-        // keys + u32 values): 16 keys per thread is fastest up to 2^19 keys
-        // (2^20 with values), 32 with LDS ranking up to 2^23 (2^25 with
-        // values).  Param "fit_size" = 0 keeps the tuned tile at every size.
         const bool fit = work ? true : tune_get(key, "fit_size", 1) != 0;
         if (fit) {
-            const uint32_t small = VB ? (1u << 20) : (1u << 19), mid = VB ? (1u << 25) : (1u << 23);
-            if (n <= small) {
+            if (n <= (1u << 20)) {
+                th = 256;
                 ipt = 16;
-                rank = kRankLdsMask;
-            } else if (n <= mid && (VB || ipt > 32)) {
+            } else if (n <= (1u << 23)) {
+                th = 256;
                 ipt = 32;
-                rank = kRankLdsMask;
             }
         }
-        // pairs: param "last_ballot" = 1 ranks the top digit by ballots
-        // (chosen on 2^27 pairs; with fit_size, only above 2^25)
-        const bool lb = !work && VB && rank == kRankLdsMask && tune_get(key, "last_ballot", 0) &&
-                        (!fit || n > (1u << 25));
-        if (rank == kRankBallot) return radix_sort<KB, VB, 24, kRankBallot>(q, keys, vals, n, kind, desc, work);
-        if (ipt == 16) return radix_sort<KB, VB, 16, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
-        if (ipt == 24) return radix_sort<KB, VB, 24, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
-        if (ipt == 32) return radix_sort<KB, VB, 32, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
-        if (ipt == 40) return radix_sort<KB, VB, 40, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
-        if (ipt == 48) return radix_sort<KB, VB, 48, kRankLdsMask>(q, keys, vals, n, kind, desc, work, lb);
-        if constexpr (VB == 0) {
-            if (ipt == 64) return radix_sort<KB, VB, 64, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
-        }
-        // any other value falls through to the generic tile below
+#define BCMI_SHAPE(T, I) \
+        if (th == T && ipt == I) return radix_sort<KB, VB, T, I>(q, keys, vals, n, kind, desc, work)
+        BCMI_SHAPE(256, 16);
+        BCMI_SHAPE(256, 32);
+        BCMI_SHAPE(256, 64);
+        BCMI_SHAPE(512, 16);
+        BCMI_SHAPE(512, 24);
+        BCMI_SHAPE(512, 32);
+        BCMI_SHAPE(1024, 16);
+#undef BCMI_SHAPE
+        // any other value falls through to the generic shape below
     }
-    return radix_sort<KB, VB, IPT, kRankLdsMask>(q, keys, vals, n, kind, desc, work);
+    if constexpr (SB >= 8) return radix_sort<KB, VB, 256, 16>(q, keys, vals, n, kind, desc, work);
+    else return radix_sort<KB, VB, 256, 32>(q, keys, vals, n, kind, desc, work);
 }
 
 template <typename KB>
@@ -348,12 +367,12 @@ static int radix_sort_vb(bcmi_queue q, void *keys, void *vals, size_t vb, uint32
 {
     if (!vals) vb = 0;
     switch (vb) {
-    case 0: return radix_sort_ipt<KB, 0>(q, keys, nullptr, n, kind, desc, work);
-    case 1: return radix_sort_ipt<KB, 1>(q, keys, vals, n, kind, desc, work);
-    case 2: return radix_sort_ipt<KB, 2>(q, keys, vals, n, kind, desc, work);
-    case 4: return radix_sort_ipt<KB, 4>(q, keys, vals, n, kind, desc, work);
-    case 8: return radix_sort_ipt<KB, 8>(q, keys, vals, n, kind, desc, work);
-    case 16: return radix_sort_ipt<KB, 16>(q, keys, vals, n, kind, desc, work);
+    case 0: return radix_sort_shape<KB, 0>(q, keys, nullptr, n, kind, desc, work);
+    case 1: return radix_sort_shape<KB, 1>(q, keys, vals, n, kind, desc, work);
+    case 2: return radix_sort_shape<KB, 2>(q, keys, vals, n, kind, desc, work);
+    case 4: return radix_sort_shape<KB, 4>(q, keys, vals, n, kind, desc, work);
+    case 8: return radix_sort_shape<KB, 8>(q, keys, vals, n, kind, desc, work);
+    case 16: return radix_sort_shape<KB, 16>(q, keys, vals, n, kind, desc, work);
     default: set_error("sort: value size %zu unsupported (1, 2, 4, 8 or 16 bytes)", vb); return BCMI_INVALID_VALUE;
     }
 }

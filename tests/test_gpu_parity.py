@@ -553,20 +553,27 @@ def _tune_get(key, param):
     return v.value
 
 
-@pytest.mark.parametrize("rank,ipt,lb", [(0, 16, 0), (0, 24, 0), (0, 32, 0), (1, 24, 0), (0, 40, 0), (0, 48, 0),
-                                         (0, 64, 0), (0, 40, 1)])
-def test_sort_tunable_tile_variants(queue, rank, ipt, lb):
-    """Every tile size / ranking the tuner may select sorts exactly, keys
-    only and with values, including partial last tiles and descending."""
-    params = ("ipt", "rank", "fit_size", "last_ballot")
+@pytest.mark.parametrize("threads,ipt", [(256, 16), (256, 32), (256, 64), (512, 16), (512, 24), (512, 32),
+                                         (1024, 16)])
+def test_sort_tunable_tile_variants(queue, threads, ipt):
+    """Every tile shape the tuner may select sorts exactly, keys only and
+    with values, including partial last tiles, several histogram shapes and
+    descending order."""
+    params = ("threads", "ipt", "fit_size")
+    defaults = {"threads": 512, "ipt": 32, "fit_size": 1}
     saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
+    hparams = ("unroll", "subs", "blocks_per_cu")
+    hsaved = [_tune_get("__bcmi_radix_hist", p) for p in hparams]
+    hshape = {(256, 16): (1, 8, 4), (256, 32): (2, 16, 2), (256, 64): (8, 8, 2), (512, 16): (4, 16, 4),
+              (512, 24): (8, 16, 2), (512, 32): (4, 16, 2), (1024, 16): (2, 8, 8)}[(threads, ipt)]
     L = ca.lib()
     try:
         for key in saved:
+            ca.check(L.bcmi_tune_set(key.encode(), b"threads", threads))
             ca.check(L.bcmi_tune_set(key.encode(), b"ipt", ipt))
-            ca.check(L.bcmi_tune_set(key.encode(), b"rank", rank))
             ca.check(L.bcmi_tune_set(key.encode(), b"fit_size", 0))  # the variant at every size
-            ca.check(L.bcmi_tune_set(key.encode(), b"last_ballot", lb))
+        for p, val in zip(hparams, hshape):
+            ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val))
         for n in (70001, (1 << 20) + 333):
             k = random_keys("uint32", n)
             d = dev(queue, k)
@@ -584,9 +591,9 @@ def test_sort_tunable_tile_variants(queue, rank, ipt, lb):
         for key, vals in saved.items():
             for p, val in zip(params, vals):
                 # an absent entry reads back as the sentinel: restore the built-in default
-                ca.check(L.bcmi_tune_set(key.encode(), p.encode(),
-                                         val if val != 0xFFFFFFFF else
-                                         {"ipt": 32, "rank": 0, "fit_size": 1, "last_ballot": 0}[p]))
+                ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val if val != 0xFFFFFFFF else defaults[p]))
+        for p, val, dflt in zip(hparams, hsaved, (4, 16, 2)):
+            ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val if val != 0xFFFFFFFF else dflt))
 
 
 # ========================================================= segmented sort
