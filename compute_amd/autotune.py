@@ -11,8 +11,10 @@ the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
 
 Tunables (the digit width is fixed at 8 bits by the per-digit LDS words;
 what varies is the tile shape and the work per thread):
-  __bcmi_radix_sort|threads, |ipt       keys-only 4-byte radix sort: workgroup size, keys per thread
-  __bcmi_radix_sort_by_key|threads, |ipt  4-byte keys with 4-byte values
+  __bcmi_radix_sort|threads, |ipt, |split, |xcd_run  keys-only radix sort: workgroup size, keys per
+                                       thread, staging rounds (4-byte keys), tiles per XCD run
+                                       (0 = dispatch order)
+  __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu  the sort's histogram kernel
   __bcmi_scan|q                        4-byte scans, items per lane
   __bcmi_reduce_by_key|q               reduce_by_key, items per lane
@@ -31,7 +33,10 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
-SORT_SHAPES = [(256, 32), (256, 64), (512, 16), (512, 24), (512, 32), (1024, 16)]  # (threads, ipt)
+# (threads, keys per thread, staging rounds)
+SORT_SHAPES = [(256, 64, 1), (512, 24, 1), (512, 32, 1), (1024, 16, 1), (256, 64, 2), (512, 24, 2), (512, 32, 2),
+               (512, 24, 3)]
+XCD_RUNS = (0, 4, 8)  # consecutive tiles per XCD (0: tile = workgroup id)
 HIST_SHAPES = [(u, nw, b) for u in (2, 4, 8) for nw in (8, 16) for b in (2, 4)]  # (unroll, subs, blocks/CU)
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
@@ -64,18 +69,22 @@ def tune_sort(ca, W, q, log2n, log):
     work = ca.DeviceArray(n, np.uint32, q)
     restore = lambda: ca.check(L.bcmi_memcpy(q.h, work.ptr, src.ptr, src.nbytes, ca.D2D, 0))
     best = None
-    for th, ipt in SORT_SHAPES:
-        _set(ca, "__bcmi_radix_sort", "threads", th)
-        _set(ca, "__bcmi_radix_sort", "ipt", ipt)
-        ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
-        ok = bool(np.array_equal(work.to_host(q), expect))
-        log({"tunable": "__bcmi_radix_sort", "threads": th, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
-        if ok and (best is None or ms < best[0]):
-            best = (ms, th, ipt)
+    for th, ipt, sp in SORT_SHAPES:
+        for xr in XCD_RUNS:
+            _set(ca, "__bcmi_radix_sort", "threads", th)
+            _set(ca, "__bcmi_radix_sort", "ipt", ipt)
+            _set(ca, "__bcmi_radix_sort", "split", sp)
+            _set(ca, "__bcmi_radix_sort", "xcd_run", xr)
+            ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
+            ok = bool(np.array_equal(work.to_host(q), expect))
+            log({"tunable": "__bcmi_radix_sort", "threads": th, "ipt": ipt, "split": sp, "xcd_run": xr,
+                 "ms": round(ms, 4), "ok": ok})
+            if ok and (best is None or ms < best[0]):
+                best = (ms, th, ipt, sp, xr)
     if best is None:
         raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
-    _set(ca, "__bcmi_radix_sort", "threads", best[1])
-    _set(ca, "__bcmi_radix_sort", "ipt", best[2])
+    for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
+        _set(ca, "__bcmi_radix_sort", p, v)
 
     # the histogram kernel, timed inside the tuned sort (queue profiler)
     best = None
@@ -113,9 +122,11 @@ def tune_sort(ca, W, q, log2n, log):
 
     first = None
     best = None
-    for th, ipt in SORT_SHAPES:
+    for th, ipt, sp, xr in [(t, i, s, x) for t, i, s in SORT_SHAPES for x in XCD_RUNS]:
         _set(ca, "__bcmi_radix_sort_by_key", "threads", th)
         _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
+        _set(ca, "__bcmi_radix_sort_by_key", "split", sp)
+        _set(ca, "__bcmi_radix_sort_by_key", "xcd_run", xr)
         ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)
         got = (wk.to_host(q).view(np.uint32), wv.to_host(q))
         if first is None:
@@ -126,13 +137,14 @@ def tune_sort(ca, W, q, log2n, log):
             first = got
         else:
             ok = bool(np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]))
-        log({"tunable": "__bcmi_radix_sort_by_key", "threads": th, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_radix_sort_by_key", "threads": th, "ipt": ipt, "split": sp, "xcd_run": xr,
+             "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, th, ipt)
+            best = (ms, th, ipt, sp, xr)
     if best is None:
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
-    _set(ca, "__bcmi_radix_sort_by_key", "threads", best[1])
-    _set(ca, "__bcmi_radix_sort_by_key", "ipt", best[2])
+    for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
+        _set(ca, "__bcmi_radix_sort_by_key", p, v)
 
 
 def tune_scan(ca, W, q, log2n, log):

@@ -36,6 +36,7 @@ BDEV int block_barrier_or(int pred)
 }
 
 constexpr int kWave = 64;
+constexpr int kLdsBytesPerCU = 160 * 1024;  // gfx950 LDS per compute unit
 
 BDEV uint32_t lane_id() { return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u)); }
 
@@ -169,6 +170,17 @@ BDEV uint32_t launch_epoch_after(const Ctrl *ctrl, uint32_t back, D after)
     const uint32_t *p = (const uint32_t *)((const char *)&ctrl->epoch + z);
     const uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     return ((__builtin_amdgcn_readfirstlane(e) - back) % kEpochMask) + 1u;
+}
+
+// The same epoch as a plain load at kernel entry.  ctrl->epoch is written
+// only by k_epoch_advance, an earlier kernel on the same stream, so it is
+// constant for this launch and visible to an ordinary load; with a uniform
+// address that load can be served by the scalar cache, off the vector
+// memory path the tile's data uses.
+BDEV uint32_t launch_epoch_plain(const Ctrl *ctrl, uint32_t back)
+{
+    const uint32_t e = ctrl->epoch;
+    return ((e - back) % kEpochMask) + 1u;
 }
 
 // Tiles are numbered by blockIdx.x.  Workgroups are dispatched in id order,

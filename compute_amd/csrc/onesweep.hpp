@@ -25,21 +25,29 @@
 // atomic add to count, one write and two reads to reorder.
 //
 // LDS: the count words alias the staging buffer (ranking finishes before
-// the first staging write), so a tile needs max(TILE * key bytes,
-// THREADS * 64) bytes plus 2-3 KB.
+// the first staging write), so a tile needs max(staged bytes, THREADS * 64)
+// bytes plus 2-3 KB, the staged bytes being the whole tile's keys (or
+// values) or, staged in SPLIT rounds, a round's keys and values.
 #pragma once
 
 #include "radix.hpp"
 
 namespace bcmi {
 
-template <int THREADS, int IPT, typename KB, int VB>
+// SPLIT = 1: the whole tile is staged, keys then values.  SPLIT > 1: the
+// tile's positions are staged in SPLIT rounds of TILE / SPLIT, keys and
+// values of a round side by side, which shrinks the LDS a workgroup holds
+// (more workgroups per CU to hide memory latency) for one more barrier per
+// round.
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT>
 struct QSweepLds {
     static constexpr int NSUB = THREADS / 16;
     static constexpr int TILE = THREADS * IPT;
-    static constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+    static constexpr int H = TILE / SPLIT;
+    static constexpr int SB = SPLIT == 1 ? ((int)sizeof(KB) > VB ? (int)sizeof(KB) : VB) * TILE
+                                         : ((int)sizeof(KB) + VB) * H;
     static constexpr int WORDS = NSUB * 256 * 4;
-    static constexpr int STAGE = TILE * SB > WORDS ? TILE * SB : WORDS;
+    static constexpr int STAGE = SB > WORDS ? SB : WORDS;
     alignas(16) unsigned char stage[STAGE];
     uint32_t goff[256];
     uint32_t part[THREADS / 256][256];
@@ -127,22 +135,26 @@ BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, 
     }
 }
 
-template <int THREADS, int IPT, typename KB, int VB, typename XF, bool FULL>
-BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, typename XF, bool FULL>
+BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
                       const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
                       uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf, Ctrl *ctrl,
-                      uint64_t *__restrict__ status, uint32_t back)
+                      uint64_t *__restrict__ status, uint32_t epoch)
 {
     using V = typename ValT<VB>::type;
     constexpr int NSUB = THREADS / 16, TILE = THREADS * IPT, PARTS = THREADS / 256, SPP = NSUB / PARTS;
     static_assert(THREADS % 256 == 0 && TILE <= 32768, "256-multiple blocks, tile positions in 15 bits");
+    static_assert(IPT % SPLIT == 0, "whole slots per staging round");
     uint32_t *words = (uint32_t *)L.stage;
     KB *s_keys = (KB *)L.stage;
     V *s_vals = (V *)L.stage;
     const uint32_t tid = threadIdx.x, ql = tid & 15, sub = tid >> 4;
-    // zero the count words: 16 per thread, four 16-byte stores
+    // each sub zeroes its own 256 count words (16 per lane, four 16-byte
+    // stores); only the sub's lanes touch them before the barrier that ends
+    // the counting, and one wave's LDS operations execute in order
+    uint32_t *my = words + sub * 256;
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ((uint4 *)words)[i * THREADS + tid] = make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) ((uint4 *)my)[i * 16 + ql] = make_uint4(0, 0, 0, 0);
     const uint32_t base = tile * (uint32_t)TILE;
     const int32_t lim = FULL ? TILE : (int32_t)(n - base) - (int32_t)(sub * 16 * IPT);
     auto valid = [&](int j) { return FULL || (int32_t)(j * 16 + ql) < lim; };
@@ -157,9 +169,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__
         for (int j = 0; j < IPT; ++j)
             if (valid(j)) v[j] = vq[j * 16];
     }
-    block_barrier();
     // 1. early counts
-    uint32_t *my = words + sub * 256;
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
         if (valid(j))
@@ -190,9 +200,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__
     const uint32_t lane = tid & 63, w = tid >> 6;
     const uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
     if (lane == 63 && w < 4) L.wsum[w] = inc;
-    uint32_t epoch = 0;
     if (part == 0) {
-        epoch = launch_epoch_after(ctrl, back, total);
         if (tile == 0) lb_store(&status[d], lb_word(epoch, LB_P, total));
         else lb_store(&status[(size_t)tile * 256 + d], lb_word(epoch, LB_A, total));
     }
@@ -244,11 +252,40 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__
         return;
     }
     // 4. reorder by position, write out coalesced
+    const uint32_t tile_n = FULL ? (uint32_t)TILE : n - base;
+    if constexpr (SPLIT > 1) {
+        // round r stages positions [r * H, r * H + H): keys, then values
+        constexpr int H = TILE / SPLIT, JR = IPT / SPLIT;
+        V *s_v = (V *)(L.stage + H * sizeof(KB));
+#pragma unroll
+        for (int r = 0; r < SPLIT; ++r) {
+            if (r > 0) block_barrier();  // the previous round's reads are done
+#pragma unroll
+            for (int j = 0; j < IPT; ++j) {
+                const uint32_t p = pos[j] - (uint32_t)(r * H);
+                if (valid(j) && p < (uint32_t)H) {
+                    s_keys[p] = k[j];
+                    if constexpr (VB > 0) s_v[p] = v[j];
+                }
+            }
+            block_barrier();
+#pragma unroll
+            for (int jj = 0; jj < JR; ++jj) {
+                const uint32_t li = jj * THREADS + tid, i = (uint32_t)(r * H) + li;
+                if (FULL || i < tile_n) {
+                    const KB key = s_keys[li];
+                    const uint32_t o = L.goff[xf.digit(key, shift)] + i;
+                    kout[o] = key;
+                    if constexpr (VB > 0) vout[o] = s_v[li];
+                }
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
         if (valid(j)) s_keys[pos[j]] = k[j];
     block_barrier();
-    const uint32_t tile_n = FULL ? (uint32_t)TILE : n - base;
     Dig8<(VB > 0 ? IPT : 1)> od;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -274,21 +311,39 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB> &L, const KB *__restrict__
     }
 }
 
-template <int THREADS, int IPT, typename KB, int VB, typename XF>
+// Tile of workgroup b.  Workgroups go to the XCDs round-robin (b mod 8), so
+// with xrun = 0 consecutive tiles run on different XCDs and the partial
+// 64-byte lines where one tile's digit run ends and the next tile's begins
+// reach HBM from two L2s.  xrun = R > 0 renumbers each group of 8R
+// workgroups so that tiles [g*8R + x*R, g*8R + x*R + R) run on XCD x and
+// those lines meet in one L2; a trailing partial group keeps b.  A tile's
+// look-back may then wait on a workgroup dispatched up to 8R - 1 later,
+// which needs 8R workgroups resident at once (the host enables it only
+// when the device holds far more).
+BDEV uint32_t xcd_tile(uint32_t b, uint32_t xrun)
+{
+    if (xrun == 0) return b;
+    const uint32_t G = 8 * xrun, g = b / G, r = b - g * G;
+    if ((g + 1) * G > gridDim.x) return b;
+    return g * G + (r & 7) * xrun + (r >> 3);
+}
+
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, typename XF>
 __global__ __launch_bounds__(THREADS) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                     const typename ValT<VB>::type *__restrict__ vin,
                                                     typename ValT<VB>::type *__restrict__ vout, uint32_t n,
                                                     uint32_t shift, uint32_t pass, XF xf, Ctrl *ctrl,
-                                                    uint64_t *__restrict__ status, uint32_t back)
+                                                    uint64_t *__restrict__ status, uint32_t back, uint32_t xrun)
 {
-    __shared__ QSweepLds<THREADS, IPT, KB, VB> lds;
-    const uint32_t tile = blockIdx.x;
+    __shared__ QSweepLds<THREADS, IPT, KB, VB, SPLIT> lds;
+    const uint32_t tile = xcd_tile(blockIdx.x, xrun);
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((uint64_t)(tile + 1) * (THREADS * IPT) <= n)
-        qsweep_body<THREADS, IPT, KB, VB, XF, true>(lds, kin, kout, vin, vout, n, tile, shift, ctrl->offsets[pass],
-                                                    xf, ctrl, status, back);
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, XF, true>(lds, kin, kout, vin, vout, n, tile, shift,
+                                                           ctrl->offsets[pass], xf, ctrl, status, epoch);
     else
-        qsweep_body<THREADS, IPT, KB, VB, XF, false>(lds, kin, kout, vin, vout, n, tile, shift, ctrl->offsets[pass],
-                                                     xf, ctrl, status, back);
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, XF, false>(lds, kin, kout, vin, vout, n, tile, shift,
+                                                            ctrl->offsets[pass], xf, ctrl, status, epoch);
 }
 
 }  // namespace bcmi

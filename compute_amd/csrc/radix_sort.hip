@@ -38,8 +38,8 @@ struct alignas(16) kchunk {
 // Each group of kHistThreads / NW lanes counts into its own LDS
 // sub-histogram: lanes adding to one bin in the same instruction serialise,
 // which matters for skewed digits (the few top-digit values of floats in a
-// narrow range); keys are read as 16-byte chunks, U chunks in flight per
-// thread.
+// narrow range); keys are read as 16-byte chunks, U per step, with the next
+// step's U chunks in flight while a step is counted.
 template <typename KB, typename XF, int U, int NW>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
                                                             XF xf, uint32_t *__restrict__ partial)
@@ -64,14 +64,29 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
     const kchunk<KB> *body = (const kchunk<KB> *)(keys + head);
     const uint64_t stride = (uint64_t)gridDim.x * kHistThreads;
     uint64_t c = (uint64_t)blockIdx.x * kHistThreads + threadIdx.x;
-    for (; c + (U - 1) * stride < nchunks; c += U * stride) {
-        kchunk<KB> a[U];
+    // U chunks per step, the next step's chunks loaded before this step's
+    // are counted (2U in flight while the LDS works)
+    if (c + (U - 1) * stride < nchunks) {
+        kchunk<KB> cur[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) a[u] = body[c + u * stride];
+        for (int u = 0; u < U; ++u) cur[u] = body[c + u * stride];
+        while (true) {
+            const uint64_t nc = c + U * stride;
+            const bool more = nc + (U - 1) * stride < nchunks;
+            kchunk<KB> nxt[U];
+            if (more) {
 #pragma unroll
-        for (int u = 0; u < U; ++u)
+                for (int u = 0; u < U; ++u) nxt[u] = body[nc + u * stride];
+            }
 #pragma unroll
-            for (int k = 0; k < kchunk<KB>::N; ++k) count(a[u].v[k]);
+            for (int u = 0; u < U; ++u)
+#pragma unroll
+                for (int k = 0; k < kchunk<KB>::N; ++k) count(cur[u].v[k]);
+            c = nc;
+            if (!more) break;
+#pragma unroll
+            for (int u = 0; u < U; ++u) cur[u] = nxt[u];
+        }
     }
     for (; c < nchunks; c += stride) {
         const kchunk<KB> a = body[c];
@@ -241,8 +256,9 @@ size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int c
     return align256(n * key_bytes) + align256(n * value_bytes) + align256((size_t)cu_count * 8 * key_bytes * 256 * 4);
 }
 
-template <typename KB, int VB, int TH, int IPT, typename XF>
-static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work)
+template <typename KB, int VB, int TH, int IPT, int SPLIT, typename XF>
+static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work,
+                         unsigned xrun)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
@@ -250,6 +266,13 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     KB *keys = (KB *)keys_;
     V *vals = (V *)vals_;
     const uint32_t tiles = (n + TILE - 1) / TILE;
+    // XCD-grouped tile order only where the device holds at least twice a
+    // group of 8 * xrun workgroups at once (LDS and thread limits of a CU)
+    {
+        const size_t per_cu_lds = (size_t)kLdsBytesPerCU / sizeof(QSweepLds<TH, IPT, KB, VB, SPLIT>);
+        const size_t per_cu = per_cu_lds < (size_t)(2048 / TH) ? per_cu_lds : (size_t)(2048 / TH);
+        if (xrun > 16 || (size_t)q->cu_count * per_cu < 16 * (size_t)xrun) xrun = 0;
+    }
     const HistShape hs = hist_shape(sizeof(KB), work == nullptr);
     uint32_t hblocks = (n + kHistThreads * 32 - 1) / (kHistThreads * 32);
     uint32_t hmax = (uint32_t)(q->cu_count * hs.blocks_per_cu);
@@ -296,9 +319,9 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
+                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
                                        vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
-                                       (uint32_t)(P - p)));
+                                       (uint32_t)(P - p), (uint32_t)xrun));
         std::swap(ks, kd);
         std::swap(vs, vd);
     }
@@ -310,55 +333,64 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     return 0;
 }
 
-template <typename KB, int VB, int TH, int IPT>
+template <typename KB, int VB, int TH, int IPT, int SPLIT = 1>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
-                      const SortWork *work)
+                      const SortWork *work, unsigned xrun)
 {
-    return with_key_xf<KB>(kind, descending,
-                           [&](auto xf) { return radix_sort_xf<KB, VB, TH, IPT>(q, keys, vals, n, xf, work); });
+    return with_key_xf<KB>(kind, descending, [&](auto xf) {
+        return radix_sort_xf<KB, VB, TH, IPT, SPLIT>(q, keys, vals, n, xf, work, xrun);
+    });
 }
 
-// Tile shape: threads per workgroup and keys per thread.  For 4-byte keys
-// with no or 4-byte values it comes from the tuning cache (key
-// "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values; params
-// "threads" and "ipt"), chosen on 2^27 keys; smaller sorts take smaller
-// tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape at
-// every size).  Other key and value widths take a fixed shape whose staging
-// buffer stays within 32-64 KB (two workgroups per CU).
+// Tile shape: threads per workgroup, keys per thread and staging rounds.
+// For 4-byte keys with no or 4-byte values it comes from the tuning cache
+// (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
+// params "threads", "ipt", "split"), chosen on 2^27 keys; smaller sorts take
+// smaller tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape
+// at every size).  Other key and value widths take a fixed shape whose
+// staging buffer stays within 32-64 KB (two workgroups per CU).
 template <typename KB, int VB>
 static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int desc,
                             const SortWork *work)
 {
     constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+    // the allocation-free sort (work != nullptr) reads no tuning cache
+    const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
+    const unsigned xrun = work ? 0u : tune_get(key, "xcd_run", 0);
     if constexpr (sizeof(KB) == 4 && (VB == 0 || VB == 4)) {
-        // the allocation-free sort (work != nullptr) reads no tuning cache
-        const char *key = VB ? "__bcmi_radix_sort_by_key" : "__bcmi_radix_sort";
         unsigned th = work ? 256 : tune_get(key, "threads", 512);
         unsigned ipt = work ? 32 : tune_get(key, "ipt", 32);
+        unsigned split = work ? 1 : tune_get(key, "split", 1);
         const bool fit = work ? true : tune_get(key, "fit_size", 1) != 0;
         if (fit) {
             if (n <= (1u << 20)) {
                 th = 256;
                 ipt = 16;
+                split = 1;
             } else if (n <= (1u << 23)) {
                 th = 256;
                 ipt = 32;
+                split = 1;
             }
         }
-#define BCMI_SHAPE(T, I) \
-        if (th == T && ipt == I) return radix_sort<KB, VB, T, I>(q, keys, vals, n, kind, desc, work)
-        BCMI_SHAPE(256, 16);
-        BCMI_SHAPE(256, 32);
-        BCMI_SHAPE(256, 64);
-        BCMI_SHAPE(512, 16);
-        BCMI_SHAPE(512, 24);
-        BCMI_SHAPE(512, 32);
-        BCMI_SHAPE(1024, 16);
+#define BCMI_SHAPE(T, I, S) \
+        if (th == T && ipt == I && split == S) return radix_sort<KB, VB, T, I, S>(q, keys, vals, n, kind, desc, work, xrun)
+        BCMI_SHAPE(256, 16, 1);
+        BCMI_SHAPE(256, 32, 1);
+        BCMI_SHAPE(256, 64, 1);
+        BCMI_SHAPE(512, 16, 1);
+        BCMI_SHAPE(512, 24, 1);
+        BCMI_SHAPE(512, 32, 1);
+        BCMI_SHAPE(1024, 16, 1);
+        BCMI_SHAPE(256, 64, 2);
+        BCMI_SHAPE(512, 24, 2);
+        BCMI_SHAPE(512, 32, 2);
+        BCMI_SHAPE(512, 24, 3);
 #undef BCMI_SHAPE
         // any other value falls through to the generic shape below
     }
-    if constexpr (SB >= 8) return radix_sort<KB, VB, 256, 16>(q, keys, vals, n, kind, desc, work);
-    else return radix_sort<KB, VB, 256, 32>(q, keys, vals, n, kind, desc, work);
+    if constexpr (SB >= 8) return radix_sort<KB, VB, 256, 16>(q, keys, vals, n, kind, desc, work, xrun);
+    else return radix_sort<KB, VB, 256, 32>(q, keys, vals, n, kind, desc, work, xrun);
 }
 
 template <typename KB>

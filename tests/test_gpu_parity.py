@@ -321,6 +321,35 @@ def test_scan_float_compensated(queue, n):
     assert np.all(err <= _scan_err_bound(x, dd))
 
 
+def test_scan_float_compensated_edge_forms(queue):
+    """The compensated float scan on partial tiles, exclusive with an init,
+    in place, and with unaligned input and output (scalar loads)."""
+    L = ca.lib()
+    for n in (100, 4096 * 3 + 5, 8192 * 4, (1 << 20) + 17):
+        x = O.gen_lognormal_f32(n)
+        d = dev(queue, x)
+        out = ca.DeviceArray(n, np.float32, queue)
+        ca.scan(queue, d, out, False)
+        dd = O.scan_f32_dd(x, False)
+        assert np.all(np.abs(out.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(x, dd)), n
+        # exclusive with init 2.5: the oracle scans [2.5, x0, x1, ...]
+        ca.scan(queue, d, out, True, init=np.float32(2.5))
+        xi = np.r_[np.float32(2.5), x[:-1]].astype(np.float32)
+        dd = O.scan_f32_dd(xi, False)
+        assert np.all(np.abs(out.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(xi, dd)), n
+        # in place
+        ca.scan(queue, d, d, False)
+        dd = O.scan_f32_dd(x, False)
+        assert np.all(np.abs(d.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(x, dd)), n
+        # input at +1 element, output at +2
+        d = dev(queue, x)
+        m = n - 2
+        ca.check(L.bcmi_scan(queue.h, ca.F32, ca.PLUS, d.at(1), out.at(2), m, None, 0, 0))
+        dd = O.scan_f32_dd(np.ascontiguousarray(x[1:1 + m]), False)
+        got = out.to_host(queue)[2:].astype(np.float64)
+        assert np.all(np.abs(got - dd) <= _scan_err_bound(x[1:1 + m], dd)), n
+
+
 # ================================================================= reduce
 @pytest.mark.parametrize("dt", ["int32", "uint32", "int64", "uint8", "int16", "uint64"])
 @pytest.mark.parametrize("op", ["plus", "multiplies", "min", "max"])
@@ -553,28 +582,36 @@ def _tune_get(key, param):
     return v.value
 
 
-@pytest.mark.parametrize("threads,ipt", [(256, 16), (256, 32), (256, 64), (512, 16), (512, 24), (512, 32),
-                                         (1024, 16)])
-def test_sort_tunable_tile_variants(queue, threads, ipt):
+@pytest.mark.parametrize("threads,ipt,split", [(256, 16, 1), (256, 32, 1), (256, 64, 1), (512, 16, 1), (512, 24, 1),
+                                               (512, 32, 1), (1024, 16, 1), (256, 64, 2), (512, 24, 2),
+                                               (512, 32, 2), (512, 24, 3)])
+def test_sort_tunable_tile_variants(queue, threads, ipt, split):
     """Every tile shape the tuner may select sorts exactly, keys only and
-    with values, including partial last tiles, several histogram shapes and
+    with values, including partial last tiles, several histogram shapes,
+    XCD-grouped tile orders (full groups and a trailing partial group) and
     descending order."""
-    params = ("threads", "ipt", "fit_size")
-    defaults = {"threads": 512, "ipt": 32, "fit_size": 1}
+    params = ("threads", "ipt", "fit_size", "xcd_run", "split")
+    defaults = {"threads": 512, "ipt": 32, "fit_size": 1, "xcd_run": 0, "split": 1}
     saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
     hparams = ("unroll", "subs", "blocks_per_cu")
     hsaved = [_tune_get("__bcmi_radix_hist", p) for p in hparams]
     hshape = {(256, 16): (1, 8, 4), (256, 32): (2, 16, 2), (256, 64): (8, 8, 2), (512, 16): (4, 16, 4),
               (512, 24): (8, 16, 2), (512, 32): (4, 16, 2), (1024, 16): (2, 8, 8)}[(threads, ipt)]
+    xrun = {(256, 16): 2, (256, 32): 8, (256, 64): 4, (512, 16): 8, (512, 24): 4, (512, 32): 2,
+            (1024, 16): 8}[(threads, ipt)]
+    if split > 1:
+        xrun = 8 // split
     L = ca.lib()
     try:
         for key in saved:
             ca.check(L.bcmi_tune_set(key.encode(), b"threads", threads))
             ca.check(L.bcmi_tune_set(key.encode(), b"ipt", ipt))
             ca.check(L.bcmi_tune_set(key.encode(), b"fit_size", 0))  # the variant at every size
+            ca.check(L.bcmi_tune_set(key.encode(), b"xcd_run", xrun))
+            ca.check(L.bcmi_tune_set(key.encode(), b"split", split))
         for p, val in zip(hparams, hshape):
             ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val))
-        for n in (70001, (1 << 20) + 333):
+        for n in (70001, (1 << 20) + 333, 5000011):
             k = random_keys("uint32", n)
             d = dev(queue, k)
             ca.sort(queue, d, api=ca.RADIX_SORT)
