@@ -42,7 +42,7 @@ EXPORTS = [
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
     "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free", "bcmi_merge_sort", "bcmi_scan_iter",
-    "bcmi_transform_iter",
+    "bcmi_transform_iter", "bcmi_convert",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -214,6 +214,7 @@ def lib():
         L.bcmi_merge_sort.argtypes = [vp, C.POINTER(Comparator), vp, sz, vp, sz, sz]
         L.bcmi_scan_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), vp, sz, vp, i, i]
         L.bcmi_transform_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), C.POINTER(Iter), vp, sz]
+        L.bcmi_convert.argtypes = [vp, i, i, vp, vp, sz]
         L.bcmi_host_malloc.argtypes = [sz, C.POINTER(vp)]
         L.bcmi_host_free.argtypes = [vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,

@@ -283,6 +283,43 @@ static int transform_iter_typed(bcmi_queue q, int op, const bcmi_iter *a, const 
 #undef BCMI_TF_ITER
 }
 
+// Element type conversion, OpenCL C's implicit conversion of the
+// reference's copy and scan kernels (`result[i] = x` with differing element
+// types, detail/copy_on_device.hpp:59-92, scan_on_gpu.hpp:86-88): integers
+// wrap, int -> float rounds to nearest, float -> int truncates toward zero
+// (out-of-range values are undefined there; here the hardware saturates).
+template <typename S, typename D>
+__global__ __launch_bounds__(kTfThreads) void k_convert(const S *__restrict__ src, D *__restrict__ dst, size_t n)
+{
+    const size_t stride = (size_t)gridDim.x * kTfThreads;
+    for (size_t i = (size_t)blockIdx.x * kTfThreads + threadIdx.x; i < n; i += stride) dst[i] = (D)src[i];
+}
+
+template <typename S>
+static int convert_from(bcmi_queue q, int dst_dtype, const void *src, void *dst, size_t n)
+{
+    const unsigned blocks = (unsigned)std::min<size_t>((n + kTfThreads - 1) / kTfThreads, (size_t)cu_blocks(q, 8));
+#define BCMI_CV(CODE, D)                                                                                          \
+    case CODE:                                                                                                    \
+        BCMI_LAUNCH(q, "convert", hipLaunchKernelGGL((k_convert<S, D>), dim3(blocks), dim3(kTfThreads), 0,        \
+                                                     q->stream, (const S *)src, (D *)dst, n));                    \
+        return 0
+    switch (dst_dtype) {
+        BCMI_CV(BCMI_I8, int8_t);
+        BCMI_CV(BCMI_U8, uint8_t);
+        BCMI_CV(BCMI_I16, int16_t);
+        BCMI_CV(BCMI_U16, uint16_t);
+        BCMI_CV(BCMI_I32, int32_t);
+        BCMI_CV(BCMI_U32, uint32_t);
+        BCMI_CV(BCMI_I64, int64_t);
+        BCMI_CV(BCMI_U64, uint64_t);
+        BCMI_CV(BCMI_F32, float);
+        BCMI_CV(BCMI_F64, double);
+    default: set_error("convert: bad destination dtype %d", dst_dtype); return BCMI_INVALID_VALUE;
+    }
+#undef BCMI_CV
+}
+
 static int check_iter(const bcmi_iter *it, bool allow_length)
 {
     BCMI_CHECK(it->kind >= BCMI_ITER_BUFFER && it->kind <= BCMI_ITER_CONSTANT, BCMI_INVALID_VALUE,
@@ -328,5 +365,25 @@ extern "C" int bcmi_transform_iter(bcmi_queue q, int dtype, int width, int op, c
     case BCMI_F64: return transform_iter_typed<double>(q, op, a, b, out, n);
     default: set_error("transform: iterator inputs of dtype %d unsupported (32/64-bit int, float, double)", dtype);
              return BCMI_INVALID_VALUE;
+    }
+}
+
+extern "C" int bcmi_convert(bcmi_queue q, int src_dtype, int dst_dtype, const void *src, void *dst, size_t n)
+{
+    BCMI_TRY(use_queue(q));
+    if (n == 0) return 0;
+    BCMI_CHECK(src && dst, BCMI_INVALID_VALUE, "convert: null pointer");
+    switch (src_dtype) {
+    case BCMI_I8: return convert_from<int8_t>(q, dst_dtype, src, dst, n);
+    case BCMI_U8: return convert_from<uint8_t>(q, dst_dtype, src, dst, n);
+    case BCMI_I16: return convert_from<int16_t>(q, dst_dtype, src, dst, n);
+    case BCMI_U16: return convert_from<uint16_t>(q, dst_dtype, src, dst, n);
+    case BCMI_I32: return convert_from<int32_t>(q, dst_dtype, src, dst, n);
+    case BCMI_U32: return convert_from<uint32_t>(q, dst_dtype, src, dst, n);
+    case BCMI_I64: return convert_from<int64_t>(q, dst_dtype, src, dst, n);
+    case BCMI_U64: return convert_from<uint64_t>(q, dst_dtype, src, dst, n);
+    case BCMI_F32: return convert_from<float>(q, dst_dtype, src, dst, n);
+    case BCMI_F64: return convert_from<double>(q, dst_dtype, src, dst, n);
+    default: set_error("convert: bad source dtype %d", src_dtype); return BCMI_INVALID_VALUE;
     }
 }

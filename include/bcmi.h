@@ -387,6 +387,15 @@ int bcmi_transform_iter(bcmi_queue queue, int dtype, int width, int op,
                         const bcmi_iter *a, const bcmi_iter *b, void *out,
                         size_t n);
 
+/* dst[i] = (dst type) src[i]: the element conversion of copy() between
+ * vectors of different element types (detail/copy_on_device.hpp:59-92) and
+ * of a scan whose output type differs from its input type
+ * (detail/scan_on_gpu.hpp:196-205: the scan computes in the input type and
+ * converts on store).  OpenCL C conversions: integers wrap, int -> float
+ * rounds to nearest, float -> int truncates toward zero. */
+int bcmi_convert(bcmi_queue queue, int src_dtype, int dst_dtype,
+                 const void *src, void *dst, size_t n);
+
 /* The key extraction of experimental::sort_by_transform
  * (experimental/sort_by_transform.hpp): keys[i] = f(field of element i),
  * where element i is record_bytes bytes at records + i * record_bytes, the

@@ -53,6 +53,20 @@ inline buffer_iterator<T> copy_dispatch(buffer_iterator<T> first, buffer_iterato
     return result + (std::ptrdiff_t)n;
 }
 
+// device -> device between element types: converted element by element on
+// the device (copy_on_device.hpp:59-92 assigns through the element types)
+template <class S, class D>
+inline typename std::enable_if<!std::is_same<S, D>::value, buffer_iterator<D> >::type copy_dispatch(
+    buffer_iterator<S> first, buffer_iterator<S> last, buffer_iterator<D> result, command_queue &queue,
+    std::true_type, std::true_type)
+{
+    const size_t n = (size_t)(last - first);
+    if (n)
+        bcmi_check(bcmi_convert(queue.get(), bcmi_dtype<S>(), bcmi_dtype<D>(), first.device_ptr(),
+                                result.device_ptr(), n));
+    return result + (std::ptrdiff_t)n;
+}
+
 // counting / constant / transform iterator into a device range: the
 // sequence generated (and mapped) on the device (test/test_constant_iterator.cpp:60-91)
 template <class InputIterator, class T>

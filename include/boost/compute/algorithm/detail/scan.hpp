@@ -5,6 +5,7 @@
 #ifndef BOOST_COMPUTE_ALGORITHM_DETAIL_SCAN_HPP
 #define BOOST_COMPUTE_ALGORITHM_DETAIL_SCAN_HPP
 
+#include <boost/compute/container/vector.hpp>
 #include <boost/compute/detail/iter_desc.hpp>
 #include <boost/compute/functional/operator.hpp>
 #include <boost/compute/system.hpp>
@@ -20,7 +21,6 @@ inline buffer_iterator<OutT> scan(buffer_iterator<InT> first, buffer_iterator<In
                                   buffer_iterator<OutT> result, bool exclusive, T init, BinaryOperator,
                                   command_queue &queue)
 {
-    static_assert(std::is_same<InT, OutT>::value, "scan: input and output element types must match");
     // minus and divides are not associative: a parallel scan has no
     // well-defined result for them (the reference's GPU scan folds in a tree
     // order, scan_on_gpu.hpp:84-93), so they are refused at compile time
@@ -28,10 +28,23 @@ inline buffer_iterator<OutT> scan(buffer_iterator<InT> first, buffer_iterator<In
                       BinaryOperator::bcmi_op == BCMI_MIN || BinaryOperator::bcmi_op == BCMI_MAX,
                   "scan: the operator must be plus, multiplies, min or max");
     const size_t n = (size_t)(last - first);
-    const OutT ini = (OutT)init;
-    bcmi_check(bcmi_scan(queue.get(), bcmi_dtype<OutT>(), BinaryOperator::bcmi_op, first.device_ptr(),
-                         result.device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
-                         BCMI_PRECISION_COMPENSATED));
+    // the scan computes in the input element type and converts on store
+    // (scan_on_gpu.hpp:196-205, the local scan's scratch is of the input
+    // type); init is taken in the input type as well
+    const InT ini = (InT)init;
+    if (std::is_same<InT, OutT>::value) {
+        bcmi_check(bcmi_scan(queue.get(), bcmi_dtype<InT>(), BinaryOperator::bcmi_op, first.device_ptr(),
+                             result.device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
+                             BCMI_PRECISION_COMPENSATED));
+    } else if (n) {
+        ::boost::compute::vector<InT> tmp(n, queue.get_context());
+        bcmi_check(bcmi_scan(queue.get(), bcmi_dtype<InT>(), BinaryOperator::bcmi_op, first.device_ptr(),
+                             tmp.begin().device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
+                             BCMI_PRECISION_COMPENSATED));
+        bcmi_check(bcmi_convert(queue.get(), bcmi_dtype<InT>(), bcmi_dtype<OutT>(), tmp.begin().device_ptr(),
+                                result.device_ptr(), n));
+        queue.finish();  // tmp is released on return
+    }
     return result + (std::ptrdiff_t)n;
 }
 
@@ -50,14 +63,25 @@ template <class InputIterator, class OutT, class T, class BinaryOperator>
 inline buffer_iterator<OutT> scan(InputIterator first, InputIterator last, buffer_iterator<OutT> result,
                                   bool exclusive, T init, BinaryOperator, command_queue &queue)
 {
-    static_assert(std::is_same<typename iter_desc<InputIterator>::value_type, OutT>::value,
-                  "scan: input and output element types must match");
+    typedef typename iter_desc<InputIterator>::value_type InT;
     scan_op_check<BinaryOperator>();
     const size_t n = (size_t)(last - first);
-    const OutT ini = (OutT)init;
+    const InT ini = (InT)init;
     const bcmi_iter d = iter_desc<InputIterator>::get(first);
-    bcmi_check(bcmi_scan_iter(queue.get(), bcmi_dtype<OutT>(), 1, BinaryOperator::bcmi_op, &d, result.device_ptr(),
-                              n, exclusive ? &ini : 0, exclusive ? 1 : 0, BCMI_PRECISION_COMPENSATED));
+    if (std::is_same<InT, OutT>::value) {
+        bcmi_check(bcmi_scan_iter(queue.get(), bcmi_dtype<InT>(), 1, BinaryOperator::bcmi_op, &d,
+                                  result.device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
+                                  BCMI_PRECISION_COMPENSATED));
+    } else if (n) {
+        // computed in the generated sequence's type, converted on store
+        ::boost::compute::vector<InT> tmp(n, queue.get_context());
+        bcmi_check(bcmi_scan_iter(queue.get(), bcmi_dtype<InT>(), 1, BinaryOperator::bcmi_op, &d,
+                                  tmp.begin().device_ptr(), n, exclusive ? &ini : 0, exclusive ? 1 : 0,
+                                  BCMI_PRECISION_COMPENSATED));
+        bcmi_check(bcmi_convert(queue.get(), bcmi_dtype<InT>(), bcmi_dtype<OutT>(), tmp.begin().device_ptr(),
+                                result.device_ptr(), n));
+        queue.finish();  // tmp is released on return
+    }
     return result + (std::ptrdiff_t)n;
 }
 

@@ -88,7 +88,7 @@ int main()
         float data[] = {2.0f, 0.0f, 0.0f, 0.0f, 0.0f, 3.0f, 0.0f, 0.0f, 0.0f, 0.0f, 4.0f, 0.0f};
         bc::vector<bc::float4_> vector(reinterpret_cast<bc::float4_ *>(data),
                                        reinterpret_cast<bc::float4_ *>(data) + 3, queue);
-        float product;
+        float product = 0.0f;
         bc::transform_reduce(vector.begin(), vector.end(), &product, bc::length<bc::float4_>(),
                              bc::multiplies<float>(), queue);
         CHECK(close(product, 24.0f, 1e-4f));
@@ -96,7 +96,7 @@ int main()
     {  // test_transform_reduce.cpp:62-90 mean_and_std_dev
         float data[] = {1, 2, 3, 4, 5, 6, 7, 8, 9, 10};
         bc::vector<float> vector(data, data + 10, queue);
-        float sum;
+        float sum = 0.0f;
         bc::reduce(vector.begin(), vector.end(), &sum, bc::plus<float>(), queue);
         float mean = sum / vector.size();
         CHECK(close(mean, 5.5f, 1e-4f));
@@ -148,6 +148,53 @@ int main()
         bc::copy(bc::make_constant_iterator(42, 0), bc::make_constant_iterator(42, 10), vector.begin(), queue);
         CHECK(host(vector, queue) == std::vector<int>(10, 42));
     }
+    {  // test_copy.cpp:299-331 check_copy_type: int -> short on the device
+        int data[] = {1, 2, 3, 4, 5, 6, 7, 8};
+        bc::vector<int> a(data, data + 8, queue);
+        bc::vector<short> c(8, context);
+        bc::copy(a.begin(), a.end(), c.begin(), queue);
+        CHECK(host(c, queue) == (std::vector<short>{1, 2, 3, 4, 5, 6, 7, 8}));
+        // float -> int truncates toward zero, int -> float rounds to nearest
+        float fd[] = {1.75f, -1.75f, 2.5f, -0.5f, 1e9f};
+        bc::vector<float> f(fd, fd + 5, queue);
+        bc::vector<int> fi(5, context);
+        bc::copy(f.begin(), f.end(), fi.begin(), queue);
+        CHECK(host(fi, queue) == (std::vector<int>{1, -1, 2, 0, 1000000000}));
+        int big[] = {16777217, -16777219, 3};
+        bc::vector<int> b(big, big + 3, queue);
+        bc::vector<float> bf(3, context);
+        bc::copy(b.begin(), b.end(), bf.begin(), queue);
+        CHECK(host(bf, queue) == (std::vector<float>{16777216.0f, -16777220.0f, 3.0f}));
+    }
+    {  // scans whose output type differs from the input type: computed in
+       // the input type, converted on store (scan_on_gpu.hpp:196-205)
+        int data[] = {1, 2, 3, 4, 5};
+        bc::vector<int> input(data, data + 5, queue);
+        bc::vector<float> out(5, context);
+        bc::inclusive_scan(input.begin(), input.end(), out.begin(), queue);
+        CHECK(host(out, queue) == (std::vector<float>{1, 3, 6, 10, 15}));
+        bc::exclusive_scan(input.begin(), input.end(), out.begin(), 100, queue);
+        CHECK(host(out, queue) == (std::vector<float>{100, 101, 103, 106, 110}));
+        // int sums that wrap: the int result, then converted
+        const int n = 100003;
+        std::vector<int> x(n);
+        for (int i = 0; i < n; ++i) x[i] = (int)(((unsigned)i * 2654435761u) >> 1);
+        bc::vector<int> dx(x.begin(), x.end(), queue);
+        bc::vector<double> dd(n, context);
+        bc::inclusive_scan(dx.begin(), dx.end(), dd.begin(), queue);
+        std::vector<double> hd = host(dd, queue);
+        unsigned acc = 0;
+        bool ok = true;
+        for (int i = 0; i < n; ++i) {
+            acc += (unsigned)x[i];
+            ok = ok && hd[i] == (double)(int)acc;
+        }
+        CHECK(ok);
+        // a counting range scanned into float elements
+        bc::vector<float> cf(10, context);
+        bc::inclusive_scan(bc::make_counting_iterator(1), bc::make_counting_iterator(11), cf.begin(), queue);
+        CHECK(host(cf, queue) == (std::vector<float>{1, 3, 6, 10, 15, 21, 28, 36, 45, 55}));
+    }
     {  // larger generated and mapped ranges against std:: loops
         const int n = 300007;
         bc::vector<long> out(n, context);
@@ -158,7 +205,7 @@ int main()
         for (int i = 0; i < n; ++i) e[i] = (acc += -5000L + i);
         CHECK(h == e);
         std::vector<int> x(n);
-        for (int i = 0; i < n; ++i) x[i] = (i * 7919) % 1001 - 500;
+        for (int i = 0; i < n; ++i) x[i] = (int)(((long)i * 7919) % 1001) - 500;
         bc::vector<int> dx(x.begin(), x.end(), queue), dy(n, context);
         bc::exclusive_scan(bc::make_transform_iterator(dx.begin(), bc::abs<int>()),
                            bc::make_transform_iterator(dx.end(), bc::abs<int>()), dy.begin(), 3, bc::max<int>(),
