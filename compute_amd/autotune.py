@@ -11,11 +11,11 @@ the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
 
 Tunables (the digit width is fixed at 8 bits by the per-digit LDS words;
 what varies is the tile shape and the work per thread):
-  __bcmi_radix_sort|threads, |ipt, |split, |xcd_run  keys-only radix sort: workgroup size, keys per
-                                       thread, staging rounds (4-byte keys), tiles per XCD run
-                                       (0 = dispatch order)
+  __bcmi_radix_sort|threads, |ipt, |split, |xcd_run, |lookback  keys-only radix sort: workgroup size,
+                                       keys per thread, staging rounds (4-byte keys), tiles per XCD run
+                                       (0 = dispatch order), look-back words per round trip
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
-  __bcmi_radix_hist|unroll, |subs, |blocks_per_cu  the sort's histogram kernel
+  __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_scan|q                        4-byte scans, items per lane
   __bcmi_reduce_by_key|q               reduce_by_key, items per lane
   __bcmi_segscan|q                     segmented scans, items per lane
@@ -37,7 +37,8 @@ DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 SORT_SHAPES = [(256, 64, 1), (512, 24, 1), (512, 32, 1), (1024, 16, 1), (256, 64, 2), (512, 24, 2), (512, 32, 2),
                (512, 24, 3)]
 XCD_RUNS = (0, 4, 8)  # consecutive tiles per XCD (0: tile = workgroup id)
-HIST_SHAPES = [(u, nw, b) for u in (2, 4, 8) for nw in (8, 16) for b in (2, 4)]  # (unroll, subs, blocks/CU)
+# (unroll, subs, blocks/CU, bank-grouped layout)
+HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
 
@@ -85,13 +86,16 @@ def tune_sort(ca, W, q, log2n, log):
         raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
     for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
         _set(ca, "__bcmi_radix_sort", p, v)
+    _tune_lookback(ca, q, "__bcmi_radix_sort", best, lambda: ca.sort(q, work), restore,
+                   lambda: bool(np.array_equal(work.to_host(q), expect)), log)
 
     # the histogram kernel, timed inside the tuned sort (queue profiler)
     best = None
-    for u, nw, b in HIST_SHAPES:
+    for u, nw, b, g in HIST_SHAPES:
         _set(ca, "__bcmi_radix_hist", "unroll", u)
         _set(ca, "__bcmi_radix_hist", "subs", nw)
         _set(ca, "__bcmi_radix_hist", "blocks_per_cu", b)
+        _set(ca, "__bcmi_radix_hist", "bank_groups", g)
         q.profile_reset()
         q.profile(True)
         for _ in range(5):
@@ -102,15 +106,16 @@ def tune_sort(ca, W, q, log2n, log):
         q.profile_reset()
         ms = tot / max(cnt, 1)
         ok = bool(np.array_equal(work.to_host(q), expect))
-        log({"tunable": "__bcmi_radix_hist", "unroll": u, "subs": nw, "blocks_per_cu": b, "ms": round(ms, 4),
-             "ok": ok})
+        log({"tunable": "__bcmi_radix_hist", "unroll": u, "subs": nw, "blocks_per_cu": b, "bank_groups": g,
+             "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, u, nw, b)
+            best = (ms, u, nw, b, g)
     if best is None:
         raise RuntimeError("autotune: no histogram shape produced the sorted output")
     _set(ca, "__bcmi_radix_hist", "unroll", best[1])
     _set(ca, "__bcmi_radix_hist", "subs", best[2])
     _set(ca, "__bcmi_radix_hist", "blocks_per_cu", best[3])
+    _set(ca, "__bcmi_radix_hist", "bank_groups", best[4])
 
     k, v = W.sort_by_key_f32(n)
     sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
@@ -145,6 +150,33 @@ def tune_sort(ca, W, q, log2n, log):
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
     for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
         _set(ca, "__bcmi_radix_sort_by_key", p, v)
+
+    def same_as_first():
+        return bool(np.array_equal(wk.to_host(q).view(np.uint32), first[0]) and
+                    np.array_equal(wv.to_host(q), first[1]))
+
+    _tune_lookback(ca, q, "__bcmi_radix_sort_by_key", best, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY),
+                   restore2, same_as_first, log)
+
+
+LOOKBACK_SHAPES = {(256, 64, 2), (512, 32, 2)}  # shapes built with look-back windows 2, 8, 16 besides 4
+
+
+def _tune_lookback(ca, q, key, best, run, restore, check, log):
+    """Look-back words per round trip for the chosen tile shape (the shapes
+    in LOOKBACK_SHAPES are built with 2, 4, 8 and 16)."""
+    if tuple(best[1:4]) not in LOOKBACK_SHAPES:
+        _set(ca, key, "lookback", 4)
+        return
+    lbest = None
+    for lb in (2, 4, 8, 16):
+        _set(ca, key, "lookback", lb)
+        ms = _median_ms(ca, q, run, restore)
+        ok = check()
+        log({"tunable": key, "lookback": lb, "ms": round(ms, 4), "ok": ok})
+        if ok and (lbest is None or ms < lbest[0]):
+            lbest = (ms, lb)
+    _set(ca, key, "lookback", lbest[1] if lbest else 4)
 
 
 def tune_scan(ca, W, q, log2n, log):

@@ -18,11 +18,11 @@
 //      takes position = start + peers below it, and every peer writes the
 //      same advanced start with the mask cleared.  LDS executes one wave's
 //      instructions in order, so slot j + 1 sees slot j's update.
-// The look-back (its first reads issued before ranking) then fetches the
-// global offset of each digit, the tile is
-// reordered in LDS by position (digit runs contiguous) and written out
-// coalesced: keys, then values.  Three LDS round trips per key to rank, one
-// atomic add to count, one write and two reads to reorder.
+// The look-back fetches the global offset of each digit: its first reads are
+// issued before ranking, and the tile's first staging round is written to
+// LDS (reordered by position, digit runs contiguous) while it completes.
+// Then the tile is written out coalesced.  Three LDS round trips per key to
+// rank, one atomic add to count, one write and one read to reorder.
 //
 // LDS: the count words alias the staging buffer (ranking finishes before
 // the first staging write), so a tile needs max(staged bytes, THREADS * 64)
@@ -84,30 +84,30 @@ struct Dig8 {
     BDEV uint32_t operator[](int j) const { return (w[j >> 2] >> (8 * (j & 3))) & 0xFFu; }
 };
 
-// Look-back for digit d of tile `tile` (tile > 0).  The first kQLookback
+// Look-back for digit d of tile `tile` (tile > 0).  The first LB
 // predecessor words are read early (before the tile ranks its keys, so the
 // round trip overlaps the ranking) and handed in as wd; then the
 // predecessors' aggregates are folded in tile order until an inclusive
-// prefix, kQLookback words per round trip; at a word not yet published only
-// that word is polled.  A word read early may be older than the
-// predecessor's latest state, which is harmless: an aggregate instead of an
-// inclusive prefix only sends the walk further back.  Returns the exclusive
-// prefix (a bounded wait: after kSpinLimit polls it records a fault and
-// returns what it has).
-constexpr int kQLookback = 4;
-
+// prefix, LB words per round trip; at a word not yet published only that
+// word is polled.  A word read early may be older than the predecessor's
+// latest state, which is harmless: an aggregate instead of an inclusive
+// prefix only sends the walk further back.  Returns the exclusive prefix (a
+// bounded wait: after kSpinLimit polls it records a fault and returns what
+// it has).
+template <int LB>
 BDEV void digit_lookback_issue(const uint64_t *status, uint32_t tile, uint32_t d, uint32_t epoch,
-                               uint64_t (&wd)[kQLookback])
+                               uint64_t (&wd)[LB])
 {
 #pragma unroll
-    for (int i = 0; i < kQLookback; ++i) {
+    for (int i = 0; i < LB; ++i) {
         const int64_t t = (int64_t)tile - 1 - i;
         wd[i] = t >= 0 ? lb_load(&status[(size_t)t * 256 + d]) : lb_word(epoch, LB_P, 0);
     }
 }
 
+template <int LB>
 BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, uint32_t epoch, Ctrl *ctrl,
-                             uint64_t (&wd)[kQLookback])
+                             uint64_t (&wd)[LB])
 {
     uint32_t excl = 0, spins = 0;
     int64_t kt = (int64_t)tile - 1;
@@ -115,7 +115,7 @@ BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, 
         int used = 0;
         bool found = false, stalled = false;
 #pragma unroll
-        for (int i = 0; i < kQLookback; ++i) {
+        for (int i = 0; i < LB; ++i) {
             if (found || stalled) continue;
             if (lb_epoch(wd[i]) != epoch) {
                 stalled = true;
@@ -131,11 +131,11 @@ BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, 
             while (lb_epoch(lb_load(&status[(size_t)kt * 256 + d])) != epoch)
                 if (spin_expired(ctrl, spins)) return excl;
         }
-        digit_lookback_issue(status, (uint32_t)(kt + 1), d, epoch, wd);
+        digit_lookback_issue<LB>(status, (uint32_t)(kt + 1), d, epoch, wd);
     }
 }
 
-template <int THREADS, int IPT, typename KB, int VB, int SPLIT, typename XF, bool FULL>
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, typename XF, bool FULL>
 BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
                       const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
                       uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf, Ctrl *ctrl,
@@ -217,8 +217,8 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
     }
     block_barrier();
     // the look-back's first reads, in flight while the tile ranks
-    uint64_t lbw[kQLookback];
-    if (part == 0 && tile > 0) digit_lookback_issue(status, tile, d, epoch, lbw);
+    uint64_t lbw[LB];
+    if (part == 0 && tile > 0) digit_lookback_issue<LB>(status, tile, d, epoch, lbw);
     // 3. rank
     Pos16<IPT> pos;
     const uint32_t qbit = 1u << ql, qlt = qbit - 1u;
@@ -234,6 +234,25 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
             __hip_atomic_store(a, (run + __builtin_popcount(wd & 0xFFFFu)) << 16, __ATOMIC_RELAXED,
                                __HIP_MEMORY_SCOPE_WORKGROUP);
     }
+    // 4. reorder by position (digit runs contiguous) and write out
+    // coalesced, in SPLIT rounds of H positions (keys and values of a round
+    // side by side), or, with SPLIT = 1, keys then values.  The first round
+    // is staged while the look-back completes; the count words share the
+    // staging buffer, so every sub must have finished ranking first.
+    block_barrier();
+    constexpr int H = TILE / SPLIT;
+    V *s_v = (V *)(L.stage + (SPLIT > 1 ? H * sizeof(KB) : 0));
+    auto stage_round = [&](int r) {
+#pragma unroll
+        for (int j = 0; j < IPT; ++j) {
+            const uint32_t p = pos[j] - (uint32_t)(r * H);
+            if (valid(j) && p < (uint32_t)H) {
+                s_keys[p] = k[j];
+                if constexpr (VB > 0 && SPLIT > 1) s_v[p] = v[j];
+            }
+        }
+    };
+    stage_round(0);
     // global offsets: digit d's keys go to [goff0 + excl, goff0 + excl + total),
     // checked once per digit; a tile with an inconsistent offset (a corrupted
     // chain) writes nothing and reports
@@ -241,7 +260,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
     if (part == 0) {
         uint32_t excl = 0;
         if (tile > 0) {
-            excl = digit_lookback(status, tile, d, epoch, ctrl, lbw);
+            excl = digit_lookback<LB>(status, tile, d, epoch, ctrl, lbw);
             lb_store(&status[(size_t)tile * 256 + d], lb_word(epoch, LB_P, excl + total));
         }
         L.goff[d] = goff0[d] + excl - dstart;
@@ -251,24 +270,16 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
         if (tid == 0) __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         return;
     }
-    // 4. reorder by position, write out coalesced
     const uint32_t tile_n = FULL ? (uint32_t)TILE : n - base;
     if constexpr (SPLIT > 1) {
-        // round r stages positions [r * H, r * H + H): keys, then values
-        constexpr int H = TILE / SPLIT, JR = IPT / SPLIT;
-        V *s_v = (V *)(L.stage + H * sizeof(KB));
+        constexpr int JR = IPT / SPLIT;
 #pragma unroll
         for (int r = 0; r < SPLIT; ++r) {
-            if (r > 0) block_barrier();  // the previous round's reads are done
-#pragma unroll
-            for (int j = 0; j < IPT; ++j) {
-                const uint32_t p = pos[j] - (uint32_t)(r * H);
-                if (valid(j) && p < (uint32_t)H) {
-                    s_keys[p] = k[j];
-                    if constexpr (VB > 0) s_v[p] = v[j];
-                }
+            if (r > 0) {
+                block_barrier();  // the previous round's reads are done
+                stage_round(r);
+                block_barrier();
             }
-            block_barrier();
 #pragma unroll
             for (int jj = 0; jj < JR; ++jj) {
                 const uint32_t li = jj * THREADS + tid, i = (uint32_t)(r * H) + li;
@@ -282,10 +293,6 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
         }
         return;
     }
-#pragma unroll
-    for (int j = 0; j < IPT; ++j)
-        if (valid(j)) s_keys[pos[j]] = k[j];
-    block_barrier();
     Dig8<(VB > 0 ? IPT : 1)> od;
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
@@ -328,7 +335,7 @@ BDEV uint32_t xcd_tile(uint32_t b, uint32_t xrun)
     return g * G + (r & 7) * xrun + (r >> 3);
 }
 
-template <int THREADS, int IPT, typename KB, int VB, int SPLIT, typename XF>
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, typename XF>
 __global__ __launch_bounds__(THREADS) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                     const typename ValT<VB>::type *__restrict__ vin,
                                                     typename ValT<VB>::type *__restrict__ vout, uint32_t n,
@@ -339,10 +346,10 @@ __global__ __launch_bounds__(THREADS) void k_qsweep(const KB *__restrict__ kin, 
     const uint32_t tile = xcd_tile(blockIdx.x, xrun);
     const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((uint64_t)(tile + 1) * (THREADS * IPT) <= n)
-        qsweep_body<THREADS, IPT, KB, VB, SPLIT, XF, true>(lds, kin, kout, vin, vout, n, tile, shift,
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, true>(lds, kin, kout, vin, vout, n, tile, shift,
                                                            ctrl->offsets[pass], xf, ctrl, status, epoch);
     else
-        qsweep_body<THREADS, IPT, KB, VB, SPLIT, XF, false>(lds, kin, kout, vin, vout, n, tile, shift,
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, false>(lds, kin, kout, vin, vout, n, tile, shift,
                                                             ctrl->offsets[pass], xf, ctrl, status, epoch);
 }
 

@@ -35,24 +35,43 @@ struct alignas(16) kchunk {
     KB v[N];
 };
 
-// Each group of kHistThreads / NW lanes counts into its own LDS
+// Each group of L = kHistThreads / NW lanes counts into its own LDS
 // sub-histogram: lanes adding to one bin in the same instruction serialise,
 // which matters for skewed digits (the few top-digit values of floats in a
 // narrow range); keys are read as 16-byte chunks, U per step, with the next
 // step's U chunks in flight while a step is counted.
-template <typename KB, typename XF, int U, int NW>
+//
+// BG = 0: sub-histogram s is 256 consecutive words per digit position, so
+// bin d sits in bank d mod 64 for every sub and all 64 lanes of a wave
+// contend for the banks.  BG = 1 (bank groups): the 64 / L subs of a wave
+// interleave their bins so that sub sw uses only banks [sw * L, sw * L + L)
+// (bin d at row d / L, column sw * L + d mod L of a 64-word row): the subs
+// of a wave never contend, each sub's L lanes contend among L banks.
+template <typename KB, typename XF, int U, int NW, int BG>
 __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict__ keys, uint32_t n, uint32_t head,
                                                             XF xf, uint32_t *__restrict__ partial)
 {
     constexpr int P = sizeof(KB);
-    __shared__ uint32_t s_hist[NW][P][256];
-    for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) (&s_hist[0][0][0])[i] = 0;
+    constexpr int L = kHistThreads / NW;  // lanes per sub
+    static_assert(L <= kWave && kWave % L == 0, "subs within one wave");
+    __shared__ uint32_t s_hist[NW * P * 256];
+    for (int i = threadIdx.x; i < NW * P * 256; i += kHistThreads) s_hist[i] = 0;
     block_barrier();
-    uint32_t(*h)[256] = s_hist[threadIdx.x / (kHistThreads / NW)];
+    const uint32_t sub = threadIdx.x / L;
+    // word of (position p, bin d) for this lane's sub
+    auto slot = [&](int p, uint32_t d) -> uint32_t {
+        if constexpr (BG) {
+            constexpr uint32_t SPW = kWave / L;  // subs per wave
+            const uint32_t w = sub / SPW, sw = sub % SPW;
+            return ((w * P + (uint32_t)p) * SPW * 256) + (d / L) * kWave + sw * L + d % L;
+        } else {
+            return (sub * P + (uint32_t)p) * 256 + d;
+        }
+    };
     auto count = [&](KB x) {
         KB k = xf.key(x);
 #pragma unroll
-        for (int p = 0; p < P; ++p) atomicAdd(&h[p][(k >> (8 * p)) & 0xFF], 1u);
+        for (int p = 0; p < P; ++p) atomicAdd(&s_hist[slot(p, (uint32_t)(k >> (8 * p)) & 0xFF)], 1u);
     };
     // unaligned head and the tail: block 0
     const uint64_t nchunks = (n - head) / kchunk<KB>::N;
@@ -96,29 +115,38 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
     block_barrier();
     uint32_t *dst = partial + (size_t)blockIdx.x * P * 256;
     for (int i = threadIdx.x; i < P * 256; i += kHistThreads) {
+        const uint32_t p = (uint32_t)i / 256, d = (uint32_t)i % 256;
         uint32_t t = 0;
 #pragma unroll
-        for (int w = 0; w < NW; ++w) t += (&s_hist[w][0][0])[i];
+        for (uint32_t s = 0; s < (uint32_t)NW; ++s) {
+            if constexpr (BG) {
+                constexpr uint32_t SPW = kWave / L;
+                t += s_hist[((s / SPW * P + p) * SPW * 256) + (d / L) * kWave + (s % SPW) * L + d % L];
+            } else {
+                t += s_hist[(s * P + p) * 256 + d];
+            }
+        }
         dst[i] = t;
     }
 }
 
 // The histogram's shape: chunks in flight per thread, sub-histograms per
-// block (LDS: subs * key bytes KB), blocks per CU.  Tuning cache key
-// "__bcmi_radix_hist", params "unroll" (1, 2, 4, 8), "subs" (8 or 16; 8 for
-// 8-byte keys) and "blocks_per_cu"; the defaults were the fastest measured on
-// 2^27 uint32 keys.
+// block (LDS: subs * key bytes KB), blocks per CU, bank-grouped layout.
+// Tuning cache key "__bcmi_radix_hist", params "unroll" (1, 2, 4, 8), "subs"
+// (8 or 16; 8 for 8-byte keys), "blocks_per_cu" and "bank_groups" (0, 1);
+// the defaults were the fastest measured on 2^27 uint32 keys.
 struct HistShape {
-    int unroll, subs, blocks_per_cu;
+    int unroll, subs, blocks_per_cu, bank_groups;
 };
 
 static HistShape hist_shape(size_t key_bytes, bool tuned)
 {
-    HistShape h{4, 16, 2};
+    HistShape h{4, 16, 2, 0};
     if (tuned) {
         h.unroll = (int)tune_get("__bcmi_radix_hist", "unroll", 4);
         h.subs = (int)tune_get("__bcmi_radix_hist", "subs", 16);
         h.blocks_per_cu = (int)tune_get("__bcmi_radix_hist", "blocks_per_cu", 2);
+        h.bank_groups = tune_get("__bcmi_radix_hist", "bank_groups", 0) ? 1 : 0;
     }
     if (h.unroll != 1 && h.unroll != 2 && h.unroll != 8) h.unroll = 4;
     if (h.subs != 8 || key_bytes >= 8) h.subs = key_bytes >= 8 ? 8 : 16;
@@ -130,17 +158,19 @@ template <typename KB, typename XF>
 static int launch_histogram(bcmi_queue q, const HistShape &hs, uint32_t hblocks, const KB *keys, uint32_t n,
                             uint32_t head, XF xf, uint32_t *partial)
 {
-#define BCMI_HIST(U, NW)                                                                                         \
-    if (hs.unroll == U && hs.subs == NW) {                                                                      \
+#define BCMI_HIST(U, NW, BG)                                                                                     \
+    if (hs.unroll == U && hs.subs == NW && hs.bank_groups == BG) {                                              \
         BCMI_LAUNCH(q, "radix_histogram",                                                                       \
-                    hipLaunchKernelGGL((k_histogram<KB, XF, U, NW>), dim3(hblocks), dim3(kHistThreads), 0,      \
+                    hipLaunchKernelGGL((k_histogram<KB, XF, U, NW, BG>), dim3(hblocks), dim3(kHistThreads), 0,  \
                                        q->stream, keys, n, head, xf, partial));                                 \
         return 0;                                                                                               \
     }
     if constexpr (sizeof(KB) < 8) {
-        BCMI_HIST(1, 16) BCMI_HIST(2, 16) BCMI_HIST(4, 16) BCMI_HIST(8, 16)
+        BCMI_HIST(1, 16, 0) BCMI_HIST(2, 16, 0) BCMI_HIST(4, 16, 0) BCMI_HIST(8, 16, 0)
+        BCMI_HIST(1, 16, 1) BCMI_HIST(2, 16, 1) BCMI_HIST(4, 16, 1) BCMI_HIST(8, 16, 1)
     }
-    BCMI_HIST(1, 8) BCMI_HIST(2, 8) BCMI_HIST(4, 8) BCMI_HIST(8, 8)
+    BCMI_HIST(1, 8, 0) BCMI_HIST(2, 8, 0) BCMI_HIST(4, 8, 0) BCMI_HIST(8, 8, 0)
+    BCMI_HIST(1, 8, 1) BCMI_HIST(2, 8, 1) BCMI_HIST(4, 8, 1) BCMI_HIST(8, 8, 1)
 #undef BCMI_HIST
     set_error("sort: histogram shape unroll=%d subs=%d not built", hs.unroll, hs.subs);
     return BCMI_INVALID_VALUE;
@@ -256,7 +286,7 @@ size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int c
     return align256(n * key_bytes) + align256(n * value_bytes) + align256((size_t)cu_count * 8 * key_bytes * 256 * 4);
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT, typename XF>
+template <typename KB, int VB, int TH, int IPT, int SPLIT, int LB, typename XF>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work,
                          unsigned xrun)
 {
@@ -319,7 +349,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
+                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, LB, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
                                        vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
                                        (uint32_t)(P - p), (uint32_t)xrun));
         std::swap(ks, kd);
@@ -333,19 +363,21 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     return 0;
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT = 1>
+template <typename KB, int VB, int TH, int IPT, int SPLIT = 1, int LB = 4>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
                       const SortWork *work, unsigned xrun)
 {
     return with_key_xf<KB>(kind, descending, [&](auto xf) {
-        return radix_sort_xf<KB, VB, TH, IPT, SPLIT>(q, keys, vals, n, xf, work, xrun);
+        return radix_sort_xf<KB, VB, TH, IPT, SPLIT, LB>(q, keys, vals, n, xf, work, xrun);
     });
 }
 
 // Tile shape: threads per workgroup, keys per thread and staging rounds.
 // For 4-byte keys with no or 4-byte values it comes from the tuning cache
 // (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
-// params "threads", "ipt", "split"), chosen on 2^27 keys; smaller sorts take
+// params "threads", "ipt", "split", "lookback" (predecessor words per
+// look-back round trip: 4, or 2 / 8 / 16 for the 256 x 64 and 512 x 32 tiles
+// staged in 2 rounds), chosen on 2^27 keys; smaller sorts take
 // smaller tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape
 // at every size).  Other key and value widths take a fixed shape whose
 // staging buffer stays within 32-64 KB (two workgroups per CU).
@@ -361,6 +393,7 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         unsigned th = work ? 256 : tune_get(key, "threads", 512);
         unsigned ipt = work ? 32 : tune_get(key, "ipt", 32);
         unsigned split = work ? 1 : tune_get(key, "split", 1);
+        const unsigned lb = work ? 4 : tune_get(key, "lookback", 4);
         const bool fit = work ? true : tune_get(key, "fit_size", 1) != 0;
         if (fit) {
             if (n <= (1u << 20)) {
@@ -375,6 +408,17 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         }
 #define BCMI_SHAPE(T, I, S) \
         if (th == T && ipt == I && split == S) return radix_sort<KB, VB, T, I, S>(q, keys, vals, n, kind, desc, work, xrun)
+#define BCMI_SHAPE_LB(T, I, S, W)                                                                           \
+        if (th == T && ipt == I && split == S && lb == W)                                                  \
+            return radix_sort<KB, VB, T, I, S, W>(q, keys, vals, n, kind, desc, work, xrun)
+        // look-back windows other than 4 predecessors per round trip, for the tuned shapes
+        BCMI_SHAPE_LB(256, 64, 2, 2);
+        BCMI_SHAPE_LB(256, 64, 2, 8);
+        BCMI_SHAPE_LB(256, 64, 2, 16);
+        BCMI_SHAPE_LB(512, 32, 2, 2);
+        BCMI_SHAPE_LB(512, 32, 2, 8);
+        BCMI_SHAPE_LB(512, 32, 2, 16);
+#undef BCMI_SHAPE_LB
         BCMI_SHAPE(256, 16, 1);
         BCMI_SHAPE(256, 32, 1);
         BCMI_SHAPE(256, 64, 1);

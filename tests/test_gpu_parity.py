@@ -593,10 +593,10 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
     params = ("threads", "ipt", "fit_size", "xcd_run", "split")
     defaults = {"threads": 512, "ipt": 32, "fit_size": 1, "xcd_run": 0, "split": 1}
     saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
-    hparams = ("unroll", "subs", "blocks_per_cu")
+    hparams = ("unroll", "subs", "blocks_per_cu", "bank_groups")
     hsaved = [_tune_get("__bcmi_radix_hist", p) for p in hparams]
-    hshape = {(256, 16): (1, 8, 4), (256, 32): (2, 16, 2), (256, 64): (8, 8, 2), (512, 16): (4, 16, 4),
-              (512, 24): (8, 16, 2), (512, 32): (4, 16, 2), (1024, 16): (2, 8, 8)}[(threads, ipt)]
+    hshape = {(256, 16): (1, 8, 4, 1), (256, 32): (2, 16, 2, 0), (256, 64): (8, 8, 2, 1), (512, 16): (4, 16, 4, 1),
+              (512, 24): (8, 16, 2, 0), (512, 32): (4, 16, 2, 1), (1024, 16): (2, 8, 8, 0)}[(threads, ipt)]
     xrun = {(256, 16): 2, (256, 32): 8, (256, 64): 4, (512, 16): 8, (512, 24): 4, (512, 32): 2,
             (1024, 16): 8}[(threads, ipt)]
     if split > 1:
@@ -629,8 +629,40 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
             for p, val in zip(params, vals):
                 # an absent entry reads back as the sentinel: restore the built-in default
                 ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val if val != 0xFFFFFFFF else defaults[p]))
-        for p, val, dflt in zip(hparams, hsaved, (4, 16, 2)):
+        for p, val, dflt in zip(hparams, hsaved, (4, 16, 2, 0)):
             ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val if val != 0xFFFFFFFF else dflt))
+
+
+@pytest.mark.parametrize("threads,ipt,lb", [(256, 64, 2), (256, 64, 8), (256, 64, 16), (512, 32, 2), (512, 32, 8),
+                                            (512, 32, 16)])
+def test_sort_lookback_window_variants(queue, threads, ipt, lb):
+    """The look-back windows the tuner may select for the 2-round tiles:
+    keys and pairs, many tiles (the walk crosses windows), XCD-grouped."""
+    L = ca.lib()
+    params = ("threads", "ipt", "split", "lookback", "xcd_run", "fit_size")
+    keys = ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")
+    saved = {k: [_tune_get(k, p) for p in params] for k in keys}
+    defaults = {"threads": 512, "ipt": 32, "split": 1, "lookback": 4, "xcd_run": 0, "fit_size": 1}
+    try:
+        for key in keys:
+            for p, val in zip(params, (threads, ipt, 2, lb, 8, 0)):
+                ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val))
+        for n in (70001, 5000011):
+            k = random_keys("uint32", n)
+            d = dev(queue, k)
+            ca.sort(queue, d, api=ca.RADIX_SORT)
+            assert np.array_equal(d.to_host(queue), np.sort(k)), n
+            kf = random_keys("float32", n)
+            v = np.arange(n, dtype=np.uint32)
+            dk, dv = dev(queue, kf), dev(queue, v)
+            ca.sort(queue, dk, dv, api=ca.SORT_BY_KEY)
+            ek, ev = O.radix_order_sort(kf, v)
+            assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), n
+            assert np.array_equal(dv.to_host(queue), ev), n
+    finally:
+        for key, vals in saved.items():
+            for p, val in zip(params, vals):
+                ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val if val != 0xFFFFFFFF else defaults[p]))
 
 
 # ========================================================= segmented sort
