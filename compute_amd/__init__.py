@@ -23,6 +23,7 @@ PLUS, MULTIPLIES, MIN, MAX, MINUS, DIVIDES = range(6)
 SORT, SORT_BY_KEY, STABLE_SORT, STABLE_SORT_BY_KEY, RADIX_SORT = range(5)
 COMPENSATED, NAIVE = 0, 1
 H2D, D2H, D2D = 1, 2, 3
+OUT_OF_RESOURCES, INVALID_VALUE, INVALID_BUFFER_SIZE = -5, -30, -61
 
 DTYPE = {np.dtype(np.int8): I8, np.dtype(np.uint8): U8, np.dtype(np.int16): I16,
          np.dtype(np.uint16): U16, np.dtype(np.int32): I32, np.dtype(np.uint32): U32,
@@ -42,7 +43,7 @@ EXPORTS = [
     "bcmi_tuning_cache_save", "bcmi_profile_enable", "bcmi_profile_query", "bcmi_profile_reset",
     "bcmi_copy_if", "bcmi_count_if", "bcmi_unique", "bcmi_partition", "bcmi_transform_reduce",
     "bcmi_extract_keys", "bcmi_host_malloc", "bcmi_host_free", "bcmi_merge_sort", "bcmi_scan_iter",
-    "bcmi_transform_iter", "bcmi_convert",
+    "bcmi_transform_iter", "bcmi_convert", "bcmi_debug_fault_probe",
 ]
 
 SELECT_VALUES, SELECT_INDEX_U32, SELECT_INDEX_U64 = 0, 1, 2
@@ -215,6 +216,7 @@ def lib():
         L.bcmi_scan_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), vp, sz, vp, i, i]
         L.bcmi_transform_iter.argtypes = [vp, i, i, i, C.POINTER(Iter), C.POINTER(Iter), vp, sz]
         L.bcmi_convert.argtypes = [vp, i, i, vp, vp, sz]
+        L.bcmi_debug_fault_probe.argtypes = [vp, C.c_uint]
         L.bcmi_host_malloc.argtypes = [sz, C.POINTER(vp)]
         L.bcmi_host_free.argtypes = [vp]
         L.bcmi_transform_reduce.argtypes = [vp, i, C.POINTER(Seq), C.POINTER(Seq), sz, C.POINTER(Map), i, vp, vp,

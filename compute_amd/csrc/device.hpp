@@ -188,14 +188,16 @@ BDEV uint32_t launch_epoch_plain(const Ctrl *ctrl, uint32_t back)
 // and is resident or finished: the look-back wait always makes progress.
 
 // Every look-back wait is bounded: after kSpinLimit polls (each a device-
-// coherent load plus s_sleep 1; on the order of a second in total) the wait
-// gives up, records ctrl->fault and proceeds, so a protocol error can never
-// hang the device.
+// coherent load plus s_sleep 1; on the order of a second in total), or
+// ctrl->spin_limit polls where a test has set one, the wait gives up,
+// records ctrl->fault and proceeds, so a protocol error can never hang the
+// device.
 constexpr uint32_t kSpinLimit = 1u << 20;
 
 BDEV bool spin_expired(Ctrl *ctrl, uint32_t &spins)
 {
-    if (++spins < kSpinLimit) {
+    const uint32_t limit = ctrl->spin_limit ? ctrl->spin_limit : kSpinLimit;
+    if (++spins < limit) {
         __builtin_amdgcn_s_sleep(1);
         return false;
     }

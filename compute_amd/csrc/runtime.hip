@@ -1,6 +1,8 @@
 // runtime.hip -- C-ABI runtime: devices, queues, memory, events, graphs,
 // profiling and the tuning cache.
 #include "runtime.hpp"
+#include "device.hpp"
+#include "fault.hpp"
 
 #include <cstdarg>
 #include <cstdio>
@@ -302,15 +304,39 @@ int bcmi_queue_finish(bcmi_queue q)
     uint32_t *fault = (uint32_t *)q->pinned;
     BCMI_HIP(hipMemcpyAsync(fault, &q->ctrl->fault, sizeof(uint32_t), hipMemcpyDeviceToHost, q->stream));
     BCMI_HIP(hipStreamSynchronize(q->stream));
-    if (*fault) {
-        uint32_t f = *fault;
+    const char *msg;
+    const int status = fault_report(*fault, &msg);
+    if (status) {
         *fault = 0;
         BCMI_HIP(hipMemcpyAsync(&q->ctrl->fault, fault, sizeof(uint32_t), hipMemcpyHostToDevice, q->stream));
         BCMI_HIP(hipStreamSynchronize(q->stream));
-        set_error(f == 1 ? "a look-back wait exceeded its spin limit; results of the last calls are invalid"
-                         : "an output index fell outside the output range and was dropped; results are invalid");
-        return BCMI_OUT_OF_RESOURCES;
+        set_error("%s", msg);
     }
+    return status;
+}
+
+// The bounded wait every look-back uses, on a word nothing ever sets, with
+// the queue's spin limit lowered for the launch: the wait gives up and
+// records kFaultSpin, and every kernel exits normally.
+__global__ void k_fault_probe(Ctrl *ctrl)
+{
+    uint32_t spins = 0;
+    while (__hip_atomic_load(&ctrl->never_set, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT) == 0)
+        if (spin_expired(ctrl, spins)) break;
+}
+
+__global__ void k_set_spin_limit(Ctrl *ctrl, uint32_t limit) { ctrl->spin_limit = limit; }
+
+int bcmi_debug_fault_probe(bcmi_queue q, unsigned spin_limit)
+{
+    BCMI_TRY(use_queue(q));
+    BCMI_CHECK(spin_limit >= 1 && spin_limit <= (1u << 16), BCMI_INVALID_VALUE,
+               "fault probe: spin limit %u outside [1, 65536]", spin_limit);
+    BCMI_LAUNCH(q, "set_spin_limit",
+                hipLaunchKernelGGL(k_set_spin_limit, dim3(1), dim3(1), 0, q->stream, q->ctrl, (uint32_t)spin_limit));
+    BCMI_LAUNCH(q, "fault_probe", hipLaunchKernelGGL(k_fault_probe, dim3(1), dim3(64), 0, q->stream, q->ctrl));
+    BCMI_LAUNCH(q, "set_spin_limit",
+                hipLaunchKernelGGL(k_set_spin_limit, dim3(1), dim3(1), 0, q->stream, q->ctrl, 0u));
     return 0;
 }
 

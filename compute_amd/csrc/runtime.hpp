@@ -35,7 +35,8 @@ namespace bcmi {
 struct Ctrl {
     uint32_t epoch;          // look-back epoch counter (advanced by k_epoch_advance)
     uint32_t fault;          // set by a kernel whose look-back wait expired or whose output index was out of range
-    uint32_t pad[2];
+    uint32_t spin_limit;     // polls before a look-back wait gives up; 0: kSpinLimit (tests lower it)
+    uint32_t never_set;      // written by no kernel and no host code: the fault probe's wait target
     uint32_t offsets[8][256];// exclusive digit offsets per radix pass
 };
 

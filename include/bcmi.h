@@ -103,8 +103,16 @@ int bcmi_device_get_info(int device, bcmi_device_info *info);
 typedef struct bcmi_queue_s *bcmi_queue;
 int bcmi_queue_create(int device, bcmi_queue *queue);
 int bcmi_queue_destroy(bcmi_queue queue);
-/* command_queue::finish (command_queue.hpp:1564): blocking */
+/* command_queue::finish (command_queue.hpp:1564): blocking.  Returns
+ * BCMI_OUT_OF_RESOURCES when a kernel since the previous finish gave up a
+ * bounded look-back wait or dropped an out-of-range output (the results of
+ * those calls are invalid; the fault is cleared). */
 int bcmi_queue_finish(bcmi_queue queue);
+/* Test support, no counterpart in the reference: runs the bounded wait the
+ * look-backs use on a word nothing sets, with the queue's spin limit lowered
+ * to spin_limit polls (1..65536) for that launch, so the next
+ * bcmi_queue_finish reports the give-up path.  No kernel faults or hangs. */
+int bcmi_debug_fault_probe(bcmi_queue queue, unsigned spin_limit);
 int bcmi_queue_device(bcmi_queue queue, int *device);
 /* the underlying hipStream_t, for interop */
 void *bcmi_queue_native(bcmi_queue queue);

@@ -9,6 +9,7 @@
 #include <cstdio>
 #include <numeric>
 #include <random>
+#include <string>
 #include <vector>
 
 namespace bc = boost::compute;
@@ -565,6 +566,36 @@ int main()
             thrown = e.error_code() == -1;
         }
         CHECK(thrown);
+    }
+    {  // a kernel that gives up a bounded look-back wait: the next finish
+       // raises CL_OUT_OF_RESOURCES, the one after succeeds, and the queue
+       // keeps working (the probe's lowered spin limit is restored)
+        queue.finish();
+        CHECK(bcmi_debug_fault_probe(queue.get(), 64) == 0);
+        int code = 0;
+        std::string what;
+        try {
+            queue.finish();
+        } catch (const bc::opencl_error &e) {
+            code = e.error_code();
+            what = e.what();
+        }
+        CHECK(code == -5 && what.find("spin limit") != std::string::npos);
+        bool clean = true;
+        try {
+            queue.finish();
+        } catch (const bc::opencl_error &) {
+            clean = false;
+        }
+        CHECK(clean);
+        std::vector<int> h(100000);
+        for (size_t i = 0; i < h.size(); ++i) h[i] = (int)((i * 2654435761u) % 100003u) - 50000;
+        bc::vector<int> d(h.begin(), h.end(), queue);
+        bc::sort(d.begin(), d.end(), queue);
+        std::vector<int> got(h.size());
+        bc::copy(d.begin(), d.end(), got.begin(), queue);
+        std::sort(h.begin(), h.end());
+        CHECK(got == h);
     }
     queue.finish();
     std::printf("%d failures\n", failures);
