@@ -11,12 +11,13 @@ the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
 
 Tunables (the digit width is fixed at 8 bits by the per-digit LDS words;
 what varies is the tile shape and the work per thread):
-  __bcmi_radix_sort|threads, |ipt, |split, |xcd_run, |lookback  keys-only radix sort: workgroup size,
-                                       keys per thread, staging rounds (4-byte keys), tiles per XCD run
-                                       (0 = dispatch order), look-back words per round trip
+  __bcmi_radix_sort|threads, |ipt, |split, |lookback, |waves, |xcd_run  keys-only radix sort: workgroup
+                                       size, keys per thread, staging rounds (4-byte keys), look-back words
+                                       per round trip, register bound (waves per SIMD), tiles per XCD run
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_scan|q                        4-byte scans, items per lane
+  __bcmi_scan_f32|q                    the compensated float scan, items per lane
   __bcmi_reduce_by_key|q               reduce_by_key, items per lane
   __bcmi_segscan|q                     segmented scans, items per lane
   __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
@@ -33,11 +34,13 @@ import numpy as np
 HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
-# (threads, keys per thread, staging rounds)
-SORT_SHAPES = [(256, 64, 1), (512, 24, 1), (512, 32, 1), (1024, 16, 1), (256, 64, 2), (512, 24, 2), (512, 32, 2),
-               (512, 24, 3)]
-XCD_RUNS = (0, 4, 8)  # consecutive tiles per XCD (0: tile = workgroup id)
-# (unroll, subs, blocks/CU, bank-grouped layout)
+# (threads, keys per thread, staging rounds, look-back words per round trip,
+# register bound in waves per SIMD (0: none)): the combinations built
+SORT_SHAPES = [(256, 64, 1, 4, 0), (512, 24, 1, 4, 0), (512, 32, 1, 4, 0), (1024, 16, 1, 4, 0), (256, 64, 2, 4, 0),
+               (512, 24, 2, 4, 0), (512, 32, 2, 4, 0), (512, 24, 3, 4, 0), (256, 64, 2, 2, 0), (512, 32, 2, 2, 0),
+               (256, 64, 2, 2, 4), (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8)]
+SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "xcd_run")
+XCD_RUNS = (4, 8)  # consecutive tiles per XCD (0, dispatch order, measured slower on every shape)
 HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]
 SCAN_Q = (4, 8, 16)
 SEGSCAN_Q = (2, 4, 8)
@@ -69,25 +72,10 @@ def tune_sort(ca, W, q, log2n, log):
     src = ca.DeviceArray.from_host(keys, q)
     work = ca.DeviceArray(n, np.uint32, q)
     restore = lambda: ca.check(L.bcmi_memcpy(q.h, work.ptr, src.ptr, src.nbytes, ca.D2D, 0))
-    best = None
-    for th, ipt, sp in SORT_SHAPES:
-        for xr in XCD_RUNS:
-            _set(ca, "__bcmi_radix_sort", "threads", th)
-            _set(ca, "__bcmi_radix_sort", "ipt", ipt)
-            _set(ca, "__bcmi_radix_sort", "split", sp)
-            _set(ca, "__bcmi_radix_sort", "xcd_run", xr)
-            ms = _median_ms(ca, q, lambda: ca.sort(q, work), restore)
-            ok = bool(np.array_equal(work.to_host(q), expect))
-            log({"tunable": "__bcmi_radix_sort", "threads": th, "ipt": ipt, "split": sp, "xcd_run": xr,
-                 "ms": round(ms, 4), "ok": ok})
-            if ok and (best is None or ms < best[0]):
-                best = (ms, th, ipt, sp, xr)
+    best = _tune_shapes(ca, q, "__bcmi_radix_sort", lambda: ca.sort(q, work), restore,
+                        lambda: bool(np.array_equal(work.to_host(q), expect)), log)
     if best is None:
         raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
-    for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
-        _set(ca, "__bcmi_radix_sort", p, v)
-    _tune_lookback(ca, q, "__bcmi_radix_sort", best, lambda: ca.sort(q, work), restore,
-                   lambda: bool(np.array_equal(work.to_host(q), expect)), log)
 
     # the histogram kernel, timed inside the tuned sort (queue profiler)
     best = None
@@ -125,58 +113,44 @@ def tune_sort(ca, W, q, log2n, log):
         ca.check(L.bcmi_memcpy(q.h, wk.ptr, sk.ptr, sk.nbytes, ca.D2D, 0))
         ca.check(L.bcmi_memcpy(q.h, wv.ptr, sv.ptr, sv.nbytes, ca.D2D, 0))
 
-    first = None
-    best = None
-    for th, ipt, sp, xr in [(t, i, s, x) for t, i, s in SORT_SHAPES for x in XCD_RUNS]:
-        _set(ca, "__bcmi_radix_sort_by_key", "threads", th)
-        _set(ca, "__bcmi_radix_sort_by_key", "ipt", ipt)
-        _set(ca, "__bcmi_radix_sort_by_key", "split", sp)
-        _set(ca, "__bcmi_radix_sort_by_key", "xcd_run", xr)
-        ms = _median_ms(ca, q, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2)
+    first = []
+
+    def check_pairs():
         got = (wk.to_host(q).view(np.uint32), wv.to_host(q))
-        if first is None:
+        if not first:
             # stable order by the radix key: the payload is the input index,
             # so ties must keep increasing indices
             rk = got[0] ^ np.where(got[0] >> 31, np.uint32(0xFFFFFFFF), np.uint32(0x80000000))
             ok = bool(np.all(rk[1:] >= rk[:-1]) and np.all((rk[1:] > rk[:-1]) | (got[1][1:] > got[1][:-1])))
-            first = got
-        else:
-            ok = bool(np.array_equal(got[0], first[0]) and np.array_equal(got[1], first[1]))
-        log({"tunable": "__bcmi_radix_sort_by_key", "threads": th, "ipt": ipt, "split": sp, "xcd_run": xr,
-             "ms": round(ms, 4), "ok": ok})
-        if ok and (best is None or ms < best[0]):
-            best = (ms, th, ipt, sp, xr)
+            if ok:
+                first.append(got)
+            return ok
+        return bool(np.array_equal(got[0], first[0][0]) and np.array_equal(got[1], first[0][1]))
+
+    best = _tune_shapes(ca, q, "__bcmi_radix_sort_by_key", lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2,
+                        check_pairs, log)
     if best is None:
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
-    for p, v in zip(("threads", "ipt", "split", "xcd_run"), best[1:]):
-        _set(ca, "__bcmi_radix_sort_by_key", p, v)
-
-    def same_as_first():
-        return bool(np.array_equal(wk.to_host(q).view(np.uint32), first[0]) and
-                    np.array_equal(wv.to_host(q), first[1]))
-
-    _tune_lookback(ca, q, "__bcmi_radix_sort_by_key", best, lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY),
-                   restore2, same_as_first, log)
 
 
-LOOKBACK_SHAPES = {(256, 64, 2), (512, 32, 2)}  # shapes built with look-back windows 2, 8, 16 besides 4
-
-
-def _tune_lookback(ca, q, key, best, run, restore, check, log):
-    """Look-back words per round trip for the chosen tile shape (the shapes
-    in LOOKBACK_SHAPES are built with 2, 4, 8 and 16)."""
-    if tuple(best[1:4]) not in LOOKBACK_SHAPES:
-        _set(ca, key, "lookback", 4)
-        return
-    lbest = None
-    for lb in (2, 4, 8, 16):
-        _set(ca, key, "lookback", lb)
-        ms = _median_ms(ca, q, run, restore)
-        ok = check()
-        log({"tunable": key, "lookback": lb, "ms": round(ms, 4), "ok": ok})
-        if ok and (lbest is None or ms < lbest[0]):
-            lbest = (ms, lb)
-    _set(ca, key, "lookback", lbest[1] if lbest else 4)
+def _tune_shapes(ca, q, key, run, restore, check, log):
+    """Every built (tile shape, look-back window, register bound) x XCD run
+    for one sort; keeps and sets the fastest correct one."""
+    best = None
+    for shape in SORT_SHAPES:
+        for xr in XCD_RUNS:
+            vals = shape + (xr,)
+            for p, v in zip(SORT_PARAMS, vals):
+                _set(ca, key, p, v)
+            ms = _median_ms(ca, q, run, restore)
+            ok = check()
+            log(dict({"tunable": key}, **dict(zip(SORT_PARAMS, vals)), ms=round(ms, 4), ok=ok))
+            if ok and (best is None or ms < best[0]):
+                best = (ms, vals)
+    if best is not None:
+        for p, v in zip(SORT_PARAMS, best[1]):
+            _set(ca, key, p, v)
+    return best
 
 
 def tune_scan(ca, W, q, log2n, log):
@@ -198,6 +172,25 @@ def tune_scan(ca, W, q, log2n, log):
     if best is None:
         raise RuntimeError("autotune: no scan candidate matched")
     _set(ca, "__bcmi_scan", "q", best[1])
+
+    # the compensated float inclusive scan, within the double-double bound
+    xf = W.lognormal_f32(n)
+    dd = W.scan_dd(xf)
+    bound = np.spacing(np.abs(dd).astype(np.float32)).astype(np.float64) + \
+        2.0 ** -50 * np.cumsum(np.abs(xf.astype(np.float64)))
+    dxf = ca.DeviceArray.from_host(xf, q)
+    dof = ca.DeviceArray(n, np.float32, q)
+    best = None
+    for qq in SCAN_Q:
+        _set(ca, "__bcmi_scan_f32", "q", qq)
+        ms = _median_ms(ca, q, lambda: ca.scan(q, dxf, dof, False))
+        ok = bool(np.all(np.abs(dof.to_host(q).astype(np.float64) - dd) <= bound))
+        log({"tunable": "__bcmi_scan_f32", "q": qq, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, qq)
+    if best is None:
+        raise RuntimeError("autotune: no compensated float scan candidate within the bound")
+    _set(ca, "__bcmi_scan_f32", "q", best[1])
 
 
 REDUCE_CANDIDATES = [(2, 4), (4, 4), (8, 4), (2, 8), (4, 8), (8, 8)]  # (unroll, blocks per CU)

@@ -54,6 +54,22 @@ struct QSweepLds {
     uint32_t wsum[4];
 };
 
+// x, with the compiler made to forget how it was computed (so values
+// derived from it earlier are recomputed, not kept alive in registers).
+template <typename KB>
+BDEV KB opaque_reg(KB x)
+{
+    if constexpr (sizeof(KB) <= 4) {
+        uint32_t u = (uint32_t)x;
+        asm volatile("" : "+v"(u));
+        return (KB)u;
+    } else {
+        uint64_t u = (uint64_t)x;
+        asm volatile("" : "+v"(u));
+        return (KB)u;
+    }
+}
+
 // Tile positions (below 2^16), two per register.
 template <int IPT>
 struct Pos16 {
@@ -225,7 +241,10 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
 #pragma unroll
     for (int j = 0; j < IPT; ++j) {
         const bool ok = valid(j);
-        uint32_t *a = &my[xf.digit(k[j], shift)];
+        // the digit is recomputed here rather than kept from the counting
+        // loop: keeping all IPT word addresses alive across the scan would
+        // cost IPT more VGPRs per lane (occupancy is register-bound)
+        uint32_t *a = &my[xf.digit(opaque_reg(k[j]), shift)];
         if (ok) __hip_atomic_fetch_or(a, qbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint32_t wd = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint32_t run = wd >> 16;
@@ -335,8 +354,12 @@ BDEV uint32_t xcd_tile(uint32_t b, uint32_t xrun)
     return g * G + (r & 7) * xrun + (r >> 3);
 }
 
-template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, typename XF>
-__global__ __launch_bounds__(THREADS) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
+// WPE: the waves per SIMD the register allocation must leave room for (1 =
+// no bound beyond the block size).  Occupancy of this kernel is bound by
+// its VGPRs: a bound of 4 fits four 256-thread workgroups per CU in 128
+// VGPRs, 8 fits four 512-thread workgroups in 64.
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, int WPE, typename XF>
+__global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                     const typename ValT<VB>::type *__restrict__ vin,
                                                     typename ValT<VB>::type *__restrict__ vout, uint32_t n,
                                                     uint32_t shift, uint32_t pass, XF xf, Ctrl *ctrl,

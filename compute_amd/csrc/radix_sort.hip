@@ -286,7 +286,7 @@ size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int c
     return align256(n * key_bytes) + align256(n * value_bytes) + align256((size_t)cu_count * 8 * key_bytes * 256 * 4);
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT, int LB, typename XF>
+template <typename KB, int VB, int TH, int IPT, int SPLIT, int LB, int WPE, typename XF>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work,
                          unsigned xrun)
 {
@@ -349,7 +349,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, LB, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
+                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, LB, WPE, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
                                        vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
                                        (uint32_t)(P - p), (uint32_t)xrun));
         std::swap(ks, kd);
@@ -363,12 +363,12 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     return 0;
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT = 1, int LB = 4>
+template <typename KB, int VB, int TH, int IPT, int SPLIT = 1, int LB = 4, int WPE = 1>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
                       const SortWork *work, unsigned xrun)
 {
     return with_key_xf<KB>(kind, descending, [&](auto xf) {
-        return radix_sort_xf<KB, VB, TH, IPT, SPLIT, LB>(q, keys, vals, n, xf, work, xrun);
+        return radix_sort_xf<KB, VB, TH, IPT, SPLIT, LB, WPE>(q, keys, vals, n, xf, work, xrun);
     });
 }
 
@@ -376,8 +376,9 @@ static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind
 // For 4-byte keys with no or 4-byte values it comes from the tuning cache
 // (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
 // params "threads", "ipt", "split", "lookback" (predecessor words per
-// look-back round trip: 4, or 2 / 8 / 16 for the 256 x 64 and 512 x 32 tiles
-// staged in 2 rounds), chosen on 2^27 keys; smaller sorts take
+// look-back round trip) and "waves" (a register bound, waves per SIMD; 0 =
+// none), chosen on 2^27 keys; the lookback / waves combinations built are
+// listed below; smaller sorts take
 // smaller tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape
 // at every size).  Other key and value widths take a fixed shape whose
 // staging buffer stays within 32-64 KB (two workgroups per CU).
@@ -394,31 +395,39 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         unsigned ipt = work ? 32 : tune_get(key, "ipt", 32);
         unsigned split = work ? 1 : tune_get(key, "split", 1);
         const unsigned lb = work ? 4 : tune_get(key, "lookback", 4);
-        const bool fit = work ? true : tune_get(key, "fit_size", 1) != 0;
-        if (fit) {
+        const unsigned wpe = work ? 0 : tune_get(key, "waves", 0);
+        bool tuned_shape = true;
+        if (work || tune_get(key, "fit_size", 1) != 0) {
             if (n <= (1u << 20)) {
                 th = 256;
                 ipt = 16;
                 split = 1;
+                tuned_shape = false;
             } else if (n <= (1u << 23)) {
                 th = 256;
                 ipt = 32;
                 split = 1;
+                tuned_shape = false;
             }
         }
+        // register bounds (param "waves": waves per SIMD) that raise the
+        // workgroups per CU to what the LDS allows without spills, and
+        // look-back windows other than 4 words per round trip, for the
+        // tuned shapes; any other combination takes the plain shape
+#define BCMI_SHAPE_LB(T, I, S, W, E)                                                                        \
+        if (th == T && ipt == I && split == S && lb == W && wpe == E)                                      \
+            return radix_sort<KB, VB, T, I, S, W, E>(q, keys, vals, n, kind, desc, work, xrun)
+        if (tuned_shape) {
+            BCMI_SHAPE_LB(256, 64, 2, 2, 0);
+            BCMI_SHAPE_LB(512, 32, 2, 2, 0);
+            BCMI_SHAPE_LB(256, 64, 2, 2, 4);
+            BCMI_SHAPE_LB(512, 24, 2, 2, 8);
+            BCMI_SHAPE_LB(512, 32, 2, 2, 7);
+            BCMI_SHAPE_LB(512, 16, 1, 4, 8);
+        }
+#undef BCMI_SHAPE_LB
 #define BCMI_SHAPE(T, I, S) \
         if (th == T && ipt == I && split == S) return radix_sort<KB, VB, T, I, S>(q, keys, vals, n, kind, desc, work, xrun)
-#define BCMI_SHAPE_LB(T, I, S, W)                                                                           \
-        if (th == T && ipt == I && split == S && lb == W)                                                  \
-            return radix_sort<KB, VB, T, I, S, W>(q, keys, vals, n, kind, desc, work, xrun)
-        // look-back windows other than 4 predecessors per round trip, for the tuned shapes
-        BCMI_SHAPE_LB(256, 64, 2, 2);
-        BCMI_SHAPE_LB(256, 64, 2, 8);
-        BCMI_SHAPE_LB(256, 64, 2, 16);
-        BCMI_SHAPE_LB(512, 32, 2, 2);
-        BCMI_SHAPE_LB(512, 32, 2, 8);
-        BCMI_SHAPE_LB(512, 32, 2, 16);
-#undef BCMI_SHAPE_LB
         BCMI_SHAPE(256, 16, 1);
         BCMI_SHAPE(256, 32, 1);
         BCMI_SHAPE(256, 64, 1);

@@ -633,19 +633,19 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
             ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val if val != 0xFFFFFFFF else dflt))
 
 
-@pytest.mark.parametrize("threads,ipt,lb", [(256, 64, 2), (256, 64, 8), (256, 64, 16), (512, 32, 2), (512, 32, 8),
-                                            (512, 32, 16)])
-def test_sort_lookback_window_variants(queue, threads, ipt, lb):
-    """The look-back windows the tuner may select for the 2-round tiles:
+@pytest.mark.parametrize("threads,ipt,split,lb,waves", [(256, 64, 2, 2, 0), (512, 32, 2, 2, 0), (256, 64, 2, 2, 4),
+                                                       (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8)])
+def test_sort_lookback_and_register_bound_variants(queue, threads, ipt, split, lb, waves):
+    """The look-back windows and register bounds the tuner may select:
     keys and pairs, many tiles (the walk crosses windows), XCD-grouped."""
     L = ca.lib()
-    params = ("threads", "ipt", "split", "lookback", "xcd_run", "fit_size")
+    params = ("threads", "ipt", "split", "lookback", "waves", "xcd_run", "fit_size")
     keys = ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")
     saved = {k: [_tune_get(k, p) for p in params] for k in keys}
-    defaults = {"threads": 512, "ipt": 32, "split": 1, "lookback": 4, "xcd_run": 0, "fit_size": 1}
+    defaults = {"threads": 512, "ipt": 32, "split": 1, "lookback": 4, "waves": 0, "xcd_run": 0, "fit_size": 1}
     try:
         for key in keys:
-            for p, val in zip(params, (threads, ipt, 2, lb, 8, 0)):
+            for p, val in zip(params, (threads, ipt, split, lb, waves, 8, 0)):
                 ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val))
         for n in (70001, 5000011):
             k = random_keys("uint32", n)
