@@ -17,10 +17,13 @@ what varies is the tile shape and the work per thread):
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_scan|q                        4-byte scans, items per lane
-  __bcmi_scan_f32|q                    the compensated float scan, items per lane
-  __bcmi_reduce_by_key|q               reduce_by_key, items per lane
-  __bcmi_segscan|q                     segmented scans, items per lane
+  __bcmi_scan_f32|q, |acc              the compensated float scan: items per lane, accumulators
+                                       (1: FP64, 0: TwoSum FP32 pairs)
+  __bcmi_reduce_by_key|q, |acc         reduce_by_key: items per lane, float accumulators (as above)
+  __bcmi_segscan|q, |acc               segmented scans: items per lane, float accumulators
   __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
+  __bcmi_segsort|small_max, |msd       segmented sort: largest LDS-sorted class (capacity 256 << c, also
+                                       the MSD group size), MSD route for large segments on or off
 
     python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
 """
@@ -181,16 +184,55 @@ def tune_scan(ca, W, q, log2n, log):
     dxf = ca.DeviceArray.from_host(xf, q)
     dof = ca.DeviceArray(n, np.float32, q)
     best = None
-    for qq in SCAN_Q:
+    for acc, qq in [(a, b) for a in (0, 1) for b in SCAN_Q]:
+        _set(ca, "__bcmi_scan_f32", "acc", acc)
         _set(ca, "__bcmi_scan_f32", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.scan(q, dxf, dof, False))
         ok = bool(np.all(np.abs(dof.to_host(q).astype(np.float64) - dd) <= bound))
-        log({"tunable": "__bcmi_scan_f32", "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_scan_f32", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, qq)
+            best = (ms, acc, qq)
     if best is None:
         raise RuntimeError("autotune: no compensated float scan candidate within the bound")
-    _set(ca, "__bcmi_scan_f32", "q", best[1])
+    _set(ca, "__bcmi_scan_f32", "acc", best[1])
+    _set(ca, "__bcmi_scan_f32", "q", best[2])
+
+
+SEGSORT_CANDIDATES = [(4, 1), (3, 1), (2, 1), (4, 0), (3, 0)]  # (largest small class, MSD route)
+
+
+def tune_segsort(ca, W, q, nseg, log):
+    """The batched segmented sort's routing thresholds, on the benchmark's
+    lognormal segment lengths (uint64 keys, uint32 values): every candidate
+    must reproduce the first candidate's output bit for bit."""
+    off = W.segments(nseg)
+    n = int(off[-1])
+    k, v = W.segment_data(n)
+    sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
+    wk, wv = ca.DeviceArray(n, np.uint64, q), ca.DeviceArray(n, np.uint32, q)
+    L = ca.lib()
+
+    def restore():
+        ca.check(L.bcmi_memcpy(q.h, wk.ptr, sk.ptr, sk.nbytes, ca.D2D, 0))
+        ca.check(L.bcmi_memcpy(q.h, wv.ptr, sv.ptr, sv.nbytes, ca.D2D, 0))
+
+    ref = None
+    best = None
+    for cls, msd in SEGSORT_CANDIDATES:
+        _set(ca, "__bcmi_segsort", "small_max", cls)
+        _set(ca, "__bcmi_segsort", "msd", msd)
+        plan = ca.SegmentedPlan(q, np.uint64, np.uint32, off)
+        ms = _median_ms(ca, q, lambda: plan.run(q, wk, wv), restore)
+        got = (wk.to_host(q), wv.to_host(q))
+        if ref is None:
+            ref = got
+        ok = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
+        log({"tunable": "__bcmi_segsort", "small_max": cls, "msd": msd, "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, cls, msd)
+        del plan
+    _set(ca, "__bcmi_segsort", "small_max", best[1])
+    _set(ca, "__bcmi_segsort", "msd", best[2])
 
 
 REDUCE_CANDIDATES = [(2, 4), (4, 4), (8, 4), (2, 8), (4, 8), (8, 8)]  # (unroll, blocks per CU)
@@ -238,19 +280,21 @@ def tune_segscan(ca, W, q, log2n, log):
     ko, vo = ca.DeviceArray(n, np.int32, q), ca.DeviceArray(n, np.float32, q)
     cnt = ca.DeviceArray(1, np.uint64, q)
     best = None
-    for qq in SEGSCAN_Q:
+    for acc, qq in [(a, b) for a in (0, 1) for b in SEGSCAN_Q]:
+        _set(ca, "__bcmi_reduce_by_key", "acc", acc)
         _set(ca, "__bcmi_reduce_by_key", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.check(L.bcmi_reduce_by_key(q.h, ca.I32, ca.F32, ca.PLUS, dk.ptr, dv.ptr, n,
                                                                         ko.ptr, vo.ptr, cnt.ptr, 0)))
         ok = int(cnt.to_host(q)[0]) == runs and bool(np.array_equal(ko.to_host(q, runs), last_keys))
         got = vo.to_host(q, runs).astype(np.float64)
         ok = ok and bool(np.all(np.abs(got - sums) <= 2.0 ** -23 * np.abs(sums) + 2.0 ** -40 * mags))
-        log({"tunable": "__bcmi_reduce_by_key", "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_reduce_by_key", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, qq)
+            best = (ms, acc, qq)
     if best is None:
         raise RuntimeError("autotune: no reduce_by_key candidate matched")
-    _set(ca, "__bcmi_reduce_by_key", "q", best[1])
+    _set(ca, "__bcmi_reduce_by_key", "acc", best[1])
+    _set(ca, "__bcmi_reduce_by_key", "q", best[2])
     # segmented exclusive scan of the same runs (each run restarts at 0),
     # accumulated run by run in float64: step t adds the t-th element of
     # every run longer than t (runs ordered by length, so a prefix of them)
@@ -269,17 +313,19 @@ def tune_segscan(ca, W, q, log2n, log):
         acc[:c] += v64[pos]
         amag[:c] += np.abs(v64[pos])
     best = None
-    for qq in SEGSCAN_Q:
+    for acc, qq in [(a, b) for a in (0, 1) for b in SEGSCAN_Q]:
+        _set(ca, "__bcmi_segscan", "acc", acc)
         _set(ca, "__bcmi_segscan", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0))
         got = vo.to_host(q).astype(np.float64)
         ok = bool(np.all(np.abs(got - ex) <= 2.0 ** -23 * np.abs(ex) + 2.0 ** -36 * emag))
-        log({"tunable": "__bcmi_segscan", "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_segscan", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, qq)
+            best = (ms, acc, qq)
     if best is None:
         raise RuntimeError("autotune: no segmented scan candidate matched")
-    _set(ca, "__bcmi_segscan", "q", best[1])
+    _set(ca, "__bcmi_segscan", "acc", best[1])
+    _set(ca, "__bcmi_segscan", "q", best[2])
 
 
 def main(argv=None):
@@ -288,7 +334,8 @@ def main(argv=None):
     ap.add_argument("--log2n-sort", type=int, default=27)
     ap.add_argument("--log2n-scan", type=int, default=28)
     ap.add_argument("--log2n-segscan", type=int, default=27)
-    ap.add_argument("--only", default="sort,scan,segscan,reduce")
+    ap.add_argument("--segsort-nseg", type=int, default=65536)
+    ap.add_argument("--only", default="sort,scan,segscan,segsort,reduce")
     args = ap.parse_args(argv)
     sys.path.insert(0, os.path.dirname(HERE))
     import compute_amd as ca
@@ -303,6 +350,8 @@ def main(argv=None):
         tune_scan(ca, W, q, args.log2n_scan, log)
     if "segscan" in only:
         tune_segscan(ca, W, q, args.log2n_segscan, log)
+    if "segsort" in only:
+        tune_segsort(ca, W, q, args.segsort_nseg, log)
     if "reduce" in only:
         tune_reduce(ca, W, q, args.log2n_scan, log)
     q.finish()

@@ -63,6 +63,29 @@ struct ScanCompF32 {
     }
 };
 
+// The same precision class with FP64 accumulators: each float element is
+// widened once and the tile's running sums, wave scans and carries are
+// doubles (53 bits instead of a TwoSum pair's 48, one add instead of six
+// instructions).  FP64 vector math runs at half the FP32 rate here, so this
+// is about a third of the pair arithmetic.
+struct ScanF64AccF32 {
+    using S = double;
+    using Cr = double;
+    static BHD S id() { return 0.0; }
+    static BDEV S add(S s, float x) { return s + (double)x; }
+    static BDEV S merge(S a, S b) { return a + b; }
+    static BDEV Cr carry(S s) { return s; }
+    static BHD Cr cid() { return 0.0; }
+    static BDEV Cr cmerge(Cr a, Cr b) { return a + b; }
+    static BDEV float out(Cr prefix, S s) { return (float)(prefix + s); }
+    static Cr host_init(const void *p)
+    {
+        float v;
+        memcpy(&v, p, sizeof(float));
+        return (double)v;
+    }
+};
+
 // Where a tile's elements come from.  ArraySrc: an array (every plain
 // scan).  IterIn (ops.hpp): a sequence as the reference's iterators give
 // it, mapped element by element.
@@ -93,7 +116,7 @@ struct ScanLds {
 // aligned, so full chunks move as one 16-byte load / store.
 template <typename T, typename P, bool VEC, int Q, bool FULL, typename Src>
 BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
-                    typename P::Cr init, int exclusive, uint32_t back)
+                    typename P::Cr init, int exclusive, uint32_t epoch)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
@@ -150,7 +173,6 @@ BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, 
     for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, L.wave[i]);
     if (w == 0) {
         const S total = P::merge(P::merge(L.wave[0], L.wave[1]), P::merge(L.wave[2], L.wave[3]));
-        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
                                             [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
         if (lane == 0) L.prefix = prefix;
@@ -164,7 +186,15 @@ BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, 
 #pragma unroll
         for (int k = 0; k < V; ++k) {
             const S before = s;
-            s = P::add(s, x[q][k]);
+            T xk = x[q][k];
+            if constexpr (sizeof(S) > sizeof(T) && sizeof(T) == 4) {
+                // widened again here rather than kept widened since the fold
+                // (that would double the registers the tile's elements hold)
+                uint32_t u = __builtin_bit_cast(uint32_t, xk);
+                asm volatile("" : "+v"(u));
+                xk = __builtin_bit_cast(T, u);
+            }
+            s = P::add(s, xk);
             y[k] = P::out(prefix, exclusive ? before : s);
         }
         T *dst = wout + q * 64 * V + lane * V;
@@ -188,10 +218,11 @@ __global__ __launch_bounds__(kScanThreads) void k_scan(const Src src, T *out, si
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
     __shared__ ScanLds<P> lds;
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        scan_body<T, P, VEC, Q, true>(lds, src, out, n, ctrl, status, init, exclusive, back);
+        scan_body<T, P, VEC, Q, true>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
     else
-        scan_body<T, P, VEC, Q, false>(lds, src, out, n, ctrl, status, init, exclusive, back);
+        scan_body<T, P, VEC, Q, false>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
 }
 
 template <typename T, typename P, int Q, typename Src>
@@ -215,8 +246,9 @@ static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t 
     return 0;
 }
 
-// Chunks per thread come from the tuning cache (key "__bcmi_scan", param
-// "q": 4, 8 or 16) for 4-byte array scans; 4 otherwise.
+// Chunks per thread come from the tuning cache (key "__bcmi_scan", or
+// "__bcmi_scan_f32" for the compensated float scan; param "q": 4, 8 or 16)
+// for 4-byte array scans; 4 otherwise.
 template <typename T, typename P, typename Src>
 static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, const void *init, int exclusive)
 {
@@ -227,7 +259,10 @@ static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, cons
     typename P::Cr ini = P::cid();
     if (exclusive && init) ini = P::host_init(init);
     if constexpr (sizeof(T) == 4 && std::is_same<Src, ArraySrc<T>>::value) {
-        const unsigned qq = tune_get("__bcmi_scan", "q", 16);
+        // the compensated float scan holds a TwoSum pair per chunk, so it has
+        // its own chunk count (fewer chunks: fewer VGPRs, more resident tiles)
+        const bool comp = std::is_same<P, ScanCompF32>::value || std::is_same<P, ScanF64AccF32>::value;
+        const unsigned qq = tune_get(comp ? "__bcmi_scan_f32" : "__bcmi_scan", "q", 16);
         if (qq == 16) return launch_scan<T, P, 16>(q, src, out, n, status, ini, exclusive);
         if (qq == 8) return launch_scan<T, P, 8>(q, src, out, n, status, ini, exclusive);
     }
@@ -272,8 +307,13 @@ extern "C" int bcmi_scan(bcmi_queue q, int dtype, int op, const void *in, void *
     case BCMI_I64: return scan_plain<int64_t>(q, op, in, out, n, init, exclusive);
     case BCMI_U64: return scan_plain<uint64_t>(q, op, in, out, n, init, exclusive);
     case BCMI_F32:
-        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED)
+        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED) {
+            // tuning cache key "__bcmi_scan_f32", param "acc": 1 = FP64
+            // accumulators, 0 = TwoSum FP32 pairs
+            if (tune_get("__bcmi_scan_f32", "acc", 1))
+                return run_scan<float, ScanF64AccF32>(q, in, out, n, init, exclusive);
             return run_scan<float, ScanCompF32>(q, in, out, n, init, exclusive);
+        }
         return scan_plain<float>(q, op, in, out, n, init, exclusive);
     case BCMI_F64: return scan_plain<double>(q, op, in, out, n, init, exclusive);
     default: set_error("scan: bad dtype %d", dtype); return BCMI_INVALID_VALUE;

@@ -430,6 +430,7 @@ struct bcmi_segplan_s {
     int descending = 0;
     uint64_t total = 0;  // elements covered by the offsets
     int small_cls = 4;      // largest small class: capacity kSegSmallThreads << small_cls
+    int msd = 1;            // large segments may take the MSD route
     uint32_t nslots = 0;     // group slots of the MSD route, over all large segments
     uint32_t nsmall[5] = {0, 0, 0, 0, 0};  // classes IPT 1, 2, 4, 8, 16
     uint32_t ninsert = 0;
@@ -512,7 +513,7 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
         BCMI_LAUNCH(q, "segsort_route",
                     hipLaunchKernelGGL(k_seg_route, dim3(pl->nlarge), dim3(256), 0, q->stream, pl->large, pl->seg_off,
                                        (uint32_t)P, pl->slot_begin, pl->groups, pl->route,
-                                       (uint32_t)(kSegSmallThreads << pl->small_cls), (uint32_t)(P >= 2)));
+                                       (uint32_t)(kSegSmallThreads << pl->small_cls), (uint32_t)(P >= 2 && pl->msd)));
         if (P >= 2) {
             BCMI_LAUNCH(q, "segsort_histogram",
                         hipLaunchKernelGGL((k_seg_histogram<KB, XF>), dim3(pl->nchunks), dim3(kHistThreads), 0,
@@ -610,7 +611,11 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     const uint64_t tile = (uint64_t)kSortThreads * large_ipt(kb, value_bytes);
     SegTables tb;
     std::string err;
-    const int st0 = build_seg_tables(host_offsets, nseg, (size_t)kb + value_bytes, flt, tile, tb, err);
+    // tuning cache key "__bcmi_segsort": "small_max" caps the small classes
+    // (segments above go to the large path and its MSD groups of the same
+    // capacity), "msd" = 0 sends every large segment down the LSD passes
+    const int small_max = (int)tune_get("__bcmi_segsort", "small_max", 4);
+    const int st0 = build_seg_tables(host_offsets, nseg, (size_t)kb + value_bytes, flt, tile, tb, err, small_max);
     BCMI_CHECK(st0 == 0, st0, "%s", err.c_str());
     const int cmax = tb.small_cls;
     std::vector<SegSpan>(&small)[5] = tb.small;
@@ -623,6 +628,7 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     pl->key_dtype = key_dtype;
     pl->value_bytes = value_bytes;
     pl->descending = descending ? 1 : 0;
+    pl->msd = tune_get("__bcmi_segsort", "msd", 1) ? 1 : 0;
     pl->total = total;
     pl->nlarge = (uint32_t)large.size();
     pl->ntiles = (uint32_t)tiles.size();

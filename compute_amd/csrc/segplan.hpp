@@ -62,12 +62,13 @@ inline int seg_small_class(size_t elem_bytes)
 
 // Builds the tables; returns 0, or BCMI_INVALID_VALUE (offsets decrease) /
 // BCMI_INVALID_BUFFER_SIZE (a segment too long) with a message in err.
-// tile: elements per onesweep tile of a large segment.
+// tile: elements per onesweep tile of a large segment; small_cls_cap: the
+// largest small class allowed (the LDS budget may lower it further).
 inline int build_seg_tables(const uint64_t *offsets, size_t nseg, size_t elem_bytes, bool float_keys,
-                             uint64_t tile, SegTables &t, std::string &err)
+                             uint64_t tile, SegTables &t, std::string &err, int small_cls_cap = 4)
 {
     t = SegTables();
-    t.small_cls = seg_small_class(elem_bytes);
+    t.small_cls = std::max(0, std::min(seg_small_class(elem_bytes), small_cls_cap));
     const uint64_t small_max = (uint64_t)kSegSmallThreads << t.small_cls;
     t.total = nseg ? offsets[nseg] : 0;
     char buf[160];

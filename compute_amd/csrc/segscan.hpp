@@ -53,6 +53,7 @@ struct SegPlain {
     // value of (prefix . local): the prefix value counts only if no head
     // occurred in the local part
     static BDEV VT fin(CA p, A s, bool use_p) { return (VT)(use_p ? apply_op<A, OP>(p, s) : s); }
+    static BDEV VT fin_local(A s) { return (VT)s; }
     static BDEV VT fin_init(VT init, CA p, A s, bool use_p)
     {
         return (VT)apply_op<A, OP>((A)init, use_p ? apply_op<A, OP>(p, s) : s);
@@ -69,9 +70,30 @@ struct SegCompF32 {
     static BHD CA cid() { return 0.0; }
     static BDEV CA cmerge(CA a, CA b) { return a + b; }
     static BDEV float fin(CA p, A s, bool use_p) { return (float)(use_p ? p + cf_value(s) : cf_value(s)); }
+    static BDEV float fin_local(A s) { return (float)cf_value(s); }
     static BDEV float fin_init(float init, CA p, A s, bool use_p)
     {
         return (float)((double)init + (use_p ? p + cf_value(s) : cf_value(s)));
+    }
+};
+
+// The same precision class with FP64 accumulators: float elements widened
+// once, run sums and carries in double (one add per element instead of a
+// TwoSum pair's six instructions; FP64 vector math is half the FP32 rate).
+struct SegF64AccF32 {
+    using A = double;
+    using CA = double;
+    static BHD A id() { return 0.0; }
+    static BDEV A add(A a, float x) { return a + (double)x; }
+    static BDEV A merge(A a, A b) { return a + b; }
+    static BDEV CA carry(A a) { return a; }
+    static BHD CA cid() { return 0.0; }
+    static BDEV CA cmerge(CA a, CA b) { return a + b; }
+    static BDEV float fin(CA p, A s, bool use_p) { return (float)(use_p ? p + s : s); }
+    static BDEV float fin_local(A s) { return (float)s; }
+    static BDEV float fin_init(float init, CA p, A s, bool use_p)
+    {
+        return (float)((double)init + (use_p ? p + s : s));
     }
 };
 
@@ -108,7 +130,7 @@ struct alignas(V * sizeof(KT) >= 16 ? 16 : V * sizeof(KT)) key_chunk {
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
 BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                        KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
-                       uint64_t *count_out, uint32_t back)
+                       uint64_t *count_out, uint32_t epoch)
 {
     using A = typename P::A;
     using CA = typename P::CA;
@@ -192,7 +214,6 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
         const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
         const C agg{P::carry(total.v), total.n};
         const C cid{P::cid(), 0u};
-        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
             return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
         }, ctrl);
@@ -271,7 +292,7 @@ struct SegLdsBlk {
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
 BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                            KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
-                           uint64_t *count_out, uint32_t back)
+                           uint64_t *count_out, uint32_t epoch)
 {
     static_assert(sizeof(KT) == 4 && sizeof(VT) == 4, "blocked form: 4-byte keys and values");
     using A = typename P::A;
@@ -331,14 +352,29 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
         if (!key_eq(before, kb[e])) headm |= 1u << e;
     }
     if (lane == 0 && wstart == 0) headm |= 1u;
-    // fold of this lane's (head, value) pairs, one wave scan
+    // fold of this lane's (head, value) pairs, one wave scan.  reduce_by_key
+    // also keeps, at every head, the run that ends just before it when that
+    // run began in this lane (its sum is complete here; parked in the lane's
+    // row of the transpose buffer, free now, rather than in registers), and
+    // the sum of the lane's elements before its first head (that run came
+    // from before the lane and is finished once the lane's prefix is known).
     const S sid{P::id(), 0u};
     auto mrg = [](S a, S b) { return seg_merge<A, P>(a, b); };
     S t = sid;
+    VT y[MODE == SEG_RBK ? 1 : E];
+    uint32_t *row = buf + lane * BlkBuf<Q>::ROW;
+    A tfh = P::id();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
         if (ok(e)) {
-            if (headm >> e & 1u) t = S{P::add(P::id(), xb[e]), t.n + 1};
+            const bool head = headm >> e & 1u;
+            if constexpr (MODE == SEG_RBK) {
+                if (head) {
+                    if (t.n) row[e] = __builtin_bit_cast(uint32_t, P::fin_local(t.v));
+                    else tfh = t.v;
+                }
+            }
+            if (head) t = S{P::add(P::id(), xb[e]), t.n + 1};
             else t.v = P::add(t.v, xb[e]);
         }
     }
@@ -352,7 +388,6 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
         const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
         const C agg{P::carry(total.v), total.n};
         const C cid{P::cid(), 0u};
-        const uint32_t epoch = launch_epoch_after(ctrl, back, low_word(total));
         const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
             return C{b.n ? b.v : P::cmerge(a.v, b.v), a.n + b.n};
         }, ctrl);
@@ -360,42 +395,52 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     }
     block_barrier();
     const C prefix = L.prefix;
-    S s = mrg(wpre, pre);  // exclusive local pair before this lane's first element
-    VT y[E];
+    const S s0 = mrg(wpre, pre);  // exclusive local pair before this lane's first element
+    if constexpr (MODE == SEG_RBK) {
+        // runs end before each head (i > 0) and at element n - 1; run r is
+        // written at index r, the number of heads before the run's end minus
+        // one.  Only the run ending at the lane's first head, and a final
+        // run with no head in the lane, continue from before the lane.
+        const VT vfirst = P::fin(prefix.v, P::merge(s0.v, tfh), s0.n == 0);
+        const uint32_t nb0 = prefix.n + s0.n;  // heads before this lane
+        uint32_t hc = 0;                       // this lane's heads before e
 #pragma unroll
-    for (int e = 0; e < E; ++e) {
-        if (!ok(e)) continue;
-        const size_t i = wstart + (size_t)lane * E + e;
-        const bool head = headm >> e & 1u;
-        if constexpr (MODE == SEG_RBK) {
+        for (int e = 0; e < E; ++e) {
+            if (!ok(e)) continue;
+            const size_t i = wstart + (size_t)lane * E + e;
+            const bool head = headm >> e & 1u;
             if (head && i > 0) {
-                // finished run i-1 = the exclusive pair at i
-                const uint32_t nb = prefix.n + s.n;
+                const uint32_t nb = nb0 + hc;
                 if (nb >= 1 && nb <= n) {
                     keys_out[nb - 1] = e == 0 ? kprev : kb[e - 1];
-                    out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
+                    out[nb - 1] = hc == 0 ? vfirst : __builtin_bit_cast(VT, row[e]);
                 } else {
                     __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
             }
-        } else if constexpr (MODE == SEG_EXCL) {
-            y[e] = head ? init : P::fin_init(init, prefix.v, s.v, s.n == 0);
-        }
-        if (head) s = S{P::add(P::id(), xb[e]), s.n + 1};
-        else s.v = P::add(s.v, xb[e]);
-        if constexpr (MODE == SEG_INCL) y[e] = P::fin(prefix.v, s.v, s.n == 0);
-        if constexpr (MODE == SEG_RBK) {
+            hc += head ? 1u : 0u;
             if (i == n - 1) {
-                const uint32_t nb = prefix.n + s.n;
+                const uint32_t nb = nb0 + hc;
                 if (nb >= 1 && nb <= n) {
                     keys_out[nb - 1] = kb[e];
-                    out[nb - 1] = P::fin(prefix.v, s.v, s.n == 0);
+                    out[nb - 1] = hc ? P::fin_local(t.v) : P::fin(prefix.v, P::merge(s0.v, t.v), s0.n == 0);
                 } else {
                     __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 }
                 *count_out = nb;
             }
         }
+        return;
+    }
+    S s = s0;
+#pragma unroll
+    for (int e = 0; e < E; ++e) {
+        if (!ok(e)) continue;
+        const bool head = headm >> e & 1u;
+        if constexpr (MODE == SEG_EXCL) y[e] = head ? init : P::fin_init(init, prefix.v, s.v, s.n == 0);
+        if (head) s = S{P::add(P::id(), xb[e]), s.n + 1};
+        else s.v = P::add(s.v, xb[e]);
+        if constexpr (MODE == SEG_INCL) y[e] = P::fin(prefix.v, s.v, s.n == 0);
     }
     if constexpr (MODE != SEG_RBK) {
         VT ys[Q][V];
@@ -426,22 +471,23 @@ __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ 
 {
     constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
     const bool full = (size_t)(blockIdx.x + 1) * TILE <= n;
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4 && Q <= 8) {
         __shared__ SegLdsBlk<P, Q> lds;
         if (full)
             segscan_body_blk<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                            count_out, back);
+                                                            count_out, epoch);
         else
             segscan_body_blk<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                             count_out, back);
+                                                             count_out, epoch);
     } else {
         __shared__ SegLds<P> lds;
         if (full)
             segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                        count_out, back);
+                                                        count_out, epoch);
         else
             segscan_body<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                         count_out, back);
+                                                         count_out, epoch);
     }
 }
 
@@ -493,8 +539,13 @@ static int seg_val_op(bcmi_queue q, int op, int precision, const void *keys, con
                       void *keys_out, size_t n, const void *init, uint64_t *count_out)
 {
     if constexpr (std::is_same<VT, float>::value) {
-        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED)
+        if (op == BCMI_PLUS && precision == BCMI_PRECISION_COMPENSATED) {
+            // tuning cache key "__bcmi_reduce_by_key" / "__bcmi_segscan",
+            // param "acc": 1 = FP64 accumulators, 0 = TwoSum FP32 pairs
+            if (tune_get(MODE == SEG_RBK ? "__bcmi_reduce_by_key" : "__bcmi_segscan", "acc", 1))
+                return run_seg<KT, VT, SegF64AccF32, MODE>(q, keys, vals, out, keys_out, n, init, count_out);
             return run_seg<KT, VT, SegCompF32, MODE>(q, keys, vals, out, keys_out, n, init, count_out);
+        }
     }
     if constexpr (MODE != SEG_RBK) {
         // segmented scans are plus-scans

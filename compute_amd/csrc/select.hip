@@ -192,7 +192,7 @@ BDEV typename SelShape<T>::F sel_flags(const T (&x)[SelShape<T>::Q][SelShape<T>:
 
 template <typename T, int MODE, bool FULL>
 BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n, void *out_a, void *out_b,
-                      const SelPred &p, Ctrl *ctrl, uint64_t *status, uint64_t *count_out, uint32_t back)
+                      const SelPred &p, Ctrl *ctrl, uint64_t *status, uint64_t *count_out, uint32_t epoch)
 {
     using Sh = SelShape<T>;
     using C = typename SelCmp<T>::type;
@@ -301,7 +301,6 @@ BDEV void select_body(SelLds<T, MODE> &L, const T *abase, size_t head, size_t n,
     const uint32_t vcount = (uint32_t)(vhi_abs - vlo_abs);
     const uint32_t vlo = (uint32_t)(vlo_abs - tstart);
     if (w == 0) {
-        const uint32_t epoch = launch_epoch_after(ctrl, back, f);
         const uint32_t prefix = lookback_scan<uint32_t>(status, tile, epoch, cnt, 0u, 0u,
                                                         [](uint32_t a, uint32_t b) { return a + b; }, ctrl);
         if (lane == 0) {
@@ -385,10 +384,11 @@ __global__ __launch_bounds__(kSelThreads) __attribute__((amdgpu_waves_per_eu(BCM
     constexpr size_t TILE = SelShape<T>::TILE;
     __shared__ SelLds<T, MODE> lds;
     const size_t tstart = (size_t)blockIdx.x * TILE;
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if (tstart >= head && tstart + TILE <= head + n)
-        select_body<T, MODE, true>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, back);
+        select_body<T, MODE, true>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, epoch);
     else
-        select_body<T, MODE, false>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, back);
+        select_body<T, MODE, false>(lds, abase, head, n, out_a, out_b, p, ctrl, status, count_out, epoch);
 }
 
 // stable_partition, pass 2: out[i] = tmp[i] below the selected count, and

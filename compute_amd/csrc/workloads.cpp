@@ -96,6 +96,16 @@ double wl_sum_f32_dd(const float *x, size_t n, double init)
     return a.value();
 }
 
+// double-double inclusive prefix sums of x, each rounded to double into out
+void wl_scan_f32_dd(const float *x, double *out, size_t n)
+{
+    DD a;
+    for (size_t i = 0; i < n; ++i) {
+        a.add((double)x[i]);
+        out[i] = a.value();
+    }
+}
+
 // inclusive (exclusive != 0: exclusive, seeded with init) prefix sums
 void wl_err_scan_f32(const float *x, const float *got, size_t n, int exclusive, double init, double *err)
 {

@@ -24,6 +24,7 @@ def lib():
         L.wl_segments.restype = sz
         L.wl_segment_data.argtypes = [vp, vp, sz]
         L.wl_runs_i32_f32.argtypes = [vp, vp, sz, sz]
+        L.wl_scan_f32_dd.argtypes = [vp, vp, sz]
         L.wl_runs_i32_f32.restype = sz
         dp = C.POINTER(C.c_double)
         L.wl_sum_f32_dd.argtypes = [vp, sz, C.c_double]
@@ -99,6 +100,14 @@ def _err(fn, *args):
 def sum_dd(x, init=0.0):
     """Double-double value of init + sum(x) (float32 x)."""
     return lib().wl_sum_f32_dd(_p(np.ascontiguousarray(x, np.float32)), x.size, float(init))
+
+
+def scan_dd(x):
+    """Double-double inclusive prefix sums of float32 x, as float64."""
+    x = np.ascontiguousarray(x, np.float32)
+    out = np.empty(x.size, np.float64)
+    lib().wl_scan_f32_dd(_p(x), _p(out), x.size)
+    return out
 
 
 def err_scan(x, got, exclusive, init=0.0):

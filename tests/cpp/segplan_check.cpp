@@ -8,6 +8,7 @@
 #include "../../compute_amd/csrc/segplan.hpp"
 
 #include <cstdio>
+#include <algorithm>
 #include <map>
 #include <random>
 #include <string>
@@ -24,14 +25,14 @@ static int failures = 0;
         }                                                                           \
     } while (0)
 
-static void check_plan(const std::vector<uint64_t> &off, size_t eb, bool flt, uint64_t tile)
+static void check_plan(const std::vector<uint64_t> &off, size_t eb, bool flt, uint64_t tile, int cls_cap = 4)
 {
     const size_t nseg = off.size() - 1;
     SegTables t;
     std::string err;
-    CHECK(build_seg_tables(off.data(), nseg, eb, flt, tile, t, err) == 0);
+    CHECK(build_seg_tables(off.data(), nseg, eb, flt, tile, t, err, cls_cap) == 0);
     CHECK(t.total == off[nseg]);
-    CHECK(t.small_cls >= 0 && t.small_cls <= 4);
+    CHECK(t.small_cls >= 0 && t.small_cls <= 4 && t.small_cls <= std::max(cls_cap, 0));
     const uint64_t small_max = (uint64_t)kSegSmallThreads << t.small_cls;
     // which path each segment took, by base
     std::map<uint64_t, int> seen;
@@ -118,7 +119,7 @@ int main()
             }
             off[s + 1] = off[s] + len;
         }
-        check_plan(off, ebs[rep % 5], rep % 2 == 0, tiles[rep % 3]);
+        check_plan(off, ebs[rep % 5], rep % 2 == 0, tiles[rep % 3], 4 - rep % 5);
     }
     // refused: decreasing offsets, an over-long segment
     {

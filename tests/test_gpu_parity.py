@@ -694,6 +694,70 @@ def test_segmented_sort_matches_per_segment_calls(queue, dt, vdt):
         assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), desc
 
 
+@pytest.mark.parametrize("acc", [0, 1])
+def test_float_accumulator_policies(queue, acc):
+    """Both float accumulator policies the tuner may pick (1: FP64
+    accumulators, 0: TwoSum FP32 pairs) meet the compensated bound for the
+    scan, reduce_by_key and the segmented scan, on whole and partial tiles."""
+    L = ca.lib()
+    keys_t = ("__bcmi_scan_f32", "__bcmi_reduce_by_key", "__bcmi_segscan")
+    saved = [_tune_get(k, "acc") for k in keys_t]
+    try:
+        for k in keys_t:
+            ca.check(L.bcmi_tune_set(k.encode(), b"acc", acc))
+        for n in (4096 * 3 + 5, (1 << 20) + 17):
+            x = O.gen_lognormal_f32(n)
+            out = ca.DeviceArray(n, np.float32, queue)
+            ca.scan(queue, dev(queue, x), out, False)
+            dd = O.scan_f32_dd(x, False)
+            assert np.all(np.abs(out.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(x, dd)), n
+            keys, vals = O.gen_runs_i32_f32(n)
+            ko = ca.DeviceArray(n, np.int32, queue)
+            r = ca.reduce_by_key(queue, dev(queue, keys), dev(queue, vals), ko, out)
+            ek, es = O.reduce_by_key_f32_dd(keys, vals)
+            assert r == ek.size and np.array_equal(ko.to_host(queue)[:r], ek), n
+            absum = np.add.reduceat(np.abs(vals.astype(np.float64)), np.r_[0, np.flatnonzero(np.diff(keys)) + 1])
+            bound = np.spacing(np.abs(es).astype(np.float32)).astype(np.float64) + 2.0 ** -50 * absum
+            assert np.all(np.abs(out.to_host(queue)[:r].astype(np.float64) - es) <= bound), n
+            for excl in (False, True):
+                ca.segmented_scan(queue, dev(queue, keys), dev(queue, vals), out, excl, init=0.0 if excl else None)
+                dd = O.segmented_scan_f32_dd(keys, vals, excl)
+                bound = np.spacing(np.abs(dd).astype(np.float32)).astype(np.float64) + \
+                    2.0 ** -50 * np.cumsum(np.abs(vals.astype(np.float64)))
+                assert np.all(np.abs(out.to_host(queue).astype(np.float64) - dd) <= bound), (n, excl)
+    finally:
+        for k, val in zip(keys_t, saved):
+            ca.check(L.bcmi_tune_set(k.encode(), b"acc", val if val != 0xFFFFFFFF else 1))
+
+
+@pytest.mark.parametrize("small_max,msd", [(2, 1), (3, 0), (0, 1)])
+def test_segmented_sort_routing_thresholds(queue, small_max, msd):
+    """Every routing threshold the tuner may pick (small classes capped,
+    MSD route off) gives the per-segment result: segments between the cap
+    and 4096 go to the large path and, with the MSD route, to its groups."""
+    L = ca.lib()
+    saved = [_tune_get("__bcmi_segsort", p) for p in ("small_max", "msd")]
+    try:
+        ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"small_max", small_max))
+        ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"msd", msd))
+        lens = [0, 1, 2, 31, 32, 200, 256, 257, 700, 1025, 2049, 4096, 4097, 9000, 30000]
+        lens += list(RNG.integers(1, 5000, 40))
+        off = _segments(lens)
+        n = int(off[-1])
+        for dt in ("uint64", "float32"):
+            k = random_keys(dt, n)
+            v = np.arange(n, dtype=np.uint32)
+            dk, dv = dev(queue, k), dev(queue, v)
+            plan = ca.SegmentedPlan(queue, np.dtype(dt), np.uint32, off)
+            plan.run(queue, dk, dv)
+            ek, ev = O.segmented_sort(k, v, off)
+            assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), dt
+            assert np.array_equal(dv.to_host(queue), ev), dt
+    finally:
+        for p, val, d in zip(("small_max", "msd"), saved, (4, 1)):
+            ca.check(L.bcmi_tune_set(b"__bcmi_segsort", p.encode(), val if val != 0xFFFFFFFF else d))
+
+
 def test_segmented_sort_float_small_segments_keep_signed_zero_order(queue):
     # below 32 elements stable_sort_by_key is the insertion sort: +0.0 and
     # -0.0 compare equal and keep input order; from 32 on, radix order puts
