@@ -346,63 +346,88 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                                uint32_t pass, uint32_t passes,
                                                                const uint32_t *__restrict__ seg_off, XF xf,
                                                                Ctrl *ctrl, uint64_t *__restrict__ status,
-                                                               uint32_t back, const uint32_t *__restrict__ route,
-                                                               uint32_t want)
+                                                               uint32_t back, uint32_t want)
 {
+    // the tile carries its segment's route (one load before a tile of the
+    // other route exits)
     const SegTile t = tiles[blockIdx.x];
-    if (route[t.seg] != want) return;  // the segment takes the other route
+    if (t.route != want) return;  // the segment takes the other route
     const size_t b = t.base;
     onesweep_tile<KB, VB, IPT, kRankLdsMask, XF>(kin + b, kout + b, VB ? vin + b : vin, VB ? vout + b : vout, t.len,
                                              t.tile, shift, seg_off + ((size_t)t.seg * passes + pass) * 256, xf,
                                              ctrl, status + (size_t)(blockIdx.x - t.tile) * 256, back);
 }
 
-// One block (256 threads, thread = digit) per large segment.
+// One block (256 threads, thread = digit) per large segment.  The top-digit
+// buckets are packed greedily into groups of at most `cap` elements: a
+// group that starts at bucket b ends before next(b), the first bucket whose
+// end lies more than cap past b's start (a binary search over the bucket
+// starts), and the group starts are the chain 0, next(0), next(next(0)), ...
+// That chain is marked in 8 rounds of pointer doubling (it has at most 256
+// nodes), so no thread walks the buckets one by one (a serial walk per
+// segment is bound by the CU's one scalar unit).  A group's slot is its
+// start's rank among the marks.  All threads then clear the segment's unused
+// group slots and tag its tiles with the route.
 __global__ __launch_bounds__(256) void k_seg_route(const SegSpan *__restrict__ large,
                                                    const uint32_t *__restrict__ seg_off, uint32_t passes,
                                                    const uint32_t *__restrict__ slot_begin, SegSpan *groups,
-                                                   uint32_t *route, uint32_t cap, uint32_t enable)
+                                                   uint32_t *route, SegTile *tiles, uint32_t tile_len, uint32_t cap,
+                                                   uint32_t enable)
 {
     __shared__ uint32_t s_start[257];
+    __shared__ uint32_t s_jump[2][257];
+    __shared__ uint32_t s_mark[257];
+    __shared__ uint32_t s_wave[4];
     const uint32_t s = blockIdx.x, d = threadIdx.x;
     const SegSpan sg = large[s];
     const uint32_t *off = seg_off + ((size_t)s * passes + passes - 1) * 256;
     s_start[d] = off[d];
-    if (d == 0) s_start[256] = sg.len;
+    if (d == 0) {
+        s_start[256] = sg.len;
+        s_jump[0][256] = s_jump[1][256] = 256;
+        s_mark[256] = 0;
+    }
     block_barrier();
-    const uint32_t size = s_start[d + 1] - s_start[d];
-    const int big = block_barrier_or(size > cap);
+    const uint32_t sd = s_start[d];
+    const int big = block_barrier_or(s_start[d + 1] - sd > cap);
     const uint32_t b0 = slot_begin[s], b1 = slot_begin[s + 1];
-    if (d != 0) return;
-    // count the greedy groups first, so the route is decided before anything is written
-    uint32_t ngroups = 0, gl = 0;
-    for (int b = 0; b < 256; ++b) {
-        const uint32_t z = s_start[b + 1] - s_start[b];
-        if (gl > 0 && gl + z > cap) {
-            ++ngroups;
-            gl = 0;
-        }
-        gl += z;
+    // next(d): the first c > d with s_start[c + 1] - s_start[d] > cap, else 256
+    uint32_t lo = d + 1, hi = 256;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (s_start[mid + 1] - sd > cap) hi = mid;
+        else lo = mid + 1;
     }
-    if (gl > 0) ++ngroups;
+    const uint32_t nx = lo;
+    s_jump[0][d] = nx;
+    s_mark[d] = d == 0;
+    block_barrier();
+    // round t: every marked node marks its 2^t-th successor; jumps double
+#pragma unroll 1
+    for (int t = 0; t < 8; ++t) {
+        const uint32_t j = s_jump[t & 1][d];
+        if (s_mark[d]) s_mark[j] = 1;  // j = 256 is the sentinel slot
+        s_jump[(t & 1) ^ 1][d] = s_jump[t & 1][j];
+        block_barrier();
+    }
+    const bool m = s_mark[d] != 0;
+    const uint64_t bal = ballot(m);
+    const uint32_t w = d >> 6;
+    if ((d & 63) == 0) s_wave[w] = popc64(bal);
+    block_barrier();
+    uint32_t before = 0, ngroups = 0;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        before += (uint32_t)i < w ? s_wave[i] : 0u;
+        ngroups += s_wave[i];
+    }
     const bool msd = enable && !big && ngroups <= b1 - b0;
-    route[s] = msd ? kRouteMsd : kRouteLsd;
-    uint32_t slot = b0;
-    if (msd) {
-        uint32_t gs = 0;
-        gl = 0;
-        for (int b = 0; b < 256; ++b) {
-            const uint32_t z = s_start[b + 1] - s_start[b];
-            if (gl > 0 && gl + z > cap) {
-                groups[slot++] = SegSpan{sg.base + gs, gl, 0};
-                gl = 0;
-            }
-            if (gl == 0) gs = s_start[b];
-            gl += z;
-        }
-        if (gl > 0) groups[slot++] = SegSpan{sg.base + gs, gl, 0};
-    }
-    for (; slot < b1; ++slot) groups[slot] = SegSpan{0, 0, 0};
+    const uint32_t rt = msd ? kRouteMsd : kRouteLsd;
+    if (d == 0) route[s] = rt;
+    if (msd && m) groups[b0 + before + mbcnt(bal)] = SegSpan{sg.base + sd, s_start[nx] - sd, 0};
+    for (uint32_t i = (msd ? b0 + ngroups : b0) + d; i < b1; i += 256) groups[i] = SegSpan{0, 0, 0};
+    const uint32_t nt = (uint32_t)(((uint64_t)sg.len + tile_len - 1) / tile_len);
+    for (uint32_t i = d; i < nt; i += 256) tiles[sg.idx + i].route = rt;
 }
 
 // odd pass count (8-bit keys): copy the large segments back from the temporaries
@@ -450,6 +475,12 @@ struct bcmi_segplan_s {
     uint32_t *seg_off = nullptr;
     void *ktmp = nullptr;
     void *vtmp = nullptr;
+    // the small segments are sorted on a stream of the plan's own,
+    // concurrently with the large path (the two touch disjoint elements):
+    // forked from and joined back into the queue's stream, so a graph
+    // capture records two branches
+    hipStream_t aux = nullptr;
+    hipEvent_t fork = nullptr, join = nullptr;
 };
 
 namespace bcmi {
@@ -460,22 +491,18 @@ static int large_ipt(int kb, size_t vb)
     return sb >= 16 ? 8 : (sb >= 8 ? 12 : 16);
 }
 
+// The small classes (sorted in place in LDS) and the insertion-sorted
+// segments, on stream ss.
 template <typename KB, int VB, typename XF>
-static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, XF xf)
+static int run_plan_small(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT<VB>::type *vals, XF xf,
+                          hipStream_t ss)
 {
-    using V = typename ValT<VB>::type;
-    constexpr int P = sizeof(KB);
-    constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
-    constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
-    KB *keys = (KB *)keys_;
-    V *vals = (V *)vals_;
-    // small classes (sorted in place)
 #define BCMI_SMALL(CLS, IPTV)                                                                                   \
     if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
         if (pl->nsmall[CLS])                                                                                    \
             BCMI_LAUNCH(q, "segsort_small",                                                                     \
                         hipLaunchKernelGGL((k_seg_small<KB, VB, IPTV, XF>), dim3(pl->nsmall[CLS]),                  \
-                                           dim3(kSegSmallThreads), 0, q->stream, keys, vals, keys, vals,          \
+                                           dim3(kSegSmallThreads), 0, ss, keys, vals, keys, vals,                 \
                                            pl->small[CLS], xf));                                                \
     }
     BCMI_SMALL(0, 1)
@@ -489,15 +516,26 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
         if (pl->key_dtype == BCMI_F32) {
             if constexpr (sizeof(KB) == 4)
                 BCMI_LAUNCH(q, "segsort_insertion",
-                            hipLaunchKernelGGL((k_seg_insertion<float, VB>), dim3(blocks), dim3(64), 0, q->stream,
+                            hipLaunchKernelGGL((k_seg_insertion<float, VB>), dim3(blocks), dim3(64), 0, ss,
                                                (float *)keys, vals, pl->insert, pl->ninsert, pl->descending));
         } else {
             if constexpr (sizeof(KB) == 8)
                 BCMI_LAUNCH(q, "segsort_insertion",
-                            hipLaunchKernelGGL((k_seg_insertion<double, VB>), dim3(blocks), dim3(64), 0, q->stream,
+                            hipLaunchKernelGGL((k_seg_insertion<double, VB>), dim3(blocks), dim3(64), 0, ss,
                                                (double *)keys, vals, pl->insert, pl->ninsert, pl->descending));
         }
     }
+    return 0;
+}
+
+// The large segments: top-digit route decision, MSD groups, LSD passes.
+template <typename KB, int VB, typename XF>
+static int run_plan_large(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT<VB>::type *vals, XF xf)
+{
+    using V = typename ValT<VB>::type;
+    constexpr int P = sizeof(KB);
+    constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+    constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if (pl->nlarge) {
         // the top digit's histogram and offsets decide each segment's route
         // (MSD needs two digits or more); the other digits are counted only
@@ -512,8 +550,9 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
                                        (const uint32_t *)nullptr));
         BCMI_LAUNCH(q, "segsort_route",
                     hipLaunchKernelGGL(k_seg_route, dim3(pl->nlarge), dim3(256), 0, q->stream, pl->large, pl->seg_off,
-                                       (uint32_t)P, pl->slot_begin, pl->groups, pl->route,
-                                       (uint32_t)(kSegSmallThreads << pl->small_cls), (uint32_t)(P >= 2 && pl->msd)));
+                                       (uint32_t)P, pl->slot_begin, pl->groups, pl->route, pl->tiles,
+                                       (uint32_t)(kSortThreads * IPTL), (uint32_t)(kSegSmallThreads << pl->small_cls),
+                                       (uint32_t)(P >= 2 && pl->msd)));
         if (P >= 2) {
             BCMI_LAUNCH(q, "segsort_histogram",
                         hipLaunchKernelGGL((k_seg_histogram<KB, XF>), dim3(pl->nchunks), dim3(kHistThreads), 0,
@@ -532,7 +571,7 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
                     hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
                                        q->stream, keys, (KB *)pl->ktmp, vals, (V *)pl->vtmp, pl->tiles,
                                        (uint32_t)(8 * (P - 1)), (uint32_t)(P - 1), (uint32_t)P, pl->seg_off, xf,
-                                       q->ctrl, status, (uint32_t)(P + 1), pl->route, (uint32_t)kRouteMsd));
+                                       q->ctrl, status, (uint32_t)(P + 1), (uint32_t)kRouteMsd));
         // ... then every group sorted in LDS from the temporaries back into place
 #define BCMI_GROUPS(CLS, IPTV)                                                                                  \
     if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
@@ -556,7 +595,7 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
                         hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
                                            q->stream, ks, kd, vs, vd, pl->tiles, (uint32_t)(8 * p), (uint32_t)p,
                                            (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p),
-                                           pl->route, (uint32_t)kRouteLsd));
+                                           (uint32_t)kRouteLsd));
             std::swap(ks, kd);
             std::swap(vs, vd);
         }
@@ -566,6 +605,37 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
                                            ks, vals, vs, pl->large, pl->route));
     }
     return 0;
+}
+
+template <typename KB, int VB, typename XF>
+static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, XF xf)
+{
+    using V = typename ValT<VB>::type;
+    KB *keys = (KB *)keys_;
+    V *vals = (V *)vals_;
+    // the small segments go to the plan's stream when there is large work
+    // to overlap them with (not while the queue profiles kernels: its
+    // per-kernel times are taken on the queue's stream)
+    bool any_small = pl->ninsert != 0;
+    for (int c = 0; c < 5; ++c) any_small = any_small || pl->nsmall[c] != 0;
+    const bool forked = any_small && pl->nlarge != 0 && pl->aux != nullptr && !q->profile;
+    if (forked) {
+        BCMI_HIP(hipEventRecord(pl->fork, q->stream));
+        BCMI_HIP(hipStreamWaitEvent(pl->aux, pl->fork, 0));
+    }
+    int st = run_plan_small<KB, VB, XF>(q, pl, keys, vals, xf, forked ? pl->aux : q->stream);
+    // the join is recorded whatever happened, so that a capture is never
+    // left with a branch outside the queue's stream
+    if (forked) {
+        const hipError_t e = hipEventRecord(pl->join, pl->aux);
+        if (e != hipSuccess && !st) st = hip_fail(e, "segmented sort: join record", __FILE__, __LINE__);
+    }
+    if (!st) st = run_plan_large<KB, VB, XF>(q, pl, keys, vals, xf);
+    if (forked) {
+        const hipError_t e = hipStreamWaitEvent(q->stream, pl->join, 0);
+        if (e != hipSuccess && !st) st = hip_fail(e, "segmented sort: join", __FILE__, __LINE__);
+    }
+    return st;
 }
 
 template <typename KB, int VB>
@@ -703,6 +773,19 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
             return s2;
         }
     }
+    // the second stream (default priority) and its fork / join events, when
+    // there are both small and large segments
+    bool any_small = pl->ninsert != 0;
+    for (int c = 0; c < 5; ++c) any_small = any_small || pl->nsmall[c] != 0;
+    if (any_small && pl->nlarge) {
+        hipError_t e2 = hipStreamCreateWithFlags(&pl->aux, hipStreamNonBlocking);
+        if (e2 == hipSuccess) e2 = hipEventCreateWithFlags(&pl->fork, hipEventDisableTiming);
+        if (e2 == hipSuccess) e2 = hipEventCreateWithFlags(&pl->join, hipEventDisableTiming);
+        if (e2 != hipSuccess) {
+            bcmi_segmented_plan_destroy(pl);
+            return hip_fail(e2, "segmented plan stream", __FILE__, __LINE__);
+        }
+    }
     *plan = pl;
     return 0;
 }
@@ -711,6 +794,9 @@ extern "C" int bcmi_segmented_plan_destroy(bcmi_segplan plan)
 {
     if (!plan) return 0;
     (void)hipSetDevice(plan->device);
+    if (plan->join) (void)hipEventDestroy(plan->join);
+    if (plan->fork) (void)hipEventDestroy(plan->fork);
+    if (plan->aux) (void)hipStreamDestroy(plan->aux);
     if (plan->mem) BCMI_HIP(hipFree(plan->mem));
     delete plan;
     return 0;

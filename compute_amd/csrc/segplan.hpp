@@ -31,15 +31,15 @@ constexpr uint32_t kSegChunk = 1u << 16;  // keys per histogram chunk of a large
 struct SegSpan {
     uint64_t base;
     uint32_t len;
-    uint32_t idx;  // large-segment index (tiles, chunks) or unused
+    uint32_t idx;  // large segments: index of the first tile; chunks: large-segment index; else unused
 };
 
 struct SegTile {
     uint64_t base;
     uint32_t len;
-    uint32_t tile;  // tile index within the segment
-    uint32_t seg;   // large-segment index
-    uint32_t pad;
+    uint32_t tile;   // tile index within the segment
+    uint32_t seg;    // large-segment index
+    uint32_t route;  // the segment's route, written on the device before the passes run
 };
 
 struct SegTables {
@@ -94,7 +94,7 @@ inline int build_seg_tables(const uint64_t *offsets, size_t nseg, size_t elem_by
             t.small[c].push_back(SegSpan{base, (uint32_t)len, 0});
         } else {
             const uint32_t li = (uint32_t)t.large.size();
-            t.large.push_back(SegSpan{base, (uint32_t)len, li});
+            t.large.push_back(SegSpan{base, (uint32_t)len, (uint32_t)t.tiles.size()});
             for (uint64_t k = 0; k * tile < len; ++k) t.tiles.push_back(SegTile{base, (uint32_t)len, (uint32_t)k, li, 0});
             for (uint64_t c = 0; c < len; c += kSegChunk) {
                 const uint64_t cl = len - c < kSegChunk ? len - c : kSegChunk;

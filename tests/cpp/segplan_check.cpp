@@ -51,7 +51,7 @@ static void check_plan(const std::vector<uint64_t> &off, size_t eb, bool flt, ui
     size_t ti = 0;
     for (size_t li = 0; li < t.large.size(); ++li) {
         const SegSpan &s = t.large[li];
-        CHECK(s.idx == li && s.len > small_max);
+        CHECK(s.idx == ti && s.len > small_max);
         seen[s.base] += 1;
         // tiles: 0, 1, ... covering [0, len)
         const uint64_t ntiles = (s.len + tile - 1) / tile;
