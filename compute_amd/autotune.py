@@ -16,11 +16,12 @@ what varies is the tile shape and the work per thread):
                                        per round trip, register bound (waves per SIMD), tiles per XCD run
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
-  __bcmi_scan|q                        4-byte scans, items per lane
-  __bcmi_scan_f32|q, |acc              the compensated float scan: items per lane, accumulators
+  __bcmi_scan|q, |waves                4-byte scans: items per lane, waves per workgroup
+  __bcmi_scan_f32|q, |waves, |acc      the compensated float scan: items per lane, waves, accumulators
                                        (1: FP64, 0: TwoSum FP32 pairs)
-  __bcmi_reduce_by_key|q, |acc         reduce_by_key: items per lane, float accumulators (as above)
-  __bcmi_segscan|q, |acc               segmented scans: items per lane, float accumulators
+  __bcmi_reduce_by_key|q, |waves, |acc  reduce_by_key: items per lane, waves per workgroup, float
+                                       accumulators (as above)
+  __bcmi_segscan|q, |waves, |acc       segmented scans: the same
   __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
   __bcmi_segsort|small_max, |msd       segmented sort: largest LDS-sorted class (capacity 256 << c, also
                                        the MSD group size), MSD route for large segments on or off
@@ -45,8 +46,8 @@ SORT_SHAPES = [(256, 64, 1, 4, 0), (512, 24, 1, 4, 0), (512, 32, 1, 4, 0), (1024
 SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "xcd_run")
 XCD_RUNS = (4, 8)  # consecutive tiles per XCD (0, dispatch order, measured slower on every shape)
 HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]
-SCAN_Q = (4, 8, 16)
-SEGSCAN_Q = (2, 4, 8)
+SCAN_SHAPES = ((4, 8), (4, 16), (8, 8), (8, 16))  # (waves per workgroup, chunks per lane)
+SEGSCAN_SHAPES = ((4, 4), (4, 8), (8, 8))  # (waves per workgroup, chunks per lane)
 
 
 def _median_ms(ca, q, fn, restore=None, reps=7):
@@ -165,16 +166,18 @@ def tune_scan(ca, W, q, log2n, log):
     dx = ca.DeviceArray.from_host(x, q)
     do = ca.DeviceArray(n, np.int32, q)
     best = None
-    for qq in SCAN_Q:
+    for nw, qq in SCAN_SHAPES:
+        _set(ca, "__bcmi_scan", "waves", nw)
         _set(ca, "__bcmi_scan", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.scan(q, dx, do, True, init=0))
         ok = bool(np.array_equal(do.to_host(q), expect))
-        log({"tunable": "__bcmi_scan", "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_scan", "waves": nw, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, qq)
+            best = (ms, nw, qq)
     if best is None:
         raise RuntimeError("autotune: no scan candidate matched")
-    _set(ca, "__bcmi_scan", "q", best[1])
+    _set(ca, "__bcmi_scan", "waves", best[1])
+    _set(ca, "__bcmi_scan", "q", best[2])
 
     # the compensated float inclusive scan, within the double-double bound
     xf = W.lognormal_f32(n)
@@ -184,18 +187,20 @@ def tune_scan(ca, W, q, log2n, log):
     dxf = ca.DeviceArray.from_host(xf, q)
     dof = ca.DeviceArray(n, np.float32, q)
     best = None
-    for acc, qq in [(a, b) for a in (0, 1) for b in SCAN_Q]:
+    for acc, (nw, qq) in [(a, b) for a in (0, 1) for b in SCAN_SHAPES]:
         _set(ca, "__bcmi_scan_f32", "acc", acc)
+        _set(ca, "__bcmi_scan_f32", "waves", nw)
         _set(ca, "__bcmi_scan_f32", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.scan(q, dxf, dof, False))
         ok = bool(np.all(np.abs(dof.to_host(q).astype(np.float64) - dd) <= bound))
-        log({"tunable": "__bcmi_scan_f32", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_scan_f32", "acc": acc, "waves": nw, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, acc, qq)
+            best = (ms, acc, nw, qq)
     if best is None:
         raise RuntimeError("autotune: no compensated float scan candidate within the bound")
     _set(ca, "__bcmi_scan_f32", "acc", best[1])
-    _set(ca, "__bcmi_scan_f32", "q", best[2])
+    _set(ca, "__bcmi_scan_f32", "waves", best[2])
+    _set(ca, "__bcmi_scan_f32", "q", best[3])
 
 
 SEGSORT_CANDIDATES = [(4, 1), (3, 1), (2, 1), (4, 0), (3, 0)]  # (largest small class, MSD route)
@@ -280,21 +285,23 @@ def tune_segscan(ca, W, q, log2n, log):
     ko, vo = ca.DeviceArray(n, np.int32, q), ca.DeviceArray(n, np.float32, q)
     cnt = ca.DeviceArray(1, np.uint64, q)
     best = None
-    for acc, qq in [(a, b) for a in (0, 1) for b in SEGSCAN_Q]:
+    for acc, (nw, qq) in [(a, b) for a in (0, 1) for b in SEGSCAN_SHAPES]:
         _set(ca, "__bcmi_reduce_by_key", "acc", acc)
+        _set(ca, "__bcmi_reduce_by_key", "waves", nw)
         _set(ca, "__bcmi_reduce_by_key", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.check(L.bcmi_reduce_by_key(q.h, ca.I32, ca.F32, ca.PLUS, dk.ptr, dv.ptr, n,
                                                                         ko.ptr, vo.ptr, cnt.ptr, 0)))
         ok = int(cnt.to_host(q)[0]) == runs and bool(np.array_equal(ko.to_host(q, runs), last_keys))
         got = vo.to_host(q, runs).astype(np.float64)
         ok = ok and bool(np.all(np.abs(got - sums) <= 2.0 ** -23 * np.abs(sums) + 2.0 ** -40 * mags))
-        log({"tunable": "__bcmi_reduce_by_key", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_reduce_by_key", "acc": acc, "waves": nw, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, acc, qq)
+            best = (ms, acc, nw, qq)
     if best is None:
         raise RuntimeError("autotune: no reduce_by_key candidate matched")
     _set(ca, "__bcmi_reduce_by_key", "acc", best[1])
-    _set(ca, "__bcmi_reduce_by_key", "q", best[2])
+    _set(ca, "__bcmi_reduce_by_key", "waves", best[2])
+    _set(ca, "__bcmi_reduce_by_key", "q", best[3])
     # segmented exclusive scan of the same runs (each run restarts at 0),
     # accumulated run by run in float64: step t adds the t-th element of
     # every run longer than t (runs ordered by length, so a prefix of them)
@@ -313,19 +320,21 @@ def tune_segscan(ca, W, q, log2n, log):
         acc[:c] += v64[pos]
         amag[:c] += np.abs(v64[pos])
     best = None
-    for acc, qq in [(a, b) for a in (0, 1) for b in SEGSCAN_Q]:
+    for acc, (nw, qq) in [(a, b) for a in (0, 1) for b in SEGSCAN_SHAPES]:
         _set(ca, "__bcmi_segscan", "acc", acc)
+        _set(ca, "__bcmi_segscan", "waves", nw)
         _set(ca, "__bcmi_segscan", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.segmented_scan(q, dk, dv, vo, True, init=0.0))
         got = vo.to_host(q).astype(np.float64)
         ok = bool(np.all(np.abs(got - ex) <= 2.0 ** -23 * np.abs(ex) + 2.0 ** -36 * emag))
-        log({"tunable": "__bcmi_segscan", "acc": acc, "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_segscan", "acc": acc, "waves": nw, "q": qq, "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, acc, qq)
+            best = (ms, acc, nw, qq)
     if best is None:
         raise RuntimeError("autotune: no segmented scan candidate matched")
     _set(ca, "__bcmi_segscan", "acc", best[1])
-    _set(ca, "__bcmi_segscan", "q", best[2])
+    _set(ca, "__bcmi_segscan", "waves", best[2])
+    _set(ca, "__bcmi_segscan", "q", best[3])
 
 
 def main(argv=None):

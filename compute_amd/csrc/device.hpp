@@ -245,14 +245,23 @@ namespace bcmi {
 // loaded and stored coalesced) transposed through LDS so that lane l holds
 // E = Q*4 consecutive positions l*E .. l*E+E-1, and back.  The buffer has
 // one row of E words per lane, padded by 4 words so that the 16-byte
-// accesses of both directions spread over the banks.  All four waves of the
-// block call these together (they contain block barriers).
-template <int Q>
+// accesses of both directions spread over the banks.  Each wave has a
+// buffer of its own, so a transpose synchronises the wave only: one wave's
+// LDS operations execute in order, and wave_lds_sync keeps the compiler from
+// moving the reads above the writes.
+template <int Q, int NW = 4>
 struct BlkBuf {
     static constexpr int E = Q * 4;
     static constexpr int ROW = E + 4;
-    uint32_t w[4][64 * ROW];
+    uint32_t w[NW][64 * ROW];
 };
+
+BDEV void wave_lds_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
 
 // striped chunks of one wave -> this lane's row
 template <int Q, typename T>
@@ -270,7 +279,7 @@ BDEV void to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t 
         c.w = __builtin_bit_cast(uint32_t, s[q][3]);
         *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
     }
-    block_barrier();
+    wave_lds_sync();
 #pragma unroll
     for (int e = 0; e < E; e += 4) {
         const uint4 c = *(const uint4 *)(buf + lane * ROW + e);
@@ -279,7 +288,7 @@ BDEV void to_blocked(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t 
         b[e + 2] = __builtin_bit_cast(T, c.z);
         b[e + 3] = __builtin_bit_cast(T, c.w);
     }
-    block_barrier();
+    wave_lds_sync();
 }
 
 // this lane's row -> striped chunks of one wave
@@ -296,7 +305,7 @@ BDEV void to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t 
         c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
         *(uint4 *)(buf + lane * ROW + e) = c;
     }
-    block_barrier();
+    wave_lds_sync();
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         const int p = q * 256 + (int)lane * 4;
@@ -306,7 +315,7 @@ BDEV void to_striped(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t 
         s[q][2] = __builtin_bit_cast(T, c.z);
         s[q][3] = __builtin_bit_cast(T, c.w);
     }
-    block_barrier();
+    wave_lds_sync();
 }
 
 // ------------------------------------------- wave-parallel decoupled look-back

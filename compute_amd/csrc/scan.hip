@@ -21,7 +21,6 @@
 
 namespace bcmi {
 
-constexpr int kScanThreads = 256;
 
 // Integer / plain policy.  Narrow types compute in 32 bits (DPP moves 32 or
 // 64 bits) and truncate on output, which keeps modular results exact.
@@ -103,26 +102,26 @@ struct alignas(16) sc_chunk {
     T v[N];
 };
 
-template <typename P>
+template <typename P, int NW>
 struct ScanLds {
-    typename P::S wave[4];
+    typename P::S wave[NW];
     typename P::Cr prefix;
 };
 
-// One tile: Q 16-byte chunks per thread (tile = 4 waves * 64 lanes * Q
+// One tile: Q 16-byte chunks per thread (tile = NW waves * 64 lanes * Q
 // chunks), wave-striped: chunk q of lane l of wave w covers elements
 // w*WSEG + q*64*V + l*V .. +V-1 of the tile.  FULL: the tile lies below n, so
 // no element needs a bounds test.  VEC: input and output are 16-byte
 // aligned, so full chunks move as one 16-byte load / store.
-template <typename T, typename P, bool VEC, int Q, bool FULL, typename Src>
-BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
+template <typename T, typename P, bool VEC, int Q, int NW, bool FULL, typename Src>
+BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
                     typename P::Cr init, int exclusive, uint32_t epoch)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
     constexpr int V = 16 / sizeof(T);
     constexpr int WSEG = Q * 64 * V;  // elements per wave
-    constexpr int TILE = 4 * WSEG;    // 4 waves
+    constexpr int TILE = NW * WSEG;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t tile = blockIdx.x;
@@ -172,7 +171,9 @@ BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, 
     S wpre = P::id();
     for (uint32_t i = 0; i < w; ++i) wpre = P::merge(wpre, L.wave[i]);
     if (w == 0) {
-        const S total = P::merge(P::merge(L.wave[0], L.wave[1]), P::merge(L.wave[2], L.wave[3]));
+        S total = L.wave[0];
+#pragma unroll
+        for (int i = 1; i < NW; ++i) total = P::merge(total, L.wave[i]);
         const Cr prefix = lookback_scan<Cr>(status, tile, epoch, P::carry(total), init, P::cid(),
                                             [](Cr a, Cr b) { return P::cmerge(a, b); }, ctrl);
         if (lane == 0) L.prefix = prefix;
@@ -211,44 +212,47 @@ BDEV void scan_body(ScanLds<P> &L, const Src src, T *out, size_t n, Ctrl *ctrl, 
     }
 }
 
-template <typename T, typename P, bool VEC, int Q, typename Src>
-__global__ __launch_bounds__(kScanThreads) void k_scan(const Src src, T *out, size_t n, Ctrl *ctrl,
-                                                       uint64_t *status, typename P::Cr init, int exclusive,
-                                                       uint32_t back)
+// NW waves per workgroup (4, or 8: a tile twice as big, half the tiles and
+// look-backs).
+template <typename T, typename P, bool VEC, int Q, typename Src, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_scan(const Src src, T *out, size_t n, Ctrl *ctrl,
+                                                  uint64_t *status, typename P::Cr init, int exclusive,
+                                                  uint32_t back)
 {
-    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
-    __shared__ ScanLds<P> lds;
+    constexpr size_t TILE = (size_t)NW * Q * 64 * (16 / sizeof(T));
+    __shared__ ScanLds<P, NW> lds;
     const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        scan_body<T, P, VEC, Q, true>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, NW, true>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
     else
-        scan_body<T, P, VEC, Q, false>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, NW, false>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
 }
 
-template <typename T, typename P, int Q, typename Src>
+template <typename T, typename P, int Q, typename Src, int NW = 4>
 static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t *status, typename P::Cr ini,
                        int exclusive)
 {
-    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(T));
+    constexpr size_t TILE = (size_t)NW * Q * 64 * (16 / sizeof(T));
     size_t tiles = (n + TILE - 1) / TILE;
     BCMI_CHECK(tiles < (1ull << 31), BCMI_INVALID_BUFFER_SIZE, "scan: n=%zu too large", n);
     bool vec = (((uintptr_t)src.ptr | (uintptr_t)out) & 15) == 0;
     BCMI_TRY(begin_lookback(q, 1));
     if (vec) {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, true, Q, Src>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                    hipLaunchKernelGGL((k_scan<T, P, true, Q, Src, NW>), dim3((unsigned)tiles), dim3(NW * 64), 0,
                                        q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     } else {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, false, Q, Src>), dim3((unsigned)tiles), dim3(kScanThreads), 0,
+                    hipLaunchKernelGGL((k_scan<T, P, false, Q, Src, NW>), dim3((unsigned)tiles), dim3(NW * 64), 0,
                                        q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     }
     return 0;
 }
 
 // Chunks per thread come from the tuning cache (key "__bcmi_scan", or
-// "__bcmi_scan_f32" for the compensated float scan; param "q": 4, 8 or 16)
-// for 4-byte array scans; 4 otherwise.
+// "__bcmi_scan_f32" for the compensated float scan; param "q": 4, 8 or 16;
+// param "waves": 4, or 8 with q = 8 or 16) for 4-byte array scans; 4
+// chunks and 4 waves otherwise.
 template <typename T, typename P, typename Src>
 static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, const void *init, int exclusive)
 {
@@ -262,7 +266,12 @@ static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, cons
         // the compensated float scan holds a TwoSum pair per chunk, so it has
         // its own chunk count (fewer chunks: fewer VGPRs, more resident tiles)
         const bool comp = std::is_same<P, ScanCompF32>::value || std::is_same<P, ScanF64AccF32>::value;
-        const unsigned qq = tune_get(comp ? "__bcmi_scan_f32" : "__bcmi_scan", "q", 16);
+        const char *key = comp ? "__bcmi_scan_f32" : "__bcmi_scan";
+        const unsigned qq = tune_get(key, "q", 16);
+        if (tune_get(key, "waves", 4) == 8) {
+            if (qq == 16) return launch_scan<T, P, 16, Src, 8>(q, src, out, n, status, ini, exclusive);
+            if (qq == 8) return launch_scan<T, P, 8, Src, 8>(q, src, out, n, status, ini, exclusive);
+        }
         if (qq == 16) return launch_scan<T, P, 16>(q, src, out, n, status, ini, exclusive);
         if (qq == 8) return launch_scan<T, P, 8>(q, src, out, n, status, ini, exclusive);
     }

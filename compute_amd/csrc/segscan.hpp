@@ -282,15 +282,15 @@ BDEV void segscan_body(SegLds<P> &L, const KT *__restrict__ keys, const VT *vals
 // in device.hpp) so that each lane folds E = Q*4 consecutive elements and the
 // wave needs one scan of the folds instead of one per chunk: for the
 // compensated pairs that is most of the arithmetic.
-template <typename P, int Q>
+template <typename P, int Q, int NW>
 struct SegLdsBlk {
-    Seg<typename P::A> wave[4];
+    Seg<typename P::A> wave[NW];
     Seg<typename P::CA> prefix;
-    BlkBuf<Q> t;
+    BlkBuf<Q, NW> t;
 };
 
-template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, bool FULL>
-BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
+template <typename KT, typename VT, typename P, int MODE, int Q, int NW, bool VEC, bool FULL>
+BDEV void segscan_body_blk(SegLdsBlk<P, Q, NW> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                            KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
                            uint64_t *count_out, uint32_t epoch)
 {
@@ -302,7 +302,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     constexpr int V = 4;
     constexpr int E = Q * V;  // elements per lane
     constexpr int WSEG = Q * 64 * V;
-    constexpr int TILE = 4 * WSEG;
+    constexpr int TILE = NW * WSEG;
 
     const uint32_t tid = threadIdx.x, lane = tid & 63, w = tid >> 6;
     const uint32_t tile = blockIdx.x;
@@ -362,7 +362,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     auto mrg = [](S a, S b) { return seg_merge<A, P>(a, b); };
     S t = sid;
     VT y[MODE == SEG_RBK ? 1 : E];
-    uint32_t *row = buf + lane * BlkBuf<Q>::ROW;
+    uint32_t *row = buf + lane * BlkBuf<Q, NW>::ROW;
     A tfh = P::id();
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -385,7 +385,9 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     S wpre = sid;
     for (uint32_t i = 0; i < w; ++i) wpre = mrg(wpre, L.wave[i]);
     if (w == 0) {
-        const S total = mrg(mrg(L.wave[0], L.wave[1]), mrg(L.wave[2], L.wave[3]));
+        S total = L.wave[0];
+#pragma unroll
+        for (int i = 1; i < NW; ++i) total = mrg(total, L.wave[i]);
         const C agg{P::carry(total.v), total.n};
         const C cid{P::cid(), 0u};
         const C prefix = lookback_scan<C>(status, tile, epoch, agg, cid, cid, [](C a, C b) {
@@ -463,24 +465,28 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q> &L, const KT *__restrict__ keys, cons
     }
 }
 
-template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC>
-__global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
-                                                         KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
-                                                         uint64_t *status, VT init, uint64_t *count_out,
-                                                         uint32_t back)
+// NW waves per workgroup: 4, or 8 for the blocked form (a tile twice as big
+// at the same elements per lane: half the tiles, so half the look-backs and
+// per-tile latencies per element; 16 waves measured slower again).
+template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, int NW = 4>
+__global__ __launch_bounds__(NW * 64) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
+                                                     KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
+                                                     uint64_t *status, VT init, uint64_t *count_out,
+                                                     uint32_t back)
 {
-    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
+    constexpr size_t TILE = (size_t)NW * Q * 64 * (16 / sizeof(VT));
     const bool full = (size_t)(blockIdx.x + 1) * TILE <= n;
     const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4 && Q <= 8) {
-        __shared__ SegLdsBlk<P, Q> lds;
+        __shared__ SegLdsBlk<P, Q, NW> lds;
         if (full)
-            segscan_body_blk<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                            count_out, epoch);
+            segscan_body_blk<KT, VT, P, MODE, Q, NW, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status,
+                                                                init, count_out, epoch);
         else
-            segscan_body_blk<KT, VT, P, MODE, Q, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
-                                                             count_out, epoch);
+            segscan_body_blk<KT, VT, P, MODE, Q, NW, VEC, false>(lds, keys, vals, out, keys_out, n, ctrl, status,
+                                                                 init, count_out, epoch);
     } else {
+        static_assert(NW == 4, "the striped form runs 4 waves");
         __shared__ SegLds<P> lds;
         if (full)
             segscan_body<KT, VT, P, MODE, Q, VEC, true>(lds, keys, vals, out, keys_out, n, ctrl, status, init,
@@ -491,30 +497,31 @@ __global__ __launch_bounds__(kSegThreads) void k_segscan(const KT *__restrict__ 
     }
 }
 
-template <typename KT, typename VT, typename P, int MODE, int Q>
+template <typename KT, typename VT, typename P, int MODE, int Q, int NW = 4>
 static int launch_seg(bcmi_queue q, const void *keys, const void *vals, void *out, void *keys_out, size_t n,
                       uint64_t *status, VT ini, uint64_t *count_out)
 {
-    constexpr size_t TILE = (size_t)4 * Q * 64 * (16 / sizeof(VT));
+    constexpr size_t TILE = (size_t)NW * Q * 64 * (16 / sizeof(VT));
     size_t tiles = (n + TILE - 1) / TILE;
     const bool vec = (((uintptr_t)keys | (uintptr_t)vals | (MODE == SEG_RBK ? 0 : (uintptr_t)out)) & 15) == 0;
     BCMI_TRY(begin_lookback(q, 1));
     if (vec)
         BCMI_LAUNCH(q, MODE == SEG_RBK ? "reduce_by_key" : "segmented_scan",
-                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, true>), dim3((unsigned)tiles), dim3(kSegThreads),
-                                       0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out, (KT *)keys_out, n,
-                                       q->ctrl, status, ini, count_out, 1u));
+                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, true, NW>), dim3((unsigned)tiles),
+                                       dim3(NW * 64), 0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out,
+                                       (KT *)keys_out, n, q->ctrl, status, ini, count_out, 1u));
     else
         BCMI_LAUNCH(q, MODE == SEG_RBK ? "reduce_by_key" : "segmented_scan",
-                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, false>), dim3((unsigned)tiles),
-                                       dim3(kSegThreads), 0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out,
+                    hipLaunchKernelGGL((k_segscan<KT, VT, P, MODE, Q, false, NW>), dim3((unsigned)tiles),
+                                       dim3(NW * 64), 0, q->stream, (const KT *)keys, (const VT *)vals, (VT *)out,
                                        (KT *)keys_out, n, q->ctrl, status, ini, count_out, 1u));
     return 0;
 }
 
 // Chunks per thread from the tuning cache (key "__bcmi_reduce_by_key" or
 // "__bcmi_segscan", param "q": 2, 4 or 8) for 4-byte keys with 4-byte
-// values; 4 otherwise.
+// values, and waves per workgroup (param "waves": 4, or 8 with q = 8);
+// 4 chunks and 4 waves otherwise.
 template <typename KT, typename VT, typename P, int MODE>
 static int run_seg(bcmi_queue q, const void *keys, const void *vals, void *out, void *keys_out, size_t n,
                    const void *init, uint64_t *count_out)
@@ -527,7 +534,10 @@ static int run_seg(bcmi_queue q, const void *keys, const void *vals, void *out, 
     VT ini{};
     if (init) memcpy(&ini, init, sizeof(VT));
     if constexpr (sizeof(KT) == 4 && sizeof(VT) == 4) {
-        const unsigned qq = tune_get(MODE == SEG_RBK ? "__bcmi_reduce_by_key" : "__bcmi_segscan", "q", 8);
+        const char *key = MODE == SEG_RBK ? "__bcmi_reduce_by_key" : "__bcmi_segscan";
+        const unsigned qq = tune_get(key, "q", 8);
+        if (qq == 8 && tune_get(key, "waves", 4) == 8)
+            return launch_seg<KT, VT, P, MODE, 8, 8>(q, keys, vals, out, keys_out, n, status, ini, count_out);
         if (qq == 8) return launch_seg<KT, VT, P, MODE, 8>(q, keys, vals, out, keys_out, n, status, ini, count_out);
         if (qq == 2) return launch_seg<KT, VT, P, MODE, 2>(q, keys, vals, out, keys_out, n, status, ini, count_out);
     }

@@ -694,6 +694,95 @@ def test_segmented_sort_matches_per_segment_calls(queue, dt, vdt):
         assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), desc
 
 
+@pytest.mark.parametrize("waves,q", [(4, 4), (4, 8), (4, 16), (8, 8), (8, 16)])
+def test_scan_tunable_shapes(queue, waves, q):
+    """Every (waves per workgroup, chunks per lane) shape the tuner may pick
+    for the 4-byte scans: integer scans exact (exclusive with an init, in
+    place, unaligned), the compensated float scan within its bound with both
+    accumulator forms, whole and partial tiles."""
+    L = ca.lib()
+    keys_t = ("__bcmi_scan", "__bcmi_scan_f32")
+    params = ("waves", "q", "acc")
+    saved = {k: [_tune_get(k, p) for p in params] for k in keys_t}
+    try:
+        for k in keys_t:
+            ca.check(L.bcmi_tune_set(k.encode(), b"waves", waves))
+            ca.check(L.bcmi_tune_set(k.encode(), b"q", q))
+        for n in (1, 5000, 65536 * 3 + 7, (1 << 21) + 13):
+            x = RNG.integers(-1000, 1000, n).astype(np.int32)
+            d = dev(queue, x)
+            out = ca.DeviceArray(n, np.int32, queue)
+            ca.scan(queue, d, out, True, init=7)
+            exp = np.empty(n, np.int64)
+            exp[0] = 7
+            exp[1:] = 7 + np.cumsum(x[:-1], dtype=np.int64)
+            assert np.array_equal(out.to_host(queue), exp.astype(np.int32)), n
+            ca.scan(queue, d, d, False)
+            assert np.array_equal(d.to_host(queue), np.cumsum(x, dtype=np.int64).astype(np.int32)), n
+            if n > 2:
+                # input at +1 element, output at +2: neither 16-byte aligned
+                d = dev(queue, x)
+                m = n - 2
+                ca.check(L.bcmi_scan(queue.h, ca.I32, ca.PLUS, d.at(1), out.at(2), m, None, 0, 0))
+                assert np.array_equal(out.to_host(queue)[2:], O.scan(x[1:1 + m], False)), n
+            xf = O.gen_lognormal_f32(n)
+            fo = ca.DeviceArray(n, np.float32, queue)
+            for acc in (0, 1):
+                ca.check(L.bcmi_tune_set(b"__bcmi_scan_f32", b"acc", acc))
+                ca.scan(queue, dev(queue, xf), fo, False)
+                dd = O.scan_f32_dd(xf, False)
+                assert np.all(np.abs(fo.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(xf, dd)), (n, acc)
+    finally:
+        for k, vals_ in saved.items():
+            for p, val, d in zip(params, vals_, (4, 16, 1)):
+                ca.check(L.bcmi_tune_set(k.encode(), p.encode(), val if val != 0xFFFFFFFF else d))
+
+
+@pytest.mark.parametrize("waves,q", [(4, 2), (4, 4), (4, 8), (8, 8)])
+def test_segmented_scan_tunable_shapes(queue, waves, q):
+    """Every (waves per workgroup, chunks per lane) shape the tuner may pick
+    for reduce_by_key and the segmented scans: integer results exact, float
+    sums within the compensated bound, whole and partial tiles."""
+    L = ca.lib()
+    keys_t = ("__bcmi_reduce_by_key", "__bcmi_segscan")
+    params = ("waves", "q")
+    saved = {k: [_tune_get(k, p) for p in params] for k in keys_t}
+    try:
+        for k in keys_t:
+            ca.check(L.bcmi_tune_set(k.encode(), b"waves", waves))
+            ca.check(L.bcmi_tune_set(k.encode(), b"q", q))
+        for n in (1, 4097, 100003, (1 << 20) + 17):
+            lens = RNG.integers(1, 70, n)
+            keys = np.repeat(np.arange(lens.size, dtype=np.int32), lens)[:n]
+            vals = RNG.integers(-100, 100, n).astype(np.int32)
+            ko = ca.DeviceArray(n, np.int32, queue)
+            vo = ca.DeviceArray(n, np.int32, queue)
+            r = ca.reduce_by_key(queue, dev(queue, keys), dev(queue, vals), ko, vo)
+            ek, ev = O.reduce_by_key(keys, vals)
+            assert r == ek.size and np.array_equal(ko.to_host(queue)[:r], ek) and \
+                np.array_equal(vo.to_host(queue)[:r], ev), n
+            for excl in (False, True):
+                ca.segmented_scan(queue, dev(queue, keys), dev(queue, vals), vo, excl, init=4 if excl else None)
+                assert np.array_equal(vo.to_host(queue), O.segmented_scan(keys, vals, excl, 4 if excl else 0)), n
+            fk, fv = O.gen_runs_i32_f32(n)
+            fo = ca.DeviceArray(n, np.float32, queue)
+            r = ca.reduce_by_key(queue, dev(queue, fk), dev(queue, fv), ko, fo)
+            ek, es = O.reduce_by_key_f32_dd(fk, fv)
+            assert r == ek.size and np.array_equal(ko.to_host(queue)[:r], ek), n
+            absum = np.add.reduceat(np.abs(fv.astype(np.float64)), np.r_[0, np.flatnonzero(np.diff(fk)) + 1])
+            bound = np.spacing(np.abs(es).astype(np.float32)).astype(np.float64) + 2.0 ** -50 * absum
+            assert np.all(np.abs(fo.to_host(queue)[:r].astype(np.float64) - es) <= bound), n
+            ca.segmented_scan(queue, dev(queue, fk), dev(queue, fv), fo, True, init=0.0)
+            dd = O.segmented_scan_f32_dd(fk, fv, True)
+            bound = np.spacing(np.abs(dd).astype(np.float32)).astype(np.float64) + \
+                2.0 ** -50 * np.cumsum(np.abs(fv.astype(np.float64)))
+            assert np.all(np.abs(fo.to_host(queue).astype(np.float64) - dd) <= bound), n
+    finally:
+        for k, vals_ in saved.items():
+            for p, val, d in zip(params, vals_, (4, 8)):
+                ca.check(L.bcmi_tune_set(k.encode(), p.encode(), val if val != 0xFFFFFFFF else d))
+
+
 @pytest.mark.parametrize("acc", [0, 1])
 def test_float_accumulator_policies(queue, acc):
     """Both float accumulator policies the tuner may pick (1: FP64
