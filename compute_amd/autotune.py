@@ -42,7 +42,8 @@ DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 # register bound in waves per SIMD (0: none)): the combinations built
 SORT_SHAPES = [(256, 64, 1, 4, 0), (512, 24, 1, 4, 0), (512, 32, 1, 4, 0), (1024, 16, 1, 4, 0), (256, 64, 2, 4, 0),
                (512, 24, 2, 4, 0), (512, 32, 2, 4, 0), (512, 24, 3, 4, 0), (256, 64, 2, 2, 0), (512, 32, 2, 2, 0),
-               (256, 64, 2, 2, 4), (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8)]
+               (256, 64, 2, 2, 4), (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8),
+               (512, 64, 2, 2, 4), (512, 64, 4, 2, 4), (1024, 32, 2, 2, 0), (1024, 32, 4, 2, 0)]
 SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "xcd_run")
 XCD_RUNS = (4, 8)  # consecutive tiles per XCD (0, dispatch order, measured slower on every shape)
 HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]

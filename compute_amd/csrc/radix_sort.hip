@@ -424,6 +424,10 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
             BCMI_SHAPE_LB(512, 24, 2, 2, 8);
             BCMI_SHAPE_LB(512, 32, 2, 2, 7);
             BCMI_SHAPE_LB(512, 16, 1, 4, 8);
+            BCMI_SHAPE_LB(512, 64, 2, 2, 4);
+            BCMI_SHAPE_LB(512, 64, 4, 2, 4);
+            BCMI_SHAPE_LB(1024, 32, 2, 2, 0);
+            BCMI_SHAPE_LB(1024, 32, 4, 2, 0);
         }
 #undef BCMI_SHAPE_LB
 #define BCMI_SHAPE(T, I, S) \

@@ -634,7 +634,9 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
 
 
 @pytest.mark.parametrize("threads,ipt,split,lb,waves", [(256, 64, 2, 2, 0), (512, 32, 2, 2, 0), (256, 64, 2, 2, 4),
-                                                       (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8)])
+                                                       (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8),
+                                                       (512, 64, 2, 2, 4), (512, 64, 4, 2, 4), (1024, 32, 2, 2, 0),
+                                                       (1024, 32, 4, 2, 0)])
 def test_sort_lookback_and_register_bound_variants(queue, threads, ipt, split, lb, waves):
     """The look-back windows and register bounds the tuner may select:
     keys and pairs, many tiles (the walk crosses windows), XCD-grouped."""
