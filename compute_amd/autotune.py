@@ -16,6 +16,7 @@ what varies is the tile shape and the work per thread):
                                        per round trip, register bound (waves per SIMD), tiles per XCD run
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
+  __bcmi_radix_hist_by_key|(the same params)  the same for sorts with values
   __bcmi_scan|q, |waves                4-byte scans: items per lane, waves per workgroup
   __bcmi_scan_f32|q, |waves, |acc      the compensated float scan: items per lane, waves, accumulators
                                        (1: FP64, 0: TwoSum FP32 pairs)
@@ -69,6 +70,35 @@ def _set(ca, key, param, value):
     ca.check(ca.lib().bcmi_tune_set(key.encode(), param.encode(), int(value)))
 
 
+def _tune_hist(ca, q, key, run, restore, check, log):
+    """The histogram kernel's shape for one sort, timed inside that sort
+    (queue profiler)."""
+    best = None
+    for u, nw, b, g in HIST_SHAPES:
+        _set(ca, key, "unroll", u)
+        _set(ca, key, "subs", nw)
+        _set(ca, key, "blocks_per_cu", b)
+        _set(ca, key, "bank_groups", g)
+        q.profile_reset()
+        q.profile(True)
+        for _ in range(5):
+            restore()
+            run()
+        q.profile(False)
+        cnt, tot = q.profile_query("radix_histogram")
+        q.profile_reset()
+        ms = tot / max(cnt, 1)
+        ok = check()
+        log({"tunable": key, "unroll": u, "subs": nw, "blocks_per_cu": b, "bank_groups": g,
+             "ms": round(ms, 4), "ok": ok})
+        if ok and (best is None or ms < best[0]):
+            best = (ms, u, nw, b, g)
+    if best is None:
+        raise RuntimeError("autotune: no histogram shape produced the sorted output")
+    for p, v in zip(("unroll", "subs", "blocks_per_cu", "bank_groups"), best[1:]):
+        _set(ca, key, p, v)
+
+
 def tune_sort(ca, W, q, log2n, log):
     n = 1 << log2n
     L = ca.lib()
@@ -82,33 +112,8 @@ def tune_sort(ca, W, q, log2n, log):
     if best is None:
         raise RuntimeError("autotune: no radix sort candidate produced the sorted output")
 
-    # the histogram kernel, timed inside the tuned sort (queue profiler)
-    best = None
-    for u, nw, b, g in HIST_SHAPES:
-        _set(ca, "__bcmi_radix_hist", "unroll", u)
-        _set(ca, "__bcmi_radix_hist", "subs", nw)
-        _set(ca, "__bcmi_radix_hist", "blocks_per_cu", b)
-        _set(ca, "__bcmi_radix_hist", "bank_groups", g)
-        q.profile_reset()
-        q.profile(True)
-        for _ in range(5):
-            restore()
-            ca.sort(q, work)
-        q.profile(False)
-        cnt, tot = q.profile_query("radix_histogram")
-        q.profile_reset()
-        ms = tot / max(cnt, 1)
-        ok = bool(np.array_equal(work.to_host(q), expect))
-        log({"tunable": "__bcmi_radix_hist", "unroll": u, "subs": nw, "blocks_per_cu": b, "bank_groups": g,
-             "ms": round(ms, 4), "ok": ok})
-        if ok and (best is None or ms < best[0]):
-            best = (ms, u, nw, b, g)
-    if best is None:
-        raise RuntimeError("autotune: no histogram shape produced the sorted output")
-    _set(ca, "__bcmi_radix_hist", "unroll", best[1])
-    _set(ca, "__bcmi_radix_hist", "subs", best[2])
-    _set(ca, "__bcmi_radix_hist", "blocks_per_cu", best[3])
-    _set(ca, "__bcmi_radix_hist", "bank_groups", best[4])
+    _tune_hist(ca, q, "__bcmi_radix_hist", lambda: ca.sort(q, work), restore,
+               lambda: bool(np.array_equal(work.to_host(q), expect)), log)
 
     k, v = W.sort_by_key_f32(n)
     sk, sv = ca.DeviceArray.from_host(k, q), ca.DeviceArray.from_host(v, q)
@@ -136,6 +141,9 @@ def tune_sort(ca, W, q, log2n, log):
                         check_pairs, log)
     if best is None:
         raise RuntimeError("autotune: no sort_by_key candidate produced a stable sorted output")
+    # the float keys' skewed top digit favours another histogram shape
+    _tune_hist(ca, q, "__bcmi_radix_hist_by_key", lambda: ca.sort(q, wk, wv, api=ca.SORT_BY_KEY), restore2,
+               check_pairs, log)
 
 
 def _tune_shapes(ca, q, key, run, restore, check, log):

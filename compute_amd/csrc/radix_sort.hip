@@ -132,21 +132,25 @@ __global__ __launch_bounds__(kHistThreads) void k_histogram(const KB *__restrict
 
 // The histogram's shape: chunks in flight per thread, sub-histograms per
 // block (LDS: subs * key bytes KB), blocks per CU, bank-grouped layout.
-// Tuning cache key "__bcmi_radix_hist", params "unroll" (1, 2, 4, 8), "subs"
+// Tuning cache keys "__bcmi_radix_hist" and, for sorts with values,
+// "__bcmi_radix_hist_by_key"; params "unroll" (1, 2, 4, 8), "subs"
 // (8 or 16; 8 for 8-byte keys), "blocks_per_cu" and "bank_groups" (0, 1);
 // the defaults were the fastest measured on 2^27 uint32 keys.
 struct HistShape {
     int unroll, subs, blocks_per_cu, bank_groups;
 };
 
-static HistShape hist_shape(size_t key_bytes, bool tuned)
+static HistShape hist_shape(size_t key_bytes, bool tuned, bool values)
 {
     HistShape h{4, 16, 2, 0};
     if (tuned) {
-        h.unroll = (int)tune_get("__bcmi_radix_hist", "unroll", 4);
-        h.subs = (int)tune_get("__bcmi_radix_hist", "subs", 16);
-        h.blocks_per_cu = (int)tune_get("__bcmi_radix_hist", "blocks_per_cu", 2);
-        h.bank_groups = tune_get("__bcmi_radix_hist", "bank_groups", 0) ? 1 : 0;
+        // sorts with values read "__bcmi_radix_hist_by_key" where it is set
+        // (their benchmark's float keys have a skewed top digit)
+        const char *k0 = "__bcmi_radix_hist", *k1 = values ? "__bcmi_radix_hist_by_key" : k0;
+        h.unroll = (int)tune_get(k1, "unroll", tune_get(k0, "unroll", 4));
+        h.subs = (int)tune_get(k1, "subs", tune_get(k0, "subs", 16));
+        h.blocks_per_cu = (int)tune_get(k1, "blocks_per_cu", tune_get(k0, "blocks_per_cu", 2));
+        h.bank_groups = tune_get(k1, "bank_groups", tune_get(k0, "bank_groups", 0)) ? 1 : 0;
     }
     if (h.unroll != 1 && h.unroll != 2 && h.unroll != 8) h.unroll = 4;
     if (h.subs != 8 || key_bytes >= 8) h.subs = key_bytes >= 8 ? 8 : 16;
@@ -303,7 +307,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
         const size_t per_cu = per_cu_lds < (size_t)(2048 / TH) ? per_cu_lds : (size_t)(2048 / TH);
         if (xrun > 16 || (size_t)q->cu_count * per_cu < 16 * (size_t)xrun) xrun = 0;
     }
-    const HistShape hs = hist_shape(sizeof(KB), work == nullptr);
+    const HistShape hs = hist_shape(sizeof(KB), work == nullptr, VB > 0);
     uint32_t hblocks = (n + kHistThreads * 32 - 1) / (kHistThreads * 32);
     uint32_t hmax = (uint32_t)(q->cu_count * hs.blocks_per_cu);
     if (hblocks > hmax) hblocks = hmax;

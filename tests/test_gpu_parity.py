@@ -594,7 +594,8 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
     defaults = {"threads": 512, "ipt": 32, "fit_size": 1, "xcd_run": 0, "split": 1}
     saved = {k: [_tune_get(k, p) for p in params] for k in ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")}
     hparams = ("unroll", "subs", "blocks_per_cu", "bank_groups")
-    hsaved = [_tune_get("__bcmi_radix_hist", p) for p in hparams]
+    hkeys = ("__bcmi_radix_hist", "__bcmi_radix_hist_by_key")
+    hsaved = {hk: [_tune_get(hk, p) for p in hparams] for hk in hkeys}
     hshape = {(256, 16): (1, 8, 4, 1), (256, 32): (2, 16, 2, 0), (256, 64): (8, 8, 2, 1), (512, 16): (4, 16, 4, 1),
               (512, 24): (8, 16, 2, 0), (512, 32): (4, 16, 2, 1), (1024, 16): (2, 8, 8, 0)}[(threads, ipt)]
     xrun = {(256, 16): 2, (256, 32): 8, (256, 64): 4, (512, 16): 8, (512, 24): 4, (512, 32): 2,
@@ -609,8 +610,9 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
             ca.check(L.bcmi_tune_set(key.encode(), b"fit_size", 0))  # the variant at every size
             ca.check(L.bcmi_tune_set(key.encode(), b"xcd_run", xrun))
             ca.check(L.bcmi_tune_set(key.encode(), b"split", split))
-        for p, val in zip(hparams, hshape):
-            ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val))
+        for hk in hkeys:
+            for p, val in zip(hparams, hshape):
+                ca.check(L.bcmi_tune_set(hk.encode(), p.encode(), val))
         for n in (70001, (1 << 20) + 333, 5000011):
             k = random_keys("uint32", n)
             d = dev(queue, k)
@@ -629,8 +631,10 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
             for p, val in zip(params, vals):
                 # an absent entry reads back as the sentinel: restore the built-in default
                 ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val if val != 0xFFFFFFFF else defaults[p]))
-        for p, val, dflt in zip(hparams, hsaved, (4, 16, 2, 0)):
-            ca.check(L.bcmi_tune_set(b"__bcmi_radix_hist", p.encode(), val if val != 0xFFFFFFFF else dflt))
+        for hk in hkeys:
+            for p, val, dflt in zip(hparams, hsaved[hk], (4, 16, 2, 0)):
+                if val != 0xFFFFFFFF or hk == "__bcmi_radix_hist":
+                    ca.check(L.bcmi_tune_set(hk.encode(), p.encode(), val if val != 0xFFFFFFFF else dflt))
 
 
 @pytest.mark.parametrize("threads,ipt,split,lb,waves", [(256, 64, 2, 2, 0), (512, 32, 2, 2, 0), (256, 64, 2, 2, 4),
