@@ -159,7 +159,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
 {
     using V = typename ValT<VB>::type;
     constexpr int NSUB = THREADS / 16, TILE = THREADS * IPT, PARTS = THREADS / 256, SPP = NSUB / PARTS;
-    static_assert(THREADS % 256 == 0 && TILE <= 32768, "256-multiple blocks, tile positions in 15 bits");
+    static_assert(THREADS % 256 == 0 && TILE <= 65536, "256-multiple blocks, tile positions below 2^16");
     static_assert(IPT % SPLIT == 0, "whole slots per staging round");
     uint32_t *words = (uint32_t *)L.stage;
     KB *s_keys = (KB *)L.stage;

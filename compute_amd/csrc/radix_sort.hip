@@ -419,8 +419,9 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         // look-back windows other than 4 words per round trip, for the
         // tuned shapes; any other combination takes the plain shape
 #define BCMI_SHAPE_LB(T, I, S, W, E)                                                                        \
-        if (th == T && ipt == I && split == S && lb == W && wpe == E)                                      \
-            return radix_sort<KB, VB, T, I, S, W, E>(q, keys, vals, n, kind, desc, work, xrun)
+        if constexpr (sizeof(QSweepLds<T, I, KB, VB, S>) <= (size_t)kLdsBytesPerCU)                         \
+            if (th == T && ipt == I && split == S && lb == W && wpe == E)                                  \
+                return radix_sort<KB, VB, T, I, S, W, E>(q, keys, vals, n, kind, desc, work, xrun)
         if (tuned_shape) {
             BCMI_SHAPE_LB(256, 64, 2, 2, 0);
             BCMI_SHAPE_LB(512, 32, 2, 2, 0);
