@@ -456,6 +456,7 @@ struct bcmi_segplan_s {
     uint64_t total = 0;  // elements covered by the offsets
     int small_cls = 4;      // largest small class: capacity kSegSmallThreads << small_cls
     int msd = 1;            // large segments may take the MSD route
+    int ipt_large = 0;      // keys per thread of a large segment's onesweep tile
     uint32_t nslots = 0;     // group slots of the MSD route, over all large segments
     uint32_t nsmall[5] = {0, 0, 0, 0, 0};  // classes IPT 1, 2, 4, 8, 16
     uint32_t ninsert = 0;
@@ -485,9 +486,14 @@ struct bcmi_segplan_s {
 
 namespace bcmi {
 
+// Keys per thread of the large segments' onesweep tiles (256 threads): 8,
+// 12 or 16 by element width; for 8-byte keys or values the tuning cache
+// (key "__bcmi_segsort", param "ipt") may choose 24 (twice the tile, half
+// the tiles).
 static int large_ipt(int kb, size_t vb)
 {
     size_t sb = (size_t)kb > vb ? (size_t)kb : vb;
+    if (sb == 8 && tune_get("__bcmi_segsort", "ipt", 12) == 24) return 24;
     return sb >= 16 ? 8 : (sb >= 8 ? 12 : 16);
 }
 
@@ -529,13 +535,11 @@ static int run_plan_small(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT
 }
 
 // The large segments: top-digit route decision, MSD groups, LSD passes.
-template <typename KB, int VB, typename XF>
+template <typename KB, int VB, typename XF, int IPTL>
 static int run_plan_large(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT<VB>::type *vals, XF xf)
 {
     using V = typename ValT<VB>::type;
     constexpr int P = sizeof(KB);
-    constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
-    constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
     if (pl->nlarge) {
         // the top digit's histogram and offsets decide each segment's route
         // (MSD needs two digits or more); the other digits are counted only
@@ -630,7 +634,19 @@ static int run_plan_xf(bcmi_queue q, bcmi_segplan pl, void *keys_, void *vals_, 
         const hipError_t e = hipEventRecord(pl->join, pl->aux);
         if (e != hipSuccess && !st) st = hip_fail(e, "segmented sort: join record", __FILE__, __LINE__);
     }
-    if (!st) st = run_plan_large<KB, VB, XF>(q, pl, keys, vals, xf);
+    if (!st) {
+        constexpr int SB = (int)sizeof(KB) > VB ? (int)sizeof(KB) : VB;
+        constexpr int IPTL = SB >= 16 ? 8 : (SB >= 8 ? 12 : 16);
+        // the tile size the plan's tables were built for
+        bool ran = false;
+        if constexpr (SB == 8) {
+            if (pl->ipt_large == 24) {
+                st = run_plan_large<KB, VB, XF, 24>(q, pl, keys, vals, xf);
+                ran = true;
+            }
+        }
+        if (!ran) st = run_plan_large<KB, VB, XF, IPTL>(q, pl, keys, vals, xf);
+    }
     if (forked) {
         const hipError_t e = hipStreamWaitEvent(q->stream, pl->join, 0);
         if (e != hipSuccess && !st) st = hip_fail(e, "segmented sort: join", __FILE__, __LINE__);
@@ -678,7 +694,8 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
                BCMI_INVALID_VALUE, "segmented plan: value size %zu unsupported", value_bytes);
     const int kb = dtype_size(key_dtype);
     const bool flt = key_dtype == BCMI_F32 || key_dtype == BCMI_F64;
-    const uint64_t tile = (uint64_t)kSortThreads * large_ipt(kb, value_bytes);
+    const int ipt_large = large_ipt(kb, value_bytes);
+    const uint64_t tile = (uint64_t)kSortThreads * ipt_large;
     SegTables tb;
     std::string err;
     // tuning cache key "__bcmi_segsort": "small_max" caps the small classes
@@ -699,6 +716,7 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     pl->value_bytes = value_bytes;
     pl->descending = descending ? 1 : 0;
     pl->msd = tune_get("__bcmi_segsort", "msd", 1) ? 1 : 0;
+    pl->ipt_large = ipt_large;
     pl->total = total;
     pl->nlarge = (uint32_t)large.size();
     pl->ntiles = (uint32_t)tiles.size();
