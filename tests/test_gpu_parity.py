@@ -167,6 +167,45 @@ def test_sort_keys_bit_exact(queue, dt, n):
             assert np.array_equal(bits(d.to_host(queue)), bits(exp)), (api, desc)
 
 
+@pytest.mark.parametrize("dt", ["uint32", "int32", "float32", "uint64"])
+@pytest.mark.parametrize("pattern", ["one value", "three values", "top byte fixed", "low byte fixed", "two hot digits"])
+def test_sort_skewed_digits(queue, dt, pattern):
+    """Skewed digits: few distinct keys, one byte fixed (only that pass
+    skewed), two hot digits (same-address LDS atomics in counting and
+    ranking); keys only and with values, both directions, checked against
+    the oracle's stable radix order."""
+    n = 3000017
+    u = random_keys("uint64", n).view(np.uint64)
+    if pattern == "one value":
+        k = np.full(n, 0x1234, np.uint64)
+    elif pattern == "three values":
+        k = np.array([7, 0xABCD, 0x5555_0000], np.uint64)[RNG.integers(0, 3, n)]
+    elif pattern == "top byte fixed":
+        k = (u & np.uint64(0x00FF_FFFF)) | np.uint64(0x5A00_0000)
+    elif pattern == "low byte fixed":
+        k = (u & ~np.uint64(0xFF)) | np.uint64(0x3C)
+    else:
+        k = np.where(RNG.random(n) < 0.4, np.uint64(0x11), np.uint64(0xEE)) | (u & ~np.uint64(0xFF)) * \
+            (RNG.random(n) < 0.5).astype(np.uint64)
+    dtn = np.dtype(dt)
+    if dtn.kind == "f":
+        k = (k & np.uint64(0xFFFF_FFFF)).astype(np.uint32).view(np.float32)
+        k = np.where(np.isnan(k), np.float32(1.5), k).astype(np.float32)
+    else:
+        k = (k & np.uint64((1 << (8 * dtn.itemsize)) - 1)).astype(dtn.str.replace("i", "u")).view(dtn)
+    v = np.arange(n, dtype=np.uint32)
+    for desc in (False, True):
+        d = dev(queue, k)
+        ca.sort(queue, d, api=ca.SORT, descending=desc)
+        exp = O.sort_dispatch(ca.SORT, k, descending=desc)
+        assert np.array_equal(bits(d.to_host(queue)), bits(exp)), (pattern, desc)
+        dk, dv = dev(queue, k), dev(queue, v)
+        ca.sort(queue, dk, dv, api=ca.SORT_BY_KEY, descending=desc)
+        ek, ev = O.sort_dispatch(ca.SORT_BY_KEY, k, v, descending=desc)
+        assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), (pattern, desc)
+        assert np.array_equal(dv.to_host(queue), ev), (pattern, desc)
+
+
 @pytest.mark.parametrize("dt", ["uint32", "int32", "float32", "float64", "uint64", "int8", "uint16"])
 @pytest.mark.parametrize("vdt", ["uint8", "uint16", "uint32", "uint64"])
 @pytest.mark.parametrize("n", [5, 31, 32, 1000, 5000, 40000])
