@@ -11,9 +11,10 @@ the path in BCMI_TUNING_CACHE, else compute_amd/tuning/gfx950.json.
 
 Tunables (the digit width is fixed at 8 bits by the per-digit LDS words;
 what varies is the tile shape and the work per thread):
-  __bcmi_radix_sort|threads, |ipt, |split, |lookback, |waves, |xcd_run  keys-only radix sort: workgroup
-                                       size, keys per thread, staging rounds (4-byte keys), look-back words
-                                       per round trip, register bound (waves per SIMD), tiles per XCD run
+  __bcmi_radix_sort|threads, |ipt, |split, |lookback, |waves, |count_bg, |xcd_run  keys-only radix
+                                       sort: workgroup size, keys per thread, staging rounds (4-byte keys),
+                                       look-back words per round trip, register bound (waves per SIMD),
+                                       bank-grouped count words, tiles per XCD run
   __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_radix_hist_by_key|(the same params)  the same for sorts with values
@@ -41,11 +42,13 @@ DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
 # (threads, keys per thread, staging rounds, look-back words per round trip,
 # register bound in waves per SIMD (0: none)): the combinations built
-SORT_SHAPES = [(256, 64, 1, 4, 0), (512, 24, 1, 4, 0), (512, 32, 1, 4, 0), (1024, 16, 1, 4, 0), (256, 64, 2, 4, 0),
-               (512, 24, 2, 4, 0), (512, 32, 2, 4, 0), (512, 24, 3, 4, 0), (256, 64, 2, 2, 0), (512, 32, 2, 2, 0),
-               (256, 64, 2, 2, 4), (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8),
-               (512, 64, 2, 2, 4), (512, 64, 4, 2, 4), (1024, 32, 2, 2, 0), (1024, 32, 4, 2, 0)]
-SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "xcd_run")
+SORT_SHAPES = [(256, 64, 1, 4, 0, 0), (512, 24, 1, 4, 0, 0), (512, 32, 1, 4, 0, 0), (1024, 16, 1, 4, 0, 0),
+               (256, 64, 2, 4, 0, 0), (512, 24, 2, 4, 0, 0), (512, 32, 2, 4, 0, 0), (512, 24, 3, 4, 0, 0),
+               (256, 64, 2, 2, 0, 0), (512, 32, 2, 2, 0, 0), (256, 64, 2, 2, 4, 0), (512, 24, 2, 2, 8, 0),
+               (512, 32, 2, 2, 7, 0), (512, 16, 1, 4, 8, 0), (512, 64, 2, 2, 4, 0), (512, 64, 4, 2, 4, 0),
+               (1024, 32, 2, 2, 0, 0), (1024, 32, 4, 2, 0, 0),
+               (512, 64, 2, 2, 4, 1), (256, 64, 2, 2, 4, 1), (512, 32, 2, 2, 0, 1), (512, 32, 2, 2, 7, 1)]
+SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "count_bg", "xcd_run")
 XCD_RUNS = (4, 8)  # consecutive tiles per XCD (0, dispatch order, measured slower on every shape)
 HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]
 SCAN_SHAPES = ((4, 8), (4, 16), (8, 8), (8, 16))  # (waves per workgroup, chunks per lane)

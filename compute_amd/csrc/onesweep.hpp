@@ -151,7 +151,7 @@ BDEV uint32_t digit_lookback(const uint64_t *status, uint32_t tile, uint32_t d, 
     }
 }
 
-template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, typename XF, bool FULL>
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, typename XF, bool FULL, int CBG>
 BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__restrict__ kin, KB *__restrict__ kout,
                       const typename ValT<VB>::type *__restrict__ vin, typename ValT<VB>::type *__restrict__ vout,
                       uint32_t n, uint32_t tile, uint32_t shift, const uint32_t *__restrict__ goff0, XF xf, Ctrl *ctrl,
@@ -165,12 +165,29 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
     KB *s_keys = (KB *)L.stage;
     V *s_vals = (V *)L.stage;
     const uint32_t tid = threadIdx.x, ql = tid & 15, sub = tid >> 4;
+    // Count words, 256 per sub.  CBG = 0: sub s's digit d at s * 256 + d, so
+    // digit d sits in bank d mod 64 in every sub and the 64 lanes of a wave
+    // contend for the same 64 banks.  CBG = 1 (bank groups): the 4 subs of a
+    // wave share 1024 words as 16 rows of 64, sub sw owning columns
+    // [16 sw, 16 sw + 16), digit d at row d / 16, column 16 sw + d % 16, so a
+    // sub's 16 lanes contend among 16 banks of their own.
+    auto cw = [](uint32_t s, uint32_t d) -> uint32_t {
+        if constexpr (CBG) return (s >> 2) * 1024 + (d >> 4) * 64 + (s & 3) * 16 + (d & 15);
+        else return s * 256 + d;
+    };
     // each sub zeroes its own 256 count words (16 per lane, four 16-byte
     // stores); only the sub's lanes touch them before the barrier that ends
     // the counting, and one wave's LDS operations execute in order
-    uint32_t *my = words + sub * 256;
+    uint32_t *my = words + cw(sub, 0);
+    auto mw = [&](uint32_t d) -> uint32_t * {
+        if constexpr (CBG) return my + (d >> 4) * 64 + (d & 15);
+        else return my + d;
+    };
 #pragma unroll
-    for (int i = 0; i < 4; ++i) ((uint4 *)my)[i * 16 + ql] = make_uint4(0, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) {
+        if constexpr (CBG) *(uint4 *)(my + (i * 4 + (ql >> 2)) * 64 + (ql & 3) * 4) = make_uint4(0, 0, 0, 0);
+        else ((uint4 *)my)[i * 16 + ql] = make_uint4(0, 0, 0, 0);
+    }
     const uint32_t base = tile * (uint32_t)TILE;
     const int32_t lim = FULL ? TILE : (int32_t)(n - base) - (int32_t)(sub * 16 * IPT);
     auto valid = [&](int j) { return FULL || (int32_t)(j * 16 + ql) < lim; };
@@ -189,7 +206,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
 #pragma unroll
     for (int j = 0; j < IPT; ++j)
         if (valid(j))
-            __hip_atomic_fetch_add(&my[xf.digit(k[j], shift)], 1u << 16, __ATOMIC_RELAXED,
+            __hip_atomic_fetch_add(mw(xf.digit(k[j], shift)), 1u << 16, __ATOMIC_RELAXED,
                                    __HIP_MEMORY_SCOPE_WORKGROUP);
     block_barrier();
     // 2. thread (part, d) scans digit d over subs [part * SPP, part * SPP + SPP)
@@ -198,7 +215,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
     uint32_t psum = 0;
 #pragma unroll
     for (int i = 0; i < SPP; ++i) {
-        c[i] = words[(part * SPP + i) * 256 + d] >> 16;
+        c[i] = words[cw(part * SPP + i, d)] >> 16;
         psum += c[i];
     }
     uint32_t total = psum, before = 0;
@@ -227,7 +244,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
         uint32_t s0 = dstart + before;
 #pragma unroll
         for (int i = 0; i < SPP; ++i) {
-            words[(part * SPP + i) * 256 + d] = s0 << 16;
+            words[cw(part * SPP + i, d)] = s0 << 16;
             s0 += c[i];
         }
     }
@@ -244,7 +261,7 @@ BDEV void qsweep_body(QSweepLds<THREADS, IPT, KB, VB, SPLIT> &L, const KB *__res
         // the digit is recomputed here rather than kept from the counting
         // loop: keeping all IPT word addresses alive across the scan would
         // cost IPT more VGPRs per lane (occupancy is register-bound)
-        uint32_t *a = &my[xf.digit(opaque_reg(k[j]), shift)];
+        uint32_t *a = mw(xf.digit(opaque_reg(k[j]), shift));
         if (ok) __hip_atomic_fetch_or(a, qbit, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint32_t wd = __hip_atomic_load(a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         const uint32_t run = wd >> 16;
@@ -358,7 +375,7 @@ BDEV uint32_t xcd_tile(uint32_t b, uint32_t xrun)
 // no bound beyond the block size).  Occupancy of this kernel is bound by
 // its VGPRs: a bound of 4 fits four 256-thread workgroups per CU in 128
 // VGPRs, 8 fits four 512-thread workgroups in 64.
-template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, int WPE, typename XF>
+template <int THREADS, int IPT, typename KB, int VB, int SPLIT, int LB, int WPE, typename XF, int CBG = 0>
 __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) void k_qsweep(const KB *__restrict__ kin, KB *__restrict__ kout,
                                                     const typename ValT<VB>::type *__restrict__ vin,
                                                     typename ValT<VB>::type *__restrict__ vout, uint32_t n,
@@ -369,10 +386,10 @@ __global__ __launch_bounds__(THREADS) __attribute__((amdgpu_waves_per_eu(WPE))) 
     const uint32_t tile = xcd_tile(blockIdx.x, xrun);
     const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((uint64_t)(tile + 1) * (THREADS * IPT) <= n)
-        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, true>(lds, kin, kout, vin, vout, n, tile, shift,
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, true, CBG>(lds, kin, kout, vin, vout, n, tile, shift,
                                                            ctrl->offsets[pass], xf, ctrl, status, epoch);
     else
-        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, false>(lds, kin, kout, vin, vout, n, tile, shift,
+        qsweep_body<THREADS, IPT, KB, VB, SPLIT, LB, XF, false, CBG>(lds, kin, kout, vin, vout, n, tile, shift,
                                                             ctrl->offsets[pass], xf, ctrl, status, epoch);
 }
 

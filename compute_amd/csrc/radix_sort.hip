@@ -290,7 +290,7 @@ size_t radix_scratch_bytes(size_t n, size_t key_bytes, size_t value_bytes, int c
     return align256(n * key_bytes) + align256(n * value_bytes) + align256((size_t)cu_count * 8 * key_bytes * 256 * 4);
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT, int LB, int WPE, typename XF>
+template <typename KB, int VB, int TH, int IPT, int SPLIT, int LB, int WPE, int CBG, typename XF>
 static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF xf, const SortWork *work,
                          unsigned xrun)
 {
@@ -353,7 +353,7 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     V *vs = vals, *vd = vtmp;
     for (int p = 0; p < P; ++p) {
         BCMI_LAUNCH(q, "radix_onesweep",
-                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, LB, WPE, XF>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
+                    hipLaunchKernelGGL((k_qsweep<TH, IPT, KB, VB, SPLIT, LB, WPE, XF, CBG>), dim3(tiles), dim3(TH), 0, q->stream, ks, kd,
                                        vs, vd, n, (uint32_t)(8 * p), (uint32_t)p, xf, q->ctrl, status,
                                        (uint32_t)(P - p), (uint32_t)xrun));
         std::swap(ks, kd);
@@ -367,12 +367,12 @@ static int radix_sort_xf(bcmi_queue q, void *keys_, void *vals_, uint32_t n, XF 
     return 0;
 }
 
-template <typename KB, int VB, int TH, int IPT, int SPLIT = 1, int LB = 4, int WPE = 1>
+template <typename KB, int VB, int TH, int IPT, int SPLIT = 1, int LB = 4, int WPE = 1, int CBG = 0>
 static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind, int descending,
                       const SortWork *work, unsigned xrun)
 {
     return with_key_xf<KB>(kind, descending, [&](auto xf) {
-        return radix_sort_xf<KB, VB, TH, IPT, SPLIT, LB, WPE>(q, keys, vals, n, xf, work, xrun);
+        return radix_sort_xf<KB, VB, TH, IPT, SPLIT, LB, WPE, CBG>(q, keys, vals, n, xf, work, xrun);
     });
 }
 
@@ -381,7 +381,8 @@ static int radix_sort(bcmi_queue q, void *keys, void *vals, uint32_t n, int kind
 // (key "__bcmi_radix_sort", or "__bcmi_radix_sort_by_key" with values;
 // params "threads", "ipt", "split", "lookback" (predecessor words per
 // look-back round trip) and "waves" (a register bound, waves per SIMD; 0 =
-// none), chosen on 2^27 keys; the lookback / waves combinations built are
+// none), "count_bg" (1: bank-grouped count words, built for the best
+// shapes), chosen on 2^27 keys; the lookback / waves combinations built are
 // listed below; smaller sorts take
 // smaller tiles to fill the chip (param "fit_size" = 0 keeps the tuned shape
 // at every size).  Other key and value widths take a fixed shape whose
@@ -400,6 +401,7 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         unsigned split = work ? 1 : tune_get(key, "split", 1);
         const unsigned lb = work ? 4 : tune_get(key, "lookback", 4);
         const unsigned wpe = work ? 0 : tune_get(key, "waves", 0);
+        const unsigned cbg = work ? 0 : tune_get(key, "count_bg", 0);
         bool tuned_shape = true;
         if (work || tune_get(key, "fit_size", 1) != 0) {
             if (n <= (1u << 20)) {
@@ -420,9 +422,19 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
         // tuned shapes; any other combination takes the plain shape
 #define BCMI_SHAPE_LB(T, I, S, W, E)                                                                        \
         if constexpr (sizeof(QSweepLds<T, I, KB, VB, S>) <= (size_t)kLdsBytesPerCU)                         \
-            if (th == T && ipt == I && split == S && lb == W && wpe == E)                                  \
+            if (th == T && ipt == I && split == S && lb == W && wpe == E && cbg == 0)                      \
                 return radix_sort<KB, VB, T, I, S, W, E>(q, keys, vals, n, kind, desc, work, xrun)
+        // bank-grouped count words (param "count_bg" = 1) for the shapes
+        // that tune best
+#define BCMI_SHAPE_CB(T, I, S, W, E)                                                                        \
+        if constexpr (sizeof(QSweepLds<T, I, KB, VB, S>) <= (size_t)kLdsBytesPerCU)                         \
+            if (th == T && ipt == I && split == S && lb == W && wpe == E && cbg == 1)                      \
+                return radix_sort<KB, VB, T, I, S, W, E, 1>(q, keys, vals, n, kind, desc, work, xrun)
         if (tuned_shape) {
+            BCMI_SHAPE_CB(512, 64, 2, 2, 4);
+            BCMI_SHAPE_CB(256, 64, 2, 2, 4);
+            BCMI_SHAPE_CB(512, 32, 2, 2, 0);
+            BCMI_SHAPE_CB(512, 32, 2, 2, 7);
             BCMI_SHAPE_LB(256, 64, 2, 2, 0);
             BCMI_SHAPE_LB(512, 32, 2, 2, 0);
             BCMI_SHAPE_LB(256, 64, 2, 2, 4);
@@ -435,6 +447,7 @@ static int radix_sort_shape(bcmi_queue q, void *keys, void *vals, uint32_t n, in
             BCMI_SHAPE_LB(1024, 32, 4, 2, 0);
         }
 #undef BCMI_SHAPE_LB
+#undef BCMI_SHAPE_CB
 #define BCMI_SHAPE(T, I, S) \
         if (th == T && ipt == I && split == S) return radix_sort<KB, VB, T, I, S>(q, keys, vals, n, kind, desc, work, xrun)
         BCMI_SHAPE(256, 16, 1);

@@ -676,21 +676,23 @@ def test_sort_tunable_tile_variants(queue, threads, ipt, split):
                     ca.check(L.bcmi_tune_set(hk.encode(), p.encode(), val if val != 0xFFFFFFFF else dflt))
 
 
-@pytest.mark.parametrize("threads,ipt,split,lb,waves", [(256, 64, 2, 2, 0), (512, 32, 2, 2, 0), (256, 64, 2, 2, 4),
-                                                       (512, 24, 2, 2, 8), (512, 32, 2, 2, 7), (512, 16, 1, 4, 8),
-                                                       (512, 64, 2, 2, 4), (512, 64, 4, 2, 4), (1024, 32, 2, 2, 0),
-                                                       (1024, 32, 4, 2, 0)])
-def test_sort_lookback_and_register_bound_variants(queue, threads, ipt, split, lb, waves):
-    """The look-back windows and register bounds the tuner may select:
-    keys and pairs, many tiles (the walk crosses windows), XCD-grouped."""
+@pytest.mark.parametrize("threads,ipt,split,lb,waves,cbg", [
+    (256, 64, 2, 2, 0, 0), (512, 32, 2, 2, 0, 0), (256, 64, 2, 2, 4, 0), (512, 24, 2, 2, 8, 0), (512, 32, 2, 2, 7, 0),
+    (512, 16, 1, 4, 8, 0), (512, 64, 2, 2, 4, 0), (512, 64, 4, 2, 4, 0), (1024, 32, 2, 2, 0, 0), (1024, 32, 4, 2, 0, 0),
+    (512, 64, 2, 2, 4, 1), (256, 64, 2, 2, 4, 1), (512, 32, 2, 2, 0, 1), (512, 32, 2, 2, 7, 1)])
+def test_sort_lookback_and_register_bound_variants(queue, threads, ipt, split, lb, waves, cbg):
+    """The look-back windows, register bounds and count-word layouts the
+    tuner may select: keys and pairs, many tiles (the walk crosses windows),
+    XCD-grouped."""
     L = ca.lib()
-    params = ("threads", "ipt", "split", "lookback", "waves", "xcd_run", "fit_size")
+    params = ("threads", "ipt", "split", "lookback", "waves", "xcd_run", "fit_size", "count_bg")
     keys = ("__bcmi_radix_sort", "__bcmi_radix_sort_by_key")
     saved = {k: [_tune_get(k, p) for p in params] for k in keys}
-    defaults = {"threads": 512, "ipt": 32, "split": 1, "lookback": 4, "waves": 0, "xcd_run": 0, "fit_size": 1}
+    defaults = {"threads": 512, "ipt": 32, "split": 1, "lookback": 4, "waves": 0, "xcd_run": 0, "fit_size": 1,
+                "count_bg": 0}
     try:
         for key in keys:
-            for p, val in zip(params, (threads, ipt, split, lb, waves, 8, 0)):
+            for p, val in zip(params, (threads, ipt, split, lb, waves, 8, 0, cbg)):
                 ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val))
         for n in (70001, 5000011):
             k = random_keys("uint32", n)
