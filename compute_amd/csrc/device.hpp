@@ -140,39 +140,8 @@ BDEV uint64_t lb_load(const uint64_t *p)
 // ((epoch - back) mod (2^30 - 1)) + 1, so every launch on the queue gets a
 // distinct non-zero epoch, graph replays included.  No per-block atomic is
 // involved: device-scope atomics on one address serialise across XCDs.
-BDEV uint32_t launch_epoch(const Ctrl *ctrl, uint32_t back)
-{
-    uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(&ctrl->epoch), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return ((e - back) % kEpochMask) + 1u;
-}
-
-// The same epoch, read where it is first needed.  The coherent load costs a
-// round trip to the device-wide cache; issued at kernel start it holds the
-// tile's data loads behind it.  `after` (a value computed from the tile's
-// data) makes the load's address depend on the data, so it is issued after
-// the data loads.
-// The low 32 bits of a value's representation (a data dependence to hand
-// to launch_epoch_after).
-template <typename D>
-BDEV uint32_t low_word(const D &v)
-{
-    static_assert(sizeof(D) >= 4, "at least 4 bytes");
-    uint32_t w;
-    memcpy(&w, &v, 4);
-    return w;
-}
-
-template <typename D>
-BDEV uint32_t launch_epoch_after(const Ctrl *ctrl, uint32_t back, D after)
-{
-    uint32_t z;
-    asm volatile("v_mov_b32 %0, 0 ; %1" : "=v"(z) : "v"(after));
-    const uint32_t *p = (const uint32_t *)((const char *)&ctrl->epoch + z);
-    const uint32_t e = __hip_atomic_load(const_cast<uint32_t *>(p), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return ((__builtin_amdgcn_readfirstlane(e) - back) % kEpochMask) + 1u;
-}
-
-// The same epoch as a plain load at kernel entry.  ctrl->epoch is written
+//
+// The epoch is a plain load at kernel entry.  ctrl->epoch is written
 // only by k_epoch_advance, an earlier kernel on the same stream, so it is
 // constant for this launch and visible to an ordinary load; with a uniform
 // address that load can be served by the scalar cache, off the vector

@@ -226,7 +226,7 @@ BDEV void onesweep_body(OnesweepLds<KB, VB, IPT> &L, const KB *__restrict__ kin,
     // per digit (thread tid == digit): wave prefixes and the tile count
     const uint32_t c0 = L.cnt[0][tid], c1 = L.cnt[1][tid], c2 = L.cnt[2][tid], c3 = L.cnt[3][tid];
     const uint32_t total = c0 + c1 + c2 + c3;
-    const uint32_t epoch = launch_epoch_after(ctrl, back, total);  // read after the tile's loads
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if (tile == 0) lb_store(&status[tid], lb_word(epoch, LB_P, total));
     else lb_store(&status[(size_t)tile * 256 + tid], lb_word(epoch, LB_A, total));
     uint32_t inc = wave_inclusive_scan(total, 0u, [](uint32_t a, uint32_t b) { return a + b; });
@@ -340,17 +340,5 @@ BDEV void onesweep_tile(const KB *__restrict__ kin, KB *__restrict__ kout,
 // compiler's choice.
 template <int VB, int IPT>
 constexpr int onesweep_waves_per_eu() { return (VB == 0 || (VB <= 4 && IPT <= 24)) ? 4 : 1; }
-
-template <typename KB, int VB, int IPT, int RANK, typename XF>
-__global__ __launch_bounds__(kSortThreads) __attribute__((amdgpu_waves_per_eu(onesweep_waves_per_eu<VB, IPT>())))
-void k_onesweep(const KB *__restrict__ kin, KB *__restrict__ kout,
-                                                           const typename ValT<VB>::type *__restrict__ vin,
-                                                           typename ValT<VB>::type *__restrict__ vout, uint32_t n,
-                                                           uint32_t shift, uint32_t pass, XF xf, Ctrl *ctrl,
-                                                           uint64_t *__restrict__ status, uint32_t back)
-{
-    onesweep_tile<KB, VB, IPT, RANK, XF>(kin, kout, vin, vout, n, blockIdx.x, shift, ctrl->offsets[pass], xf, ctrl,
-                                     status, back);
-}
 
 }  // namespace bcmi
