@@ -25,7 +25,7 @@ what varies is the tile shape and the work per thread):
                                        accumulators (as above)
   __bcmi_segscan|q, |waves, |acc       segmented scans: the same
   __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
-  __bcmi_segsort|small_max, |msd, |ipt  segmented sort: largest LDS-sorted class (capacity 256 << c, also
+  __bcmi_segsort|small_max, |msd, |ipt, |qrank  segmented sort: largest LDS-sorted class (capacity 256 << c, also
                                        the MSD group size), MSD route for large segments on or off
 
     python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
@@ -215,8 +215,9 @@ def tune_scan(ca, W, q, log2n, log):
     _set(ca, "__bcmi_scan_f32", "q", best[3])
 
 
-SEGSORT_CANDIDATES = [(4, 1, 12), (3, 1, 12), (2, 1, 12), (4, 0, 12), (3, 0, 12), (3, 1, 24), (4, 1, 24)]
-# (largest small class, MSD route, keys per thread of a large tile)
+SEGSORT_CANDIDATES = [(4, 1, 12, 0), (3, 1, 12, 0), (2, 1, 12, 0), (4, 0, 12, 0), (3, 0, 12, 0), (3, 1, 24, 0),
+                      (4, 1, 24, 0), (3, 1, 24, 1), (3, 1, 12, 1), (4, 1, 24, 1)]
+# (largest small class, MSD route, keys per thread of a large tile, quarter-wave large passes)
 
 
 def tune_segsort(ca, W, q, nseg, log):
@@ -236,23 +237,24 @@ def tune_segsort(ca, W, q, nseg, log):
 
     ref = None
     best = None
-    for cls, msd, ipt in SEGSORT_CANDIDATES:
+    for cls, msd, ipt, qr in SEGSORT_CANDIDATES:
         _set(ca, "__bcmi_segsort", "small_max", cls)
         _set(ca, "__bcmi_segsort", "msd", msd)
         _set(ca, "__bcmi_segsort", "ipt", ipt)
+        _set(ca, "__bcmi_segsort", "qrank", qr)
         plan = ca.SegmentedPlan(q, np.uint64, np.uint32, off)
         ms = _median_ms(ca, q, lambda: plan.run(q, wk, wv), restore)
         got = (wk.to_host(q), wv.to_host(q))
         if ref is None:
             ref = got
         ok = bool(np.array_equal(got[0], ref[0]) and np.array_equal(got[1], ref[1]))
-        log({"tunable": "__bcmi_segsort", "small_max": cls, "msd": msd, "ipt": ipt, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_segsort", "small_max": cls, "msd": msd, "ipt": ipt, "qrank": qr, "ms": round(ms, 4),
+             "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, cls, msd, ipt)
+            best = (ms, cls, msd, ipt, qr)
         del plan
-    _set(ca, "__bcmi_segsort", "small_max", best[1])
-    _set(ca, "__bcmi_segsort", "msd", best[2])
-    _set(ca, "__bcmi_segsort", "ipt", best[3])
+    for p, v in zip(("small_max", "msd", "ipt", "qrank"), best[1:]):
+        _set(ca, "__bcmi_segsort", p, v)
 
 
 REDUCE_CANDIDATES = [(2, 4), (4, 4), (8, 4), (2, 8), (4, 8), (8, 8)]  # (unroll, blocks per CU)

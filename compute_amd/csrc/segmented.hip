@@ -21,6 +21,7 @@
 //     restart at every segment's first tile.
 // A plan owns its tables and temporaries, so running it allocates nothing
 // and can be captured in a hipGraph and replayed.
+#include "onesweep.hpp"
 #include "radix.hpp"
 #include "segplan.hpp"
 
@@ -358,6 +359,37 @@ void k_onesweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
                                              ctrl, status + (size_t)(blockIdx.x - t.tile) * 256, back);
 }
 
+// The same pass on the quarter-wave digit-pass body (onesweep.hpp): 16-lane
+// subs rank by OR-ed peer masks in bank-grouped count words, the tile is
+// staged while its look-back completes.  Tile, look-back chain and offsets
+// are the segment's, as above.
+template <typename KB, int VB, int IPT, typename XF>
+__global__ __launch_bounds__(kSortThreads) void k_qsweep_seg(const KB *__restrict__ kin, KB *__restrict__ kout,
+                                                             const typename ValT<VB>::type *__restrict__ vin,
+                                                             typename ValT<VB>::type *__restrict__ vout,
+                                                             const SegTile *__restrict__ tiles, uint32_t shift,
+                                                             uint32_t pass, uint32_t passes,
+                                                             const uint32_t *__restrict__ seg_off, XF xf,
+                                                             Ctrl *ctrl, uint64_t *__restrict__ status,
+                                                             uint32_t back, uint32_t want)
+{
+    __shared__ QSweepLds<kSortThreads, IPT, KB, VB, 1> lds;
+    const SegTile t = tiles[blockIdx.x];
+    if (t.route != want) return;  // the segment takes the other route
+    const uint32_t epoch = launch_epoch_plain(ctrl, back);
+    const size_t b = t.base;
+    const uint32_t *goff = seg_off + ((size_t)t.seg * passes + pass) * 256;
+    uint64_t *st = status + (size_t)(blockIdx.x - t.tile) * 256;
+    if ((uint64_t)(t.tile + 1) * (kSortThreads * IPT) <= t.len)
+        qsweep_body<kSortThreads, IPT, KB, VB, 1, 2, XF, true, 1>(lds, kin + b, kout + b, VB ? vin + b : vin,
+                                                                VB ? vout + b : vout, t.len, t.tile, shift, goff, xf,
+                                                                ctrl, st, epoch);
+    else
+        qsweep_body<kSortThreads, IPT, KB, VB, 1, 2, XF, false, 1>(lds, kin + b, kout + b, VB ? vin + b : vin,
+                                                                 VB ? vout + b : vout, t.len, t.tile, shift, goff,
+                                                                 xf, ctrl, st, epoch);
+}
+
 // One block (256 threads, thread = digit) per large segment.  The top-digit
 // buckets are packed greedily into groups of at most `cap` elements: a
 // group that starts at bucket b ends before next(b), the first bucket whose
@@ -457,6 +489,7 @@ struct bcmi_segplan_s {
     int small_cls = 4;      // largest small class: capacity kSegSmallThreads << small_cls
     int msd = 1;            // large segments may take the MSD route
     int ipt_large = 0;      // keys per thread of a large segment's onesweep tile
+    int qrank = 0;          // 1: the large passes rank by quarter-waves (k_qsweep_seg)
     uint32_t nslots = 0;     // group slots of the MSD route, over all large segments
     uint32_t nsmall[5] = {0, 0, 0, 0, 0};  // classes IPT 1, 2, 4, 8, 16
     uint32_t ninsert = 0;
@@ -571,11 +604,24 @@ static int run_plan_large(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT
         BCMI_TRY(status_array(q, (size_t)pl->ntiles * 256, &status));
         BCMI_TRY(begin_lookback(q, P + 1));
         // MSD route: top-digit partition into the temporaries ...
-        BCMI_LAUNCH(q, "segsort_onesweep_msd",
-                    hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
-                                       q->stream, keys, (KB *)pl->ktmp, vals, (V *)pl->vtmp, pl->tiles,
-                                       (uint32_t)(8 * (P - 1)), (uint32_t)(P - 1), (uint32_t)P, pl->seg_off, xf,
-                                       q->ctrl, status, (uint32_t)(P + 1), (uint32_t)kRouteMsd));
+        // (param "qrank" = 1: the quarter-wave pass body)
+#define BCMI_SEG_PASS(NAME, KS, KD, VS, VD, SHIFT, PASS, BACK, WANT)                                            \
+        do {                                                                                                    \
+            if (pl->qrank)                                                                                      \
+                BCMI_LAUNCH(q, NAME,                                                                            \
+                            hipLaunchKernelGGL((k_qsweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles),              \
+                                               dim3(kSortThreads), 0, q->stream, KS, KD, VS, VD, pl->tiles,     \
+                                               SHIFT, PASS, (uint32_t)P, pl->seg_off, xf, q->ctrl, status,      \
+                                               BACK, WANT));                                                    \
+            else                                                                                                \
+                BCMI_LAUNCH(q, NAME,                                                                            \
+                            hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles),            \
+                                               dim3(kSortThreads), 0, q->stream, KS, KD, VS, VD, pl->tiles,     \
+                                               SHIFT, PASS, (uint32_t)P, pl->seg_off, xf, q->ctrl, status,      \
+                                               BACK, WANT));                                                    \
+        } while (0)
+        BCMI_SEG_PASS("segsort_onesweep_msd", keys, (KB *)pl->ktmp, vals, (V *)pl->vtmp, (uint32_t)(8 * (P - 1)),
+                      (uint32_t)(P - 1), (uint32_t)(P + 1), (uint32_t)kRouteMsd);
         // ... then every group sorted in LDS from the temporaries back into place
 #define BCMI_GROUPS(CLS, IPTV)                                                                                  \
     if constexpr ((int)(sizeof(KB) + VB) * kSegSmallThreads * IPTV <= kSegSmallLds) {                           \
@@ -595,14 +641,12 @@ static int run_plan_large(bcmi_queue q, bcmi_segplan pl, KB *keys, typename ValT
         KB *ks = keys, *kd = (KB *)pl->ktmp;
         V *vs = vals, *vd = (V *)pl->vtmp;
         for (int p = 0; p < P; ++p) {
-            BCMI_LAUNCH(q, "segsort_onesweep",
-                        hipLaunchKernelGGL((k_onesweep_seg<KB, VB, IPTL, XF>), dim3(pl->ntiles), dim3(kSortThreads), 0,
-                                           q->stream, ks, kd, vs, vd, pl->tiles, (uint32_t)(8 * p), (uint32_t)p,
-                                           (uint32_t)P, pl->seg_off, xf, q->ctrl, status, (uint32_t)(P - p),
-                                           (uint32_t)kRouteLsd));
+            BCMI_SEG_PASS("segsort_onesweep", ks, kd, vs, vd, (uint32_t)(8 * p), (uint32_t)p, (uint32_t)(P - p),
+                          (uint32_t)kRouteLsd);
             std::swap(ks, kd);
             std::swap(vs, vd);
         }
+#undef BCMI_SEG_PASS
         if (ks != keys)
             BCMI_LAUNCH(q, "segsort_copyback",
                         hipLaunchKernelGGL((k_seg_copyback<KB, VB>), dim3(pl->nlarge), dim3(256), 0, q->stream, keys,
@@ -717,6 +761,7 @@ extern "C" int bcmi_segmented_plan_create(bcmi_queue q, int key_dtype, size_t va
     pl->descending = descending ? 1 : 0;
     pl->msd = tune_get("__bcmi_segsort", "msd", 1) ? 1 : 0;
     pl->ipt_large = ipt_large;
+    pl->qrank = tune_get("__bcmi_segsort", "qrank", 0) ? 1 : 0;
     pl->total = total;
     pl->nlarge = (uint32_t)large.size();
     pl->ntiles = (uint32_t)tiles.size();

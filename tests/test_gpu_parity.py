@@ -866,18 +866,20 @@ def test_float_accumulator_policies(queue, acc):
             ca.check(L.bcmi_tune_set(k.encode(), b"acc", val if val != 0xFFFFFFFF else 1))
 
 
-@pytest.mark.parametrize("small_max,msd,ipt", [(2, 1, 12), (3, 0, 12), (0, 1, 12), (3, 1, 24), (2, 0, 24)])
-def test_segmented_sort_routing_thresholds(queue, small_max, msd, ipt):
-    """Every routing threshold and large-tile size the tuner may pick (small
-    classes capped, MSD route off, 12 or 24 keys per thread) gives the
-    per-segment result: segments between the cap and 4096 go to the large
+@pytest.mark.parametrize("small_max,msd,ipt,qrank", [(2, 1, 12, 0), (3, 0, 12, 0), (0, 1, 12, 0), (3, 1, 24, 0),
+                                                      (2, 0, 24, 0), (3, 1, 24, 1), (3, 0, 24, 1), (2, 1, 12, 1)])
+def test_segmented_sort_routing_thresholds(queue, small_max, msd, ipt, qrank):
+    """Every routing threshold, large-tile size and large-pass ranking the
+    tuner may pick (small classes capped, MSD route off, 12 or 24 keys per
+    thread, whole-wave or quarter-wave ranking) gives the per-segment result: segments between the cap and 4096 go to the large
     path and, with the MSD route, to its groups."""
     L = ca.lib()
-    saved = [_tune_get("__bcmi_segsort", p) for p in ("small_max", "msd", "ipt")]
+    saved = [_tune_get("__bcmi_segsort", p) for p in ("small_max", "msd", "ipt", "qrank")]
     try:
         ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"small_max", small_max))
         ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"msd", msd))
         ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"ipt", ipt))
+        ca.check(L.bcmi_tune_set(b"__bcmi_segsort", b"qrank", qrank))
         lens = [0, 1, 2, 31, 32, 200, 256, 257, 700, 1025, 2049, 4096, 4097, 9000, 30000]
         lens += list(RNG.integers(1, 5000, 40))
         off = _segments(lens)
@@ -892,7 +894,7 @@ def test_segmented_sort_routing_thresholds(queue, small_max, msd, ipt):
             assert np.array_equal(bits(dk.to_host(queue)), bits(ek)), dt
             assert np.array_equal(dv.to_host(queue), ev), dt
     finally:
-        for p, val, d in zip(("small_max", "msd", "ipt"), saved, (4, 1, 12)):
+        for p, val, d in zip(("small_max", "msd", "ipt", "qrank"), saved, (4, 1, 12, 0)):
             ca.check(L.bcmi_tune_set(b"__bcmi_segsort", p.encode(), val if val != 0xFFFFFFFF else d))
 
 
