@@ -15,7 +15,7 @@ what varies is the tile shape and the work per thread):
                                        sort: workgroup size, keys per thread, staging rounds (4-byte keys),
                                        look-back words per round trip, register bound (waves per SIMD),
                                        bank-grouped count words, tiles per XCD run
-  __bcmi_radix_sort_by_key|threads, |ipt, |split, |xcd_run  the same with 4-byte values
+  __bcmi_radix_sort_by_key|(the same params)  the same with 4-byte values
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_radix_hist_by_key|(the same params)  the same for sorts with values
   __bcmi_scan|q, |waves                4-byte scans: items per lane, waves per workgroup
@@ -25,8 +25,10 @@ what varies is the tile shape and the work per thread):
                                        accumulators (as above)
   __bcmi_segscan|q, |waves, |acc       segmented scans: the same
   __bcmi_reduce|unroll, |blocks_per_cu reduce: 16-byte loads in flight, occupancy
-  __bcmi_segsort|small_max, |msd, |ipt, |qrank  segmented sort: largest LDS-sorted class (capacity 256 << c, also
-                                       the MSD group size), MSD route for large segments on or off
+  __bcmi_segsort|small_max, |msd, |ipt, |qrank  segmented sort: largest LDS-sorted class (capacity 256 << c,
+                                       also the MSD group size), MSD route for large segments on or off, keys
+                                       per thread of a large segment's tile (8-byte elements: 12 or 24),
+                                       quarter-wave ranking in the large passes
 
     python -m compute_amd.autotune [--out PATH] [--log2n-sort 27] ...
 """
@@ -41,7 +43,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 DEFAULT_OUT = os.path.join(HERE, "tuning", "gfx950.json")
 
 # (threads, keys per thread, staging rounds, look-back words per round trip,
-# register bound in waves per SIMD (0: none)): the combinations built
+# register bound in waves per SIMD (0: none), bank-grouped count words): the
+# combinations built
 SORT_SHAPES = [(256, 64, 1, 4, 0, 0), (512, 24, 1, 4, 0, 0), (512, 32, 1, 4, 0, 0), (1024, 16, 1, 4, 0, 0),
                (256, 64, 2, 4, 0, 0), (512, 24, 2, 4, 0, 0), (512, 32, 2, 4, 0, 0), (512, 24, 3, 4, 0, 0),
                (256, 64, 2, 2, 0, 0), (512, 32, 2, 2, 0, 0), (256, 64, 2, 2, 4, 0), (512, 24, 2, 2, 8, 0),
