@@ -200,8 +200,8 @@ def main():
                     help="skip the segmented sort's one-call-per-segment baseline (for profiler runs)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
-        print("bench.py runs on exactly one GPU: this build has no multi-GPU path "
-              "(the sharding design is described in DESIGN.md); use --gpus 1", file=sys.stderr)
+        print("bench.py runs on exactly one GPU: the library has no multi-GPU path "
+              "(each GPU would sort or scan its own shard); use --gpus 1", file=sys.stderr)
         sys.exit(2)
     only = SECTIONS if args.only == "all" else args.only.split(",")
 

@@ -3,7 +3,8 @@ reference's own golden vectors.
 
 Bar: bit-exact for sorts, integer scans/reductions and reduce_by_key keys;
 float results within the stated bound of the double-double oracle (the
-reference's float association is not reproduced)."""
+reference's float association is not reproduced: the device sums in FP64 or
+compensated pairs, so its error bound is tighter than a sequential float sum's)."""
 import math
 
 import numpy as np
