@@ -19,8 +19,10 @@ what varies is the tile shape and the work per thread):
   __bcmi_radix_hist|unroll, |subs, |blocks_per_cu, |bank_groups  the sort's histogram kernel
   __bcmi_radix_hist_by_key|(the same params)  the same for sorts with values
   __bcmi_scan|q, |waves                4-byte scans: items per lane, waves per workgroup
-  __bcmi_scan_f32|q, |waves, |acc      the compensated float scan: items per lane, waves, accumulators
-                                       (1: FP64, 0: TwoSum FP32 pairs)
+  __bcmi_scan_f32|q, |waves, |acc, |blocked  the compensated float scan: items per lane, waves,
+                                       accumulators (1: FP64, 0: TwoSum FP32 pairs), layout (0: striped,
+                                       1: lanes scan consecutive elements, 2: the same transposed in two
+                                       half rounds with registers bounded for 4 waves per SIMD)
   __bcmi_reduce_by_key|q, |waves, |acc  reduce_by_key: items per lane, waves per workgroup, float
                                        accumulators (as above)
   __bcmi_segscan|q, |waves, |acc       segmented scans: the same
@@ -55,6 +57,8 @@ SORT_PARAMS = ("threads", "ipt", "split", "lookback", "waves", "count_bg", "xcd_
 XCD_RUNS = (4, 8)  # consecutive tiles per XCD (0, dispatch order, measured slower on every shape)
 HIST_SHAPES = [(u, nw, b, g) for u in (4, 8) for nw in (8, 16) for b in (2, 4) for g in (0, 1)]
 SCAN_SHAPES = ((4, 8), (4, 16), (8, 8), (8, 16))  # (waves per workgroup, chunks per lane)
+# the compensated float scan's layouts ("blocked" param) with the shapes each one has
+SCAN_F32_LAYOUTS = ((0, SCAN_SHAPES), (1, ((4, 8), (4, 16), (8, 8))), (2, SCAN_SHAPES))
 SEGSCAN_SHAPES = ((4, 4), (4, 8), (8, 8))  # (waves per workgroup, chunks per lane)
 
 
@@ -202,20 +206,23 @@ def tune_scan(ca, W, q, log2n, log):
     dxf = ca.DeviceArray.from_host(xf, q)
     dof = ca.DeviceArray(n, np.float32, q)
     best = None
-    for acc, (nw, qq) in [(a, b) for a in (0, 1) for b in SCAN_SHAPES]:
+    for acc, blk, (nw, qq) in [(a, k, b) for a in (0, 1) for k, shapes in SCAN_F32_LAYOUTS for b in shapes]:
         _set(ca, "__bcmi_scan_f32", "acc", acc)
+        _set(ca, "__bcmi_scan_f32", "blocked", blk)
         _set(ca, "__bcmi_scan_f32", "waves", nw)
         _set(ca, "__bcmi_scan_f32", "q", qq)
         ms = _median_ms(ca, q, lambda: ca.scan(q, dxf, dof, False))
         ok = bool(np.all(np.abs(dof.to_host(q).astype(np.float64) - dd) <= bound))
-        log({"tunable": "__bcmi_scan_f32", "acc": acc, "waves": nw, "q": qq, "ms": round(ms, 4), "ok": ok})
+        log({"tunable": "__bcmi_scan_f32", "acc": acc, "blocked": blk, "waves": nw, "q": qq,
+             "ms": round(ms, 4), "ok": ok})
         if ok and (best is None or ms < best[0]):
-            best = (ms, acc, nw, qq)
+            best = (ms, acc, blk, nw, qq)
     if best is None:
         raise RuntimeError("autotune: no compensated float scan candidate within the bound")
     _set(ca, "__bcmi_scan_f32", "acc", best[1])
-    _set(ca, "__bcmi_scan_f32", "waves", best[2])
-    _set(ca, "__bcmi_scan_f32", "q", best[3])
+    _set(ca, "__bcmi_scan_f32", "blocked", best[2])
+    _set(ca, "__bcmi_scan_f32", "waves", best[3])
+    _set(ca, "__bcmi_scan_f32", "q", best[4])
 
 
 SEGSORT_CANDIDATES = [(4, 1, 12, 0), (3, 1, 12, 0), (2, 1, 12, 0), (4, 0, 12, 0), (3, 0, 12, 0), (3, 1, 24, 0),

@@ -108,14 +108,103 @@ struct ScanLds {
     typename P::Cr prefix;
 };
 
+// Striped chunks <-> blocked rows (lane l holds the wave's positions
+// l*E .. l*E+E-1, E = Q*4) through LDS, G chunks at a time: the chunks of
+// group g hold positions g*G*256 .. (g+1)*G*256-1, which are the rows of
+// lanes g*R .. g*R+R-1 (R = G*256/E), so the wave's buffer holds R rows of
+// E words, each padded by 4 words so that the 16-byte accesses of both
+// directions spread over the banks.  G = Q is one group of 64 rows; G =
+// Q/2 halves the buffer at the price of a second round, whose row reads
+// and writes run on half the lanes.  The buffer is the wave's own: LDS
+// operations of one wave execute in order, and wave_lds_sync keeps the
+// compiler from moving them across each other.
+template <int Q, int G>
+struct RowShape {
+    static constexpr int E = Q * 4, ROW = E + 4, R = G * 256 / E;
+    static_assert(Q % G == 0 && R * E == G * 256 && 64 % R == 0, "groups of whole rows");
+    static constexpr int WORDS = R * ROW;  // per wave
+};
+
+template <int Q, int G, typename T>
+BDEV void rows_from_chunks(const T (&s)[Q][4], T (&b)[Q * 4], uint32_t *buf, uint32_t lane)
+{
+    using RS = RowShape<Q, G>;
+    constexpr int E = RS::E, ROW = RS::ROW, R = RS::R;
+#pragma unroll
+    for (int g = 0; g < Q / G; ++g) {
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int p = j * 256 + (int)lane * 4;  // first of the chunk's 4 positions within the group
+            uint4 c;
+            c.x = __builtin_bit_cast(uint32_t, s[g * G + j][0]);
+            c.y = __builtin_bit_cast(uint32_t, s[g * G + j][1]);
+            c.z = __builtin_bit_cast(uint32_t, s[g * G + j][2]);
+            c.w = __builtin_bit_cast(uint32_t, s[g * G + j][3]);
+            *(uint4 *)(buf + (p / E) * ROW + p % E) = c;
+        }
+        wave_lds_sync();
+        if (G == Q || lane / R == (uint32_t)g) {
+            const uint32_t r = lane - g * R;
+#pragma unroll
+            for (int e = 0; e < E; e += 4) {
+                const uint4 c = *(const uint4 *)(buf + r * ROW + e);
+                b[e] = __builtin_bit_cast(T, c.x);
+                b[e + 1] = __builtin_bit_cast(T, c.y);
+                b[e + 2] = __builtin_bit_cast(T, c.z);
+                b[e + 3] = __builtin_bit_cast(T, c.w);
+            }
+        }
+        wave_lds_sync();
+    }
+}
+
+template <int Q, int G, typename T>
+BDEV void chunks_from_rows(const T (&b)[Q * 4], T (&s)[Q][4], uint32_t *buf, uint32_t lane)
+{
+    using RS = RowShape<Q, G>;
+    constexpr int E = RS::E, ROW = RS::ROW, R = RS::R;
+#pragma unroll
+    for (int g = 0; g < Q / G; ++g) {
+        if (G == Q || lane / R == (uint32_t)g) {
+            const uint32_t r = lane - g * R;
+#pragma unroll
+            for (int e = 0; e < E; e += 4) {
+                uint4 c;
+                c.x = __builtin_bit_cast(uint32_t, b[e]);
+                c.y = __builtin_bit_cast(uint32_t, b[e + 1]);
+                c.z = __builtin_bit_cast(uint32_t, b[e + 2]);
+                c.w = __builtin_bit_cast(uint32_t, b[e + 3]);
+                *(uint4 *)(buf + r * ROW + e) = c;
+            }
+        }
+        wave_lds_sync();
+#pragma unroll
+        for (int j = 0; j < G; ++j) {
+            const int p = j * 256 + (int)lane * 4;
+            const uint4 c = *(const uint4 *)(buf + (p / E) * ROW + p % E);
+            s[g * G + j][0] = __builtin_bit_cast(T, c.x);
+            s[g * G + j][1] = __builtin_bit_cast(T, c.y);
+            s[g * G + j][2] = __builtin_bit_cast(T, c.z);
+            s[g * G + j][3] = __builtin_bit_cast(T, c.w);
+        }
+        wave_lds_sync();
+    }
+}
+
 // One tile: Q 16-byte chunks per thread (tile = NW waves * 64 lanes * Q
 // chunks), wave-striped: chunk q of lane l of wave w covers elements
 // w*WSEG + q*64*V + l*V .. +V-1 of the tile.  FULL: the tile lies below n, so
 // no element needs a bounds test.  VEC: input and output are 16-byte
-// aligned, so full chunks move as one 16-byte load / store.
-template <typename T, typename P, bool VEC, int Q, int NW, bool FULL, typename Src>
-BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status,
-                    typename P::Cr init, int exclusive, uint32_t epoch)
+// aligned, so full chunks move as one 16-byte load / store.  BG > 0 (4-byte
+// elements): the loaded chunks are transposed, BG chunks at a time, into
+// blocked rows through the wave's LDS buffer (blk, RowShape<Q, BG>::WORDS
+// words per wave) so that each lane scans Q*4 consecutive elements serially
+// and the wave scans one fold per lane instead of one per chunk; the
+// results go back to striped chunks the same way.  That trades Q wave
+// scans per tile for two LDS round trips.
+template <typename T, typename P, bool VEC, int Q, int NW, bool FULL, int BG, typename Src>
+BDEV void scan_body(ScanLds<P, NW> &L, uint32_t *blk, const Src src, T *out, size_t n, Ctrl *ctrl,
+                    uint64_t *status, typename P::Cr init, int exclusive, uint32_t epoch)
 {
     using S = typename P::S;
     using Cr = typename P::Cr;
@@ -131,6 +220,10 @@ BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ct
     T *wout = out + wstart;
     const int64_t rem = FULL ? (int64_t)WSEG : (int64_t)n - (int64_t)wstart;
     auto ok = [&](int q, int k) { return FULL || (int64_t)(q * 64 * V + lane * V + k) < rem; };
+    // blocked position e of this lane (BLK)
+    auto okb = [&](int e) { return FULL || (int64_t)(lane * Q * V + e) < rem; };
+    constexpr bool BLK = BG > 0;
+    uint32_t *wblk = blk + w * RowShape<Q, BLK ? BG : Q>::WORDS;
 
     T x[Q][V];
     if (src.array()) {
@@ -155,16 +248,29 @@ BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ct
     // chunk folds, then wave scans of the folds carried across q
     auto mrg = [](S a, S b) { return P::merge(a, b); };
     S carry = P::id();
-    S pre[Q];
-#pragma unroll
-    for (int q = 0; q < Q; ++q) {
+    S pre[BLK ? 1 : Q];
+    T b[BLK ? Q * V : 1];
+    if constexpr (BLK) {
+        static_assert(V == 4, "blocked tiles of 4-byte elements");
+        rows_from_chunks<Q, BG>(x, b, wblk, lane);
         S t = P::id();
 #pragma unroll
-        for (int k = 0; k < V; ++k)
-            if (ok(q, k)) t = P::add(t, x[q][k]);
+        for (int e = 0; e < Q * V; ++e)
+            if (okb(e)) t = P::add(t, b[e]);
         const S inc = wave_inclusive_scan(t, P::id(), mrg);
-        pre[q] = P::merge(carry, wave_shr1(inc, P::id()));
-        carry = P::merge(carry, readlane(inc, 63));
+        pre[0] = wave_shr1(inc, P::id());
+        carry = readlane(inc, 63);
+    } else {
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            S t = P::id();
+#pragma unroll
+            for (int k = 0; k < V; ++k)
+                if (ok(q, k)) t = P::add(t, x[q][k]);
+            const S inc = wave_inclusive_scan(t, P::id(), mrg);
+            pre[q] = P::merge(carry, wave_shr1(inc, P::id()));
+            carry = P::merge(carry, readlane(inc, 63));
+        }
     }
     if (lane == 0) L.wave[w] = carry;
     block_barrier();
@@ -180,6 +286,47 @@ BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ct
     }
     block_barrier();
     const Cr prefix = L.prefix;
+    if constexpr (BLK) {
+        // positions past n follow every valid one in blocked order, so the
+        // running sum of a valid position never includes them.  With FP64
+        // accumulators the tile's prefix is added once, at the start of the
+        // lane's run, rather than to every element.
+        constexpr bool SEED = std::is_same<P, ScanF64AccF32>::value;
+        S s = P::merge(wpre, pre[0]);
+        if constexpr (SEED) s = prefix + s;
+        T y[Q * V];
+#pragma unroll
+        for (int e = 0; e < Q * V; ++e) {
+            const S before = s;
+            T xk = b[e];
+            if constexpr (sizeof(S) > sizeof(T)) {
+                uint32_t u = __builtin_bit_cast(uint32_t, xk);
+                asm volatile("" : "+v"(u));
+                xk = __builtin_bit_cast(T, u);
+            }
+            s = P::add(s, xk);
+            if constexpr (SEED)
+                y[e] = (T)(exclusive ? before : s);
+            else
+                y[e] = P::out(prefix, exclusive ? before : s);
+        }
+        chunks_from_rows<Q, BG>(y, x, wblk, lane);
+#pragma unroll
+        for (int q = 0; q < Q; ++q) {
+            T *dst = wout + q * 64 * V + lane * V;
+            if (VEC && FULL) {
+                sc_chunk<T> c;
+#pragma unroll
+                for (int k = 0; k < V; ++k) c.v[k] = x[q][k];
+                *(sc_chunk<T> *)dst = c;
+            } else {
+#pragma unroll
+                for (int k = 0; k < V; ++k)
+                    if (ok(q, k)) dst[k] = x[q][k];
+            }
+        }
+        return;
+    }
 #pragma unroll
     for (int q = 0; q < Q; ++q) {
         S s = P::merge(wpre, pre[q]);
@@ -214,21 +361,27 @@ BDEV void scan_body(ScanLds<P, NW> &L, const Src src, T *out, size_t n, Ctrl *ct
 
 // NW waves per workgroup (4, or 8: a tile twice as big, half the tiles and
 // look-backs).
-template <typename T, typename P, bool VEC, int Q, typename Src, int NW = 4>
-__global__ __launch_bounds__(NW * 64) void k_scan(const Src src, T *out, size_t n, Ctrl *ctrl,
-                                                  uint64_t *status, typename P::Cr init, int exclusive,
-                                                  uint32_t back)
+// WPE: waves per SIMD the register allocation must allow (1: no bound).
+template <typename T, typename P, bool VEC, int Q, typename Src, int NW = 4, int BG = 0, int WPE = 1>
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(WPE))) void k_scan(
+    const Src src, T *out, size_t n, Ctrl *ctrl, uint64_t *status, typename P::Cr init, int exclusive,
+    uint32_t back)
 {
     constexpr size_t TILE = (size_t)NW * Q * 64 * (16 / sizeof(T));
     __shared__ ScanLds<P, NW> lds;
+    uint32_t *blk = nullptr;
+    if constexpr (BG > 0) {
+        __shared__ uint32_t rows[NW * RowShape<Q, BG>::WORDS];
+        blk = rows;
+    }
     const uint32_t epoch = launch_epoch_plain(ctrl, back);
     if ((size_t)(blockIdx.x + 1) * TILE <= n)
-        scan_body<T, P, VEC, Q, NW, true>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, NW, true, BG>(lds, blk, src, out, n, ctrl, status, init, exclusive, epoch);
     else
-        scan_body<T, P, VEC, Q, NW, false>(lds, src, out, n, ctrl, status, init, exclusive, epoch);
+        scan_body<T, P, VEC, Q, NW, false, BG>(lds, blk, src, out, n, ctrl, status, init, exclusive, epoch);
 }
 
-template <typename T, typename P, int Q, typename Src, int NW = 4>
+template <typename T, typename P, int Q, typename Src, int NW = 4, int BG = 0, int WPE = 1>
 static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t *status, typename P::Cr ini,
                        int exclusive)
 {
@@ -239,11 +392,11 @@ static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t 
     BCMI_TRY(begin_lookback(q, 1));
     if (vec) {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, true, Q, Src, NW>), dim3((unsigned)tiles), dim3(NW * 64), 0,
+                    hipLaunchKernelGGL((k_scan<T, P, true, Q, Src, NW, BG, WPE>), dim3((unsigned)tiles), dim3(NW * 64), 0,
                                        q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     } else {
         BCMI_LAUNCH(q, "scan",
-                    hipLaunchKernelGGL((k_scan<T, P, false, Q, Src, NW>), dim3((unsigned)tiles), dim3(NW * 64), 0,
+                    hipLaunchKernelGGL((k_scan<T, P, false, Q, Src, NW, BG, WPE>), dim3((unsigned)tiles), dim3(NW * 64), 0,
                                        q->stream, src, out, n, q->ctrl, status, ini, exclusive, 1u));
     }
     return 0;
@@ -252,7 +405,17 @@ static int launch_scan(bcmi_queue q, const Src &src, T *out, size_t n, uint64_t 
 // Chunks per thread come from the tuning cache (key "__bcmi_scan", or
 // "__bcmi_scan_f32" for the compensated float scan; param "q": 4, 8 or 16;
 // param "waves": 4, or 8 with q = 8 or 16) for 4-byte array scans; 4
-// chunks and 4 waves otherwise.
+// chunks and 4 waves otherwise.  The compensated float scan also reads
+// param "blocked": lanes scan consecutive elements (BG above); 1 = one
+// transpose round (4 waves and q 8 or 16, or 8 waves and q 8), 2 = two
+// rounds of half the chunks each, in half the LDS, with registers bounded
+// for 4 waves per SIMD (4 or 8 waves, q 8 or 16).
+template <typename P>
+constexpr bool comp_policy()
+{
+    return std::is_same<P, ScanCompF32>::value || std::is_same<P, ScanF64AccF32>::value;
+}
+
 template <typename T, typename P, typename Src>
 static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, const void *init, int exclusive)
 {
@@ -265,9 +428,25 @@ static int run_scan_src(bcmi_queue q, const Src &src, void *out_, size_t n, cons
     if constexpr (sizeof(T) == 4 && std::is_same<Src, ArraySrc<T>>::value) {
         // the compensated float scan holds a TwoSum pair per chunk, so it has
         // its own chunk count (fewer chunks: fewer VGPRs, more resident tiles)
-        const bool comp = std::is_same<P, ScanCompF32>::value || std::is_same<P, ScanF64AccF32>::value;
+        const bool comp = comp_policy<P>();
         const char *key = comp ? "__bcmi_scan_f32" : "__bcmi_scan";
         const unsigned qq = tune_get(key, "q", 16);
+        if constexpr (comp_policy<P>()) {
+            const unsigned blocked = tune_get(key, "blocked", 0);
+            const bool w8 = tune_get(key, "waves", 4) == 8;
+            if (blocked == 1) {
+                if (w8 && qq == 8) return launch_scan<T, P, 8, Src, 8, 8>(q, src, out, n, status, ini, exclusive);
+                if (qq == 16) return launch_scan<T, P, 16, Src, 4, 16>(q, src, out, n, status, ini, exclusive);
+                if (qq == 8) return launch_scan<T, P, 8, Src, 4, 8>(q, src, out, n, status, ini, exclusive);
+            }
+            if (blocked == 2) {
+                if (w8 && qq == 16)
+                    return launch_scan<T, P, 16, Src, 8, 8, 4>(q, src, out, n, status, ini, exclusive);
+                if (w8 && qq == 8) return launch_scan<T, P, 8, Src, 8, 4, 4>(q, src, out, n, status, ini, exclusive);
+                if (qq == 16) return launch_scan<T, P, 16, Src, 4, 8, 4>(q, src, out, n, status, ini, exclusive);
+                if (qq == 8) return launch_scan<T, P, 8, Src, 4, 4, 4>(q, src, out, n, status, ini, exclusive);
+            }
+        }
         if (tune_get(key, "waves", 4) == 8) {
             if (qq == 16) return launch_scan<T, P, 16, Src, 8>(q, src, out, n, status, ini, exclusive);
             if (qq == 8) return launch_scan<T, P, 8, Src, 8>(q, src, out, n, status, ini, exclusive);

@@ -746,16 +746,17 @@ def test_segmented_sort_matches_per_segment_calls(queue, dt, vdt):
 def test_scan_tunable_shapes(queue, waves, q):
     """Every (waves per workgroup, chunks per lane) shape the tuner may pick
     for the 4-byte scans: integer scans exact (exclusive with an init, in
-    place, unaligned), the compensated float scan within its bound with both
-    accumulator forms, whole and partial tiles."""
+    place, unaligned), the compensated float scan in the striped layout within
+    its bound with both accumulator forms, whole and partial tiles."""
     L = ca.lib()
     keys_t = ("__bcmi_scan", "__bcmi_scan_f32")
-    params = ("waves", "q", "acc")
+    params = ("waves", "q", "acc", "blocked")
     saved = {k: [_tune_get(k, p) for p in params] for k in keys_t}
     try:
         for k in keys_t:
             ca.check(L.bcmi_tune_set(k.encode(), b"waves", waves))
             ca.check(L.bcmi_tune_set(k.encode(), b"q", q))
+        ca.check(L.bcmi_tune_set(b"__bcmi_scan_f32", b"blocked", 0))  # the striped layout
         for n in (1, 5000, 65536 * 3 + 7, (1 << 21) + 13):
             x = RNG.integers(-1000, 1000, n).astype(np.int32)
             d = dev(queue, x)
@@ -782,8 +783,56 @@ def test_scan_tunable_shapes(queue, waves, q):
                 assert np.all(np.abs(fo.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(xf, dd)), (n, acc)
     finally:
         for k, vals_ in saved.items():
-            for p, val, d in zip(params, vals_, (4, 16, 1)):
+            for p, val, d in zip(params, vals_, (4, 16, 1, 0)):
                 ca.check(L.bcmi_tune_set(k.encode(), p.encode(), val if val != 0xFFFFFFFF else d))
+
+
+@pytest.mark.parametrize("waves,q,blocked", [(4, 8, 1), (4, 16, 1), (8, 8, 1),
+                                             (4, 8, 2), (4, 16, 2), (8, 8, 2), (8, 16, 2)])
+def test_scan_blocked_layout(queue, waves, q, blocked):
+    """The compensated float scan with lanes scanning consecutive elements
+    (tuning param "blocked": 1 = one transpose round, 2 = two rounds of half
+    the chunks), both accumulator forms: inclusive, exclusive with an init,
+    in place and unaligned, whole and partial tiles, within the bound of the
+    double-double oracle."""
+    L = ca.lib()
+    key = "__bcmi_scan_f32"
+    params = ("waves", "q", "acc", "blocked")
+    saved = [_tune_get(key, p) for p in params]
+    try:
+        ca.check(L.bcmi_tune_set(key.encode(), b"waves", waves))
+        ca.check(L.bcmi_tune_set(key.encode(), b"q", q))
+        ca.check(L.bcmi_tune_set(key.encode(), b"blocked", blocked))
+        for n in (1, 77, 5000, 65536 * 3 + 7, (1 << 21) + 13):
+            x = O.gen_lognormal_f32(n)
+            x[::3] *= -1.0  # mixed signs: cancellation inside and across tiles
+            fo = ca.DeviceArray(n, np.float32, queue)
+            for acc in (0, 1):
+                ca.check(L.bcmi_tune_set(key.encode(), b"acc", acc))
+                ca.scan(queue, dev(queue, x), fo, False)
+                dd = O.scan_f32_dd(x, False)
+                got = fo.to_host(queue).astype(np.float64)
+                assert np.all(np.abs(got - dd) <= _scan_err_bound(x, dd)), (n, acc, "inclusive")
+                ca.scan(queue, dev(queue, x), fo, True, init=np.float32(2.5))
+                dd = O.scan_f32_dd(x, True, 2.5)
+                got = fo.to_host(queue).astype(np.float64)
+                assert np.all(np.abs(got - dd) <= _scan_err_bound(x, dd) + np.spacing(np.float32(2.5))), \
+                    (n, acc, "exclusive")
+                d = dev(queue, x)
+                ca.scan(queue, d, d, False)
+                dd = O.scan_f32_dd(x, False)
+                assert np.all(np.abs(d.to_host(queue).astype(np.float64) - dd) <= _scan_err_bound(x, dd)), \
+                    (n, acc, "in place")
+                if n > 2:
+                    d = dev(queue, x)
+                    m = n - 2
+                    ca.check(L.bcmi_scan(queue.h, ca.F32, ca.PLUS, d.at(1), fo.at(2), m, None, 0, ca.COMPENSATED))
+                    dd = O.scan_f32_dd(x[1:1 + m], False)
+                    got = fo.to_host(queue)[2:].astype(np.float64)
+                    assert np.all(np.abs(got - dd) <= _scan_err_bound(x[1:1 + m], dd)), (n, acc, "unaligned")
+    finally:
+        for p, val, d in zip(params, saved, (4, 16, 1, 0)):
+            ca.check(L.bcmi_tune_set(key.encode(), p.encode(), val if val != 0xFFFFFFFF else d))
 
 
 @pytest.mark.parametrize("waves,q", [(4, 2), (4, 4), (4, 8), (8, 8)])
