@@ -289,6 +289,24 @@ struct SegLdsBlk {
     BlkBuf<Q, NW> t;
 };
 
+// a[i] for a lane-varying i < E, without scratch: a select tree on the
+// bits of i (E - 1 selects)
+template <int E, typename T>
+BDEV T pick(const T (&a)[E], uint32_t i)
+{
+    static_assert(E >= 2 && (E & (E - 1)) == 0, "a power of two");
+    T v[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) v[j] = a[j];
+#pragma unroll
+    for (int b = 0, m = E; m > 1; ++b, m /= 2) {
+        const bool hi = (i >> b) & 1u;
+#pragma unroll
+        for (int j = 0; j < m / 2; ++j) v[j] = hi ? v[2 * j + 1] : v[2 * j];
+    }
+    return v[0];
+}
+
 template <typename KT, typename VT, typename P, int MODE, int Q, int NW, bool VEC, bool FULL>
 BDEV void segscan_body_blk(SegLdsBlk<P, Q, NW> &L, const KT *__restrict__ keys, const VT *vals, VT *out,
                            KT *__restrict__ keys_out, size_t n, Ctrl *ctrl, uint64_t *status, VT init,
@@ -345,6 +363,7 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q, NW> &L, const KT *__restrict__ keys, 
     const KT kfirst = (wstart > 0 && wstart - 1 < n) ? keys[wstart - 1] : KT(0);
     KT kprev = wave_shr1(kb[E - 1], KT(0));
     if (lane == 0) kprev = kfirst;
+    static_assert(E <= 32, "one head bit per position");
     uint32_t headm = 0;
 #pragma unroll
     for (int e = 0; e < E; ++e) {
@@ -405,32 +424,36 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q, NW> &L, const KT *__restrict__ keys, 
         // run with no head in the lane, continue from before the lane.
         const VT vfirst = P::fin(prefix.v, P::merge(s0.v, tfh), s0.n == 0);
         const uint32_t nb0 = prefix.n + s0.n;  // heads before this lane
-        uint32_t hc = 0;                       // this lane's heads before e
-#pragma unroll
-        for (int e = 0; e < E; ++e) {
-            if (!ok(e)) continue;
-            const size_t i = wstart + (size_t)lane * E + e;
-            const bool head = headm >> e & 1u;
-            if (head && i > 0) {
-                const uint32_t nb = nb0 + hc;
-                if (nb >= 1 && nb <= n) {
-                    keys_out[nb - 1] = e == 0 ? kprev : kb[e - 1];
-                    out[nb - 1] = hc == 0 ? vfirst : __builtin_bit_cast(VT, row[e]);
-                } else {
-                    __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
+        // The lane walks its heads, not its positions: one key and one value
+        // store per run end (runs average tens of elements, so a walk over
+        // all E positions would issue a mostly idle store pair at each).
+        const size_t lbase = wstart + (size_t)lane * E;
+        const int64_t lrem = (int64_t)n - (int64_t)lbase;  // positions of the lane below n
+        uint32_t hm = lrem <= 0 ? 0u : (lrem >= E ? headm : headm & ((1u << lrem) - 1u));
+        if (lbase == 0) hm &= ~1u;  // element 0 ends no run
+        while (hm) {
+            const int e = __builtin_ctz(hm);
+            hm &= hm - 1u;
+            const uint32_t hc = __builtin_popcount(headm & ((1u << e) - 1u));  // this lane's heads before e
+            const uint32_t nb = nb0 + hc;
+            if (nb >= 1 && nb <= n) {
+                keys_out[nb - 1] = e == 0 ? kprev : pick(kb, (uint32_t)e - 1u);
+                out[nb - 1] = hc == 0 ? vfirst : __builtin_bit_cast(VT, row[e]);
+            } else {
+                __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
-            hc += head ? 1u : 0u;
-            if (i == n - 1) {
-                const uint32_t nb = nb0 + hc;
-                if (nb >= 1 && nb <= n) {
-                    keys_out[nb - 1] = kb[e];
-                    out[nb - 1] = hc ? P::fin_local(t.v) : P::fin(prefix.v, P::merge(s0.v, t.v), s0.n == 0);
-                } else {
-                    __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                }
-                *count_out = nb;
+        }
+        if (lrem >= 1 && lrem <= E) {  // the last run ends at element n - 1, position lrem - 1 here
+            const uint32_t e = (uint32_t)lrem - 1u;
+            const uint32_t hc = __builtin_popcount(headm & (e == 31 ? ~0u : (2u << e) - 1u));
+            const uint32_t nb = nb0 + hc;
+            if (nb >= 1 && nb <= n) {
+                keys_out[nb - 1] = pick(kb, e);
+                out[nb - 1] = hc ? P::fin_local(t.v) : P::fin(prefix.v, P::merge(s0.v, t.v), s0.n == 0);
+            } else {
+                __hip_atomic_store(&ctrl->fault, 3u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
             }
+            *count_out = nb;
         }
         return;
     }
