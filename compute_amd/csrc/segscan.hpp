@@ -491,8 +491,10 @@ BDEV void segscan_body_blk(SegLdsBlk<P, Q, NW> &L, const KT *__restrict__ keys, 
 // NW waves per workgroup: 4, or 8 for the blocked form (a tile twice as big
 // at the same elements per lane: half the tiles, so half the look-backs and
 // per-tile latencies per element; 16 waves measured slower again).
+// Eight-wave workgroups are held to 128 VGPRs (4 waves per SIMD: two
+// workgroups per CU, as their LDS allows).
 template <typename KT, typename VT, typename P, int MODE, int Q, bool VEC, int NW = 4>
-__global__ __launch_bounds__(NW * 64) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
+__global__ __launch_bounds__(NW * 64) __attribute__((amdgpu_waves_per_eu(NW == 8 ? 4 : 1))) void k_segscan(const KT *__restrict__ keys, const VT *vals, VT *out,
                                                      KT *__restrict__ keys_out, size_t n, Ctrl *ctrl,
                                                      uint64_t *status, VT init, uint64_t *count_out,
                                                      uint32_t back)
